@@ -1,0 +1,193 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: only the tests, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg use it, always as the checker.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+EQ_ADVECTION, EQ_EULER = 0, 1
+VOL_STRONG, VOL_SPLIT = 0, 1
+TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
+AVG_STD, AVG_CHANDRASEKHAR = 0, 1
+DISS_NONE, DISS_LXF, DISS_MATRIX = 0, 1, 2
+BC_DIRICHLET, BC_INFLOW, BC_OUTFLOW, BC_SLIP = 0, 1, 2, 3
+NODES_GLL, NODES_GL = 0, 1
+RK_SSPRK33, RK_CK2N54, RK_ORK256 = 0, 1, 2
+
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.oracle_create.restype = ctypes.c_void_p
+        L.oracle_create.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _dp, _dp, _ip, _dp,
+                                    ctypes.c_int, _dp, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_double]
+        L.oracle_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_sizes.argtypes = [ctypes.c_void_p, _lp]
+        L.oracle_connectivity.argtypes = [ctypes.c_void_p, _lp, _ip, _lp, _ip, _ip]
+        L.oracle_geometry.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_coords.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_set_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
+        L.oracle_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_max_dt.restype = ctypes.c_double
+        L.oracle_max_dt.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        L.oracle_integrate.restype = ctypes.c_double
+        L.oracle_integrate.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int]
+        L.oracle_rk_run.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, ctypes.c_double,
+                                    ctypes.c_int]
+        L.oracle_build_ops.argtypes = [ctypes.c_int, ctypes.c_int] + [_dp] * 9
+        L.oracle_tableau_2n.argtypes = [ctypes.c_int, _dp, _dp, _dp]
+        L.oracle_rhs_jvp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_rk_run_jvp.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp,
+                                        ctypes.c_double, ctypes.c_int]
+        L.oracle_last_error.restype = ctypes.c_char_p
+        L.oracle_last_error.argtypes = [ctypes.c_void_p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+def build_ops(n, nodes="GLL"):
+    """Operators (xi, w, D, Ds, Dsh, lm, lp, gm, gp) from the oracle's long-double build."""
+    nt = NODES_GLL if nodes == "GLL" else NODES_GL
+    out = [np.zeros(n), np.zeros(n), np.zeros((n, n)), np.zeros((n, n)), np.zeros((n, n)),
+           np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)]
+    rc = lib().oracle_build_ops(nt, n, *[_p(a) for a in out])
+    if rc != 0:
+        raise ValueError("oracle_build_ops failed")
+    return dict(zip(["xi", "w", "D", "Ds", "Dsh", "lm", "lp", "gm", "gp"], out))
+
+
+def pack_ops(ops):
+    return np.concatenate([np.ravel(np.asarray(ops[k], dtype=np.float64))
+                           for k in ["xi", "w", "D", "Ds", "Dsh", "lm", "lp", "gm", "gp"]])
+
+
+class OracleDisc:
+    """A Cartesian multi-element DGSEM discretisation held by the oracle."""
+
+    def __init__(self, nd, n, nel, start, finish, periodic, ops, eq, eqparams,
+                 volop, tpflux, savg, sdiss, intensity=1.0):
+        self.nd, self.n = nd, n
+        packed = pack_ops(ops)
+        nel_a = np.asarray(nel, dtype=np.int32)
+        st = np.asarray(start, dtype=np.float64)
+        fi = np.asarray(finish, dtype=np.float64)
+        per = np.asarray(periodic, dtype=np.int32)
+        eqp = np.asarray(eqparams, dtype=np.float64)
+        self.h = lib().oracle_create(nd, n, _p(nel_a, _ip), _p(st), _p(fi), _p(per, _ip),
+                                     _p(packed), eq, _p(eqp), volop, tpflux, savg, sdiss,
+                                     float(intensity))
+        if not self.h:
+            raise ValueError("oracle_create failed")
+        sz = np.zeros(6, dtype=np.int64)
+        lib().oracle_sizes(self.h, _p(sz, _lp))
+        self.ne, self.nf, self.npe, self.npf, self.nv, self.nbd = [int(x) for x in sz]
+        self.ndofs = self.ne * self.npe
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_destroy(self.h)
+            self.h = None
+
+    def set_option(self, which, value):
+        lib().oracle_set_option(self.h, which, value)
+
+    def connectivity(self):
+        nd = self.nd
+        e_face = np.zeros(self.ne * 2 * nd, dtype=np.int64)
+        e_side = np.zeros(self.ne * 2 * nd, dtype=np.int32)
+        f_elem = np.zeros(self.nf * 2, dtype=np.int64)
+        f_pos = np.zeros(self.nf * 2, dtype=np.int32)
+        f_bd = np.zeros(self.nf, dtype=np.int32)
+        lib().oracle_connectivity(self.h, _p(e_face, _lp), _p(e_side, _ip), _p(f_elem, _lp),
+                                  _p(f_pos, _ip), _p(f_bd, _ip))
+        return dict(e_face=e_face.reshape(self.ne, 2 * nd), e_side=e_side.reshape(self.ne, 2 * nd),
+                    f_elem=f_elem.reshape(self.nf, 2), f_pos=f_pos.reshape(self.nf, 2), f_bd=f_bd)
+
+    def geometry(self):
+        nd = self.nd
+        e_jac = np.zeros(self.ne)
+        e_metric = np.zeros(self.ne * nd * nd)
+        e_vol = np.zeros(self.ne)
+        f_frame = np.zeros(self.nf * 3 * nd)
+        f_jac = np.zeros(self.nf)
+        lib().oracle_geometry(self.h, _p(e_jac), _p(e_metric), _p(e_vol), _p(f_frame), _p(f_jac))
+        return dict(e_jac=e_jac, e_metric=e_metric.reshape(self.ne, nd, nd).transpose(0, 2, 1),
+                    e_vol=e_vol, f_frame=f_frame.reshape(self.nf, 3, nd), f_jac=f_jac)
+
+    def coords(self):
+        c = np.zeros(self.ndofs * self.nd)
+        fc = np.zeros(self.nf * self.npf * self.nd)
+        lib().oracle_coords(self.h, _p(c), _p(fc))
+        return c.reshape(self.ndofs, self.nd), fc.reshape(self.nf, self.npf, self.nd)
+
+    def set_bc(self, ib, bctype, data=None):
+        if data is None:
+            data = np.zeros(1)
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        rc = lib().oracle_set_bc(self.h, ib, bctype, _p(data))
+        if rc != 0:
+            raise ValueError("bad boundary index")
+
+    def _check(self, rc):
+        if rc != 0:
+            raise ArithmeticError(lib().oracle_last_error(self.h).decode())
+
+    def rhs(self, Q):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        dQ = np.zeros_like(Q, order="F")
+        self._check(lib().oracle_rhs(self.h, _p(Q), _p(dQ)))
+        return dQ
+
+    def rhs_jvp(self, Q, dQdir):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        dr = np.asfortranarray(dQdir, dtype=np.float64)
+        out = np.zeros_like(Q, order="F")
+        dout = np.zeros_like(Q, order="F")
+        self._check(lib().oracle_rhs_jvp(self.h, _p(Q), _p(dr), _p(out), _p(dout)))
+        return out, dout
+
+    def max_dt(self, Q, cfl):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        return lib().oracle_max_dt(self.h, _p(Q), float(cfl))
+
+    def integrate(self, Q, ivar=0):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        return lib().oracle_integrate(self.h, _p(Q), int(ivar))
+
+    def rk_run(self, scheme, Q, dt, nsteps):
+        u = np.array(Q, dtype=np.float64, order="F", copy=True)
+        self._check(lib().oracle_rk_run(self.h, scheme, _p(u), float(dt), int(nsteps)))
+        return u
+
+    def rk_run_jvp(self, scheme, Q, dQ, dt, nsteps):
+        u = np.array(Q, dtype=np.float64, order="F", copy=True)
+        du = np.array(dQ, dtype=np.float64, order="F", copy=True)
+        self._check(lib().oracle_rk_run_jvp(self.h, scheme, _p(u), _p(du), float(dt), int(nsteps)))
+        return u, du
+
+
+def tableau_2n(scheme):
+    A, B, c = np.zeros(5), np.zeros(5), np.zeros(5)
+    lib().oracle_tableau_2n(scheme, _p(A), _p(B), _p(c))
+    return A, B, c
