@@ -1,0 +1,229 @@
+"""Headline benchmark: DOF-updates/s per RK stage of Flou's DGSEM rhs + RK stage on one
+MI355X, FP32 (chosen form) vs FP64 (plain form).
+
+Workload (default): 3-D compressible Euler, periodic 64^3 hexahedra on [0, 2π]^3, N = 5
+(216 nodes per element, 56,623,104 nodes x 5 conserved variables), Taylor-Green initial
+state (M0 = 0.1) plus seeded N(0, 1e-3^2) noise on every primitive variable,
+SplitDivOperator(ChandrasekharAverage(), LxF(StdAverage(), 1.0)), CarpenterKennedy2N54 with
+a fixed dt (CFL 0.3 of the initial state).  One "step" = one CK2N54 time step = 5 fused
+rhs+stage kernels.  value = nodes x stages / time for the FP32 form; the FP64 plain form is
+timed on the same steps and reported next to it, with the density L2 error of FP32 against
+FP64 after the same number of steps.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-fp64] [--no-cpu]
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "flou.jl_amd"))
+
+GAMMA = 1.4
+M0 = 0.1
+NEL = 64
+N_NODES = 6  # N = 5
+
+
+def build_disc(precision, nel=NEL, n=N_NODES):
+    import numpy as np  # noqa: F401
+
+    import flouhip as F
+    b = F.LagrangeBasis("GLL", n)
+    std = F.StdHex(b, F.DGSEMrec(b), 5)
+    L = 2 * math.pi
+    mesh = F.CartesianMesh(3, (0, 0, 0), (L, L, L), (nel, nel, nel))
+    mesh.apply_periodicBCs(("1", "2"), ("3", "4"), ("5", "6"))
+    eq = F.EulerEquation(3, GAMMA)
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    return F.MultielementDisc(mesh, std, eq, op, precision=precision), eq, std
+
+
+def taylor_green_state(disc, std, seed=1234):
+    """Device FP64 state (5, ndofs) of the Taylor-Green vortex plus seeded noise."""
+    import torch
+    nel, n = disc.mesh.nelements_dir[0], std.n
+    L = 2 * math.pi
+    dx = L / nel
+    xi = torch.tensor(std.basis.ξ, dtype=torch.float64, device="cuda")
+    e = torch.arange(disc.nelements, device="cuda")
+    ei, ej, ek = e % nel, (e // nel) % nel, e // (nel * nel)
+    a = torch.arange(n ** 3, device="cuda")
+    ia, ib, ic = a % n, (a // n) % n, a // (n * n)
+    x = ((ei[:, None] + (xi[ia][None, :] + 1) / 2) * dx).reshape(-1)
+    y = ((ej[:, None] + (xi[ib][None, :] + 1) / 2) * dx).reshape(-1)
+    z = ((ek[:, None] + (xi[ic][None, :] + 1) / 2) * dx).reshape(-1)
+    rho = torch.ones_like(x)
+    u = M0 * torch.sin(x) * torch.cos(y) * torch.cos(z)
+    v = -M0 * torch.cos(x) * torch.sin(y) * torch.cos(z)
+    w = torch.zeros_like(x)
+    p = 1 / GAMMA + (rho * M0 ** 2 / 16) * (torch.cos(2 * x) + torch.cos(2 * y)) * (torch.cos(2 * z) + 2)
+    g = torch.Generator(device="cuda")
+    g.manual_seed(seed)
+    prim = torch.stack([rho, u, v, w, p])
+    prim += 1e-3 * torch.randn(prim.shape, generator=g, device="cuda", dtype=torch.float64)
+    Q = torch.empty_like(prim)
+    Q[0] = prim[0]
+    Q[1:4] = prim[0] * prim[1:4]
+    Q[4] = prim[4] / (GAMMA - 1) + prim[0] * (prim[1] ** 2 + prim[2] ** 2 + prim[3] ** 2) / 2
+    return Q.contiguous()
+
+
+def run_steps(disc, Q, dt, nsteps, bufs):
+    """nsteps CK2N54 steps with ping-pong state buffers; returns the buffer holding the state."""
+    import flouhip as F
+    cur, oth, reg = Q, bufs[0], bufs[1]
+    for _ in range(nsteps):
+        for s in range(5):
+            F.rk_stage(disc, F.CarpenterKennedy2N54.code, s, dt, cur, oth, reg=reg)
+            cur, oth = oth, cur
+    return cur
+
+
+def time_form(precision, Q64, dt, steps, warmup):
+    import torch
+    disc, eq, std = build_disc(precision)
+    Q = Q64.to(disc.torch_dtype()).contiguous()
+    bufs = [disc.new_state(), disc.new_state()]
+    cur = run_steps(disc, Q, dt, warmup, bufs)
+    if cur is not Q:
+        Q.copy_(cur)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    tic = time.perf_counter()
+    t0.record(stream)
+    cur = run_steps(disc, Q, dt, steps, bufs)
+    t1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - tic
+    disc.check()
+    gpu_ms = t0.elapsed_time(t1)
+    rho = cur[0].to(torch.float64).clone()
+    ndofs = disc.ndofs
+    del bufs, Q, cur, disc
+    torch.cuda.empty_cache()
+    return dict(wall_s=wall, gpu_ms=gpu_ms, ndofs=ndofs, rho=rho)
+
+
+def cpu_baseline(nel=24, steps=2):
+    """Oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on a 24^3 sample of the same
+    workload, CK2N54 steps, DOF-updates/s per stage."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    d = O.OracleDisc(3, N_NODES, [nel] * 3, [0.0] * 3, [2 * math.pi] * 3, [1, 1, 1],
+                     O.build_ops(N_NODES, "GLL"), O.EQ_EULER, [GAMMA], O.VOL_SPLIT,
+                     O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+    X, _ = d.coords()
+    rng = np.random.default_rng(1234)
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    prim = np.stack([np.ones_like(x), M0 * np.sin(x) * np.cos(y) * np.cos(z),
+                     -M0 * np.cos(x) * np.sin(y) * np.cos(z), np.zeros_like(x),
+                     1 / GAMMA + (M0 ** 2 / 16) * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2)], 1)
+    prim += 1e-3 * rng.standard_normal(prim.shape)
+    Q = np.zeros_like(prim)
+    Q[:, 0] = prim[:, 0]
+    Q[:, 1:4] = prim[:, :1] * prim[:, 1:4]
+    Q[:, 4] = prim[:, 4] / (GAMMA - 1) + prim[:, 0] * (prim[:, 1:4] ** 2).sum(1) / 2
+    Q = np.asfortranarray(Q)
+    dt = d.max_dt(Q, 0.3)
+    tic = time.perf_counter()
+    d.rk_run(O.RK_CK2N54, Q, dt, steps)
+    t = time.perf_counter() - tic
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": d.ndofs * 5 * steps / t, "unit": "DOF-updates/s", "cores": cores,
+            "kind": "port",
+            "sample": f"oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on 24^3 elements, N=5 "
+                      f"({d.ndofs} nodes), {steps} CK2N54 steps = {5 * steps} stages, {t:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
+                    help="time one form only (profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import flouhip as F
+
+    torch.cuda.set_device(0)
+    # initial state and dt (FP64)
+    disc64, eq, std = build_disc("fp64")
+    Q64 = taylor_green_state(disc64, std)
+    dt = F.get_max_dt(Q64, disc64, eq, 0.3)
+    nn = disc64.ndofs
+    del disc64
+    torch.cuda.empty_cache()
+
+    res = {}
+    forms = ["fp32", "fp64"]
+    if args.no_fp64:
+        forms = ["fp32"]
+    if args.only:
+        forms = [args.only]
+    for form in forms:
+        res[form] = time_form(form, Q64, dt, args.steps, args.warmup)
+        print(f"# {form}: {res[form]['gpu_ms'] / args.steps:.3f} ms/step "
+              f"({res[form]['gpu_ms'] / (5 * args.steps):.3f} ms/stage)", file=sys.stderr, flush=True)
+
+    main_form = "fp32" if "fp32" in res else forms[0]
+    r = res[main_form]
+    stages = 5 * args.steps
+    ms_step = r["gpu_ms"] / args.steps
+    value = nn * stages / (r["gpu_ms"] / 1e3)
+    s_bytes = 4 if main_form == "fp32" else 8
+    bytes_per_launch = 4 * 5 * s_bytes * nn  # read u_in, read/write register, write u_out
+    launch_ms = r["gpu_ms"] / stages
+    achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+    out = {
+        "metric": "DOF-updates/s per RK stage, 3D Euler N=5, FP32 vs FP64 GPU; adjoint/forward ratio",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if main_form == "fp32" else "f64",
+        "data": "synthetic (seeded Taylor-Green + noise)",
+        "config": {"workload": "3-D Euler, periodic 64^3 hex, N=5 (56,623,104 nodes), split "
+                               "ChandrasekharAverage volume flux + LxF(StdAverage) surface, "
+                               "CarpenterKennedy2N54 fused stages, fixed dt (CFL 0.3)",
+                   "form": main_form, "nodes": nn, "stages_per_step": 5, "dt": dt},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "kernel": "k_plain<float,3,6,EULER,SPLIT,CHANDRASEKHAR> (fused rhs + 2N stage)",
+                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms},
+    }
+    if "fp64" in res and "fp32" in res:
+        import torch as _t
+        r64 = res["fp64"]
+        out["fp64"] = {"value": nn * stages / (r64["gpu_ms"] / 1e3), "ms_per_step": r64["gpu_ms"] / args.steps}
+        out["fp32_speedup_vs_fp64"] = r64["gpu_ms"] / r["gpu_ms"]
+        d = res["fp32"]["rho"] - r64["rho"]
+        out["density_rel_l2_err_fp32_vs_fp64"] = float(_t.linalg.norm(d) / _t.linalg.norm(r64["rho"]))
+        out["density_err_after_steps"] = args.steps + args.warmup
+    out["adjoint_forward_ratio"] = None
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

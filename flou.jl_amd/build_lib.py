@@ -1,0 +1,81 @@
+"""Build libflouhip.so (gfx950) from flou.jl_amd/csrc with hipcc.
+
+Usage: python flou.jl_amd/build_lib.py [-j JOBS]
+
+Every translation unit is compiled for gfx950 only.  The FP64 plain kernels and the
+reductions are built with -ffp-contract=off so that no a*b+c is fused where the reference
+rounds twice; packed FP32 instructions are disabled for all device code.
+"""
+import argparse
+import concurrent.futures as cf
+import os
+import subprocess
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libflouhip.so")
+
+ARCH = "gfx950"
+BASE = ["hipcc", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-Wno-unused-command-line-argument"]
+EXACT = ["-ffp-contract=off"]
+
+
+def units():
+    u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT)]
+    for prec in ("F64", "F32"):
+        for nd in (1, 2, 3):
+            u.append((f"rhs_plain_{prec.lower()}_{nd}d.o", "rhs_plain.hip",
+                      EXACT + [f"-DFLOUHIP_PREC_{prec}", f"-DFLOUHIP_ND={nd}"]))
+    return u
+
+
+def _compile(obj, src, flags):
+    out = os.path.join(BUILD, obj)
+    cmd = BASE + flags + ["-c", os.path.join(CSRC, src), "-o", out]
+    t = time.time()
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    msgs = "\n".join(line for line in (p.stdout + p.stderr).splitlines()
+                     if "packed-fp32-ops" not in line and line.strip())
+    return obj, p.returncode, msgs, time.time() - t
+
+
+def build(jobs=None, verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    jobs = jobs or min(8, os.cpu_count() or 4)
+    us = units()
+    objs = []
+    failed = []
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        for obj, rc, msgs, dt in ex.map(lambda a: _compile(*a), us):
+            objs.append(os.path.join(BUILD, obj))
+            if verbose:
+                print(f"  [{dt:5.1f}s] {obj}" + ("" if rc == 0 else "  FAILED"), flush=True)
+            if rc != 0:
+                failed.append((obj, msgs))
+            elif msgs and verbose:
+                print(msgs)
+    if failed:
+        for obj, msgs in failed:
+            print(f"--- {obj}\n{msgs}", file=sys.stderr)
+        raise RuntimeError("libflouhip build failed")
+    tmp = LIB + ".tmp"
+    cmd = ["hipcc", f"--offload-arch={ARCH}", "-shared", "-fPIC"] + objs + ["-o", tmp]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("link failed:\n" + p.stdout + p.stderr)
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"  built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-j", type=int, default=None)
+    a = ap.parse_args()
+    build(a.j)

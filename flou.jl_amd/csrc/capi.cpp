@@ -1,0 +1,633 @@
+// C ABI of libflouhip.so (declared in include/flouhip.h): validation of the flattened
+// MultielementDisc, upload of operators / connectivity / geometry, per-element face
+// tables, kernel dispatch and the fixed-dt time loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flouhip.h"
+#include "common.hpp"
+#include "reduce.hpp"
+
+using namespace flouhip;
+
+struct flou_hip_handle {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int nd = 0, n = 0, nv = 0, prec = 0, npe = 0, npf = 0, nbd = 0;
+    int64_t ne = 0, nf = 0, ndofs = 0;
+    int eq = 0, vol = 0, tp = 0, savg = 0, sdiss = 0, typo = 0;
+    double gamma = 1.4, intensity = 1.0, a[3] = {0, 0, 0}, anorm = 0.0;
+    int gather = 0;
+    int egeom_stride = 0;
+    // device tables
+    void* d_ops = nullptr;
+    int* d_nb = nullptr;
+    int* d_finfo = nullptr;
+    int* d_fgi = nullptr;
+    int* d_fid = nullptr;
+    void* d_egeom = nullptr;
+    void* d_fgeom = nullptr;
+    double* d_dxe = nullptr;
+    double* d_jac = nullptr;
+    double* d_w = nullptr;
+    void* d_bcstate = nullptr;
+    void* d_bcconst = nullptr;
+    int* d_bctype = nullptr;
+    int* d_err = nullptr;
+    double* d_partial = nullptr;
+    double* d_out = nullptr;
+    double* h_out = nullptr;  // pinned
+    // host mirrors of the boundary data
+    std::vector<int> bctype;
+    std::vector<double> bcstate, bcconst;
+    // time-loop work buffers
+    void* d_work[3] = {nullptr, nullptr, nullptr};
+    // kernels
+    RhsLauncherF64 l64 = nullptr;
+    RhsLauncherF32 l32 = nullptr;
+    std::string err;
+};
+
+namespace {
+
+int fail(flou_hip_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+#define HIPCHECK(h, call)                                                                   \
+    do {                                                                                    \
+        hipError_t e_ = (call);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(h, e_ == hipErrorOutOfMemory ? FLOU_HIP_ENOMEM : FLOU_HIP_EDEVICE,  \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                 \
+    } while (0)
+
+template <typename T>
+int upload(flou_hip_handle* h, void** dptr, const std::vector<T>& v) {
+    if (v.empty()) return 0;
+    HIPCHECK(h, hipMalloc(dptr, v.size() * sizeof(T)));
+    HIPCHECK(h, hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// Upload a double vector converted to the handle's precision.
+int upload_real(flou_hip_handle* h, void** dptr, const std::vector<double>& v) {
+    if (h->prec == FLOU_HIP_FP64) return upload(h, dptr, v);
+    std::vector<float> f(v.begin(), v.end());
+    return upload(h, dptr, f);
+}
+
+void free_all(flou_hip_handle* h) {
+    void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
+                    h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
+                    h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2]};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->h_out) (void)hipHostFree(h->h_out);
+}
+
+int bytes_of(const flou_hip_handle* h) { return h->prec == FLOU_HIP_FP64 ? 8 : 4; }
+
+// 2N tableaux (Carpenter & Kennedy, NASA TM-109112 (1994); Allampalli et al., JCP 228 (2009)
+// as tabulated by OrdinaryDiffEq 6.49.1).
+bool tableau(int scheme, int stage, double& A, double& B) {
+    static const double ckA[5] = {0.0, -567301805773.0 / 1357537059087.0,
+                                  -2404267990393.0 / 2016746695238.0,
+                                  -3550918686646.0 / 2091501179385.0,
+                                  -1275806237668.0 / 842570457699.0};
+    static const double ckB[5] = {1432997174477.0 / 9575080441755.0,
+                                  5161836677717.0 / 13612068292357.0,
+                                  1720146321549.0 / 2090206949498.0,
+                                  3134564353537.0 / 4481467310338.0,
+                                  2277821191437.0 / 14882151754819.0};
+    static const double orkA[5] = {0.0, -1.0, -1.55798, -1.0, -0.45031};
+    static const double orkB[5] = {0.2, 0.83204, 0.6, 0.35394, 0.2};
+    if (stage < 0 || stage > 4) return false;
+    if (scheme == FLOU_HIP_CK2N54) { A = ckA[stage]; B = ckB[stage]; return true; }
+    if (scheme == FLOU_HIP_ORK256) { A = orkA[stage]; B = orkB[stage]; return true; }
+    return false;
+}
+
+std::string& create_error() {
+    static thread_local std::string msg;
+    return msg;
+}
+
+template <typename T>
+RhsArgs<T> make_args(flou_hip_handle* h) {
+    RhsArgs<T> a;
+    std::memset(&a, 0, sizeof(a));
+    a.ndofs = h->ndofs;
+    a.ne = (int)h->ne;
+    a.nb = h->d_nb;
+    a.finfo = h->d_finfo;
+    a.fgi = h->d_fgi;
+    a.fid = h->d_fid;
+    a.egeom = (const T*)h->d_egeom;
+    a.egeom_stride = h->egeom_stride;
+    a.fgeom = (const T*)h->d_fgeom;
+    a.ops = (const T*)h->d_ops;
+    a.bcstate = (const T*)h->d_bcstate;
+    a.bcconst = (const T*)h->d_bcconst;
+    a.bctype = h->d_bctype;
+    a.gather = h->gather;
+    a.gamma = (T)h->gamma;
+    a.intensity = (T)h->intensity;
+    a.a0 = (T)h->a[0];
+    a.a1 = (T)h->a[1];
+    a.a2 = (T)h->a[2];
+    a.savg = h->savg;
+    a.sdiss = h->sdiss;
+    a.typo = h->typo;
+    a.errflag = h->d_err;
+    return a;
+}
+
+int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, double B,
+                 const void* u_in, const void* ubase, void* reg, void* out) {
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64) {
+        RhsArgs<double> a = make_args<double>(h);
+        a.Q = (const double*)u_in;
+        a.out = (double*)out;
+        a.ubase = (const double*)ubase;
+        a.reg = (double*)reg;
+        a.dt = dt; a.A = A; a.B = B;
+        a.mode = mode; a.first = first;
+        e = h->l64(a, h->stream);
+    } else {
+        RhsArgs<float> a = make_args<float>(h);
+        a.Q = (const float*)u_in;
+        a.out = (float*)out;
+        a.ubase = (const float*)ubase;
+        a.reg = (float*)reg;
+        a.dt = (float)dt; a.A = (float)A; a.B = (float)B;
+        a.mode = mode; a.first = first;
+        e = h->l32(a, h->stream);
+    }
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, std::string("kernel launch: ") + hipGetErrorString(e));
+    return FLOU_HIP_OK;
+}
+
+bool overlaps(const void* a, const void* b, size_t bytes) {
+    const char* x = (const char*)a;
+    const char* y = (const char*)b;
+    return x < y + bytes && y < x + bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* flou_hip_version(void) { return "flouhip 0.1.0 (gfx950)"; }
+
+const char* flou_hip_last_error(const flou_hip_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+const char* flou_hip_create_error(void) { return create_error().c_str(); }
+
+int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
+    if (!d || !out) return FLOU_HIP_EINVAL;
+    *out = nullptr;
+    flou_hip_handle* h = new flou_hip_handle();
+    auto bad = [&](int code, const std::string& msg) {
+        // the handle is not returned: keep the message for flou_hip_create_error()
+        create_error() = msg;
+        free_all(h);
+        delete h;
+        return code;
+    };
+    create_error().clear();
+    if (d->nd < 1 || d->nd > 3) return bad(FLOU_HIP_EINVAL, "nd must be 1, 2 or 3");
+    if (d->n < 2 || d->n > 8) return bad(FLOU_HIP_EUNSUPPORTED, "n (nodes per direction) must be in 2..8");
+    if (d->precision != FLOU_HIP_FP64 && d->precision != FLOU_HIP_FP32)
+        return bad(FLOU_HIP_EINVAL, "unknown precision");
+    if (d->equation == FLOU_HIP_EULER) {
+        if (d->nvars != d->nd + 2) return bad(FLOU_HIP_EINVAL, "Euler needs nvars = nd + 2");
+        if (!(d->gamma > 1.0)) return bad(FLOU_HIP_EINVAL, "gamma must be > 1");
+    } else if (d->equation == FLOU_HIP_ADVECTION) {
+        if (d->nvars != 1) return bad(FLOU_HIP_EINVAL, "advection needs nvars = 1");
+    } else {
+        return bad(FLOU_HIP_EINVAL, "unknown equation");
+    }
+    if (d->volume_op != FLOU_HIP_STRONG && d->volume_op != FLOU_HIP_SPLIT)
+        return bad(FLOU_HIP_EINVAL, "unknown volume operator");
+    if (d->tp_flux < 0 || d->tp_flux > 2) return bad(FLOU_HIP_EINVAL, "unknown two-point flux");
+    if (d->surface_avg < 0 || d->surface_avg > 1 || d->surface_diss < 0 || d->surface_diss > 2)
+        return bad(FLOU_HIP_EINVAL, "unknown surface flux");
+    if (d->equation == FLOU_HIP_ADVECTION) {
+        if (d->volume_op == FLOU_HIP_SPLIT && d->tp_flux != FLOU_HIP_TP_STDAVG)
+            return bad(FLOU_HIP_EINVAL, "advection has only the StdAverage two-point flux");
+        if (d->surface_avg != FLOU_HIP_AVG_STD || d->surface_diss == FLOU_HIP_DISS_MATRIX)
+            return bad(FLOU_HIP_EINVAL, "advection has only StdAverage and LxF surface fluxes");
+    }
+    if (d->nelements < 1 || d->nelements > (int64_t)1 << 30) return bad(FLOU_HIP_EINVAL, "bad nelements");
+    if (d->nfaces < 1 || d->nfaces > (int64_t)1 << 31) return bad(FLOU_HIP_EINVAL, "bad nfaces");
+    if (d->nboundaries < 0) return bad(FLOU_HIP_EINVAL, "bad nboundaries");
+    if (d->geometry == FLOU_HIP_GEOM_CURVED)
+        return bad(FLOU_HIP_EUNSUPPORTED, "per-node (curved) geometry is not implemented on the GPU path");
+    if (d->geometry != FLOU_HIP_GEOM_AFFINE) return bad(FLOU_HIP_EINVAL, "unknown geometry kind");
+    if (!d->D || !d->Ds || !d->Dsharp || !d->l_minus || !d->l_plus || !d->dg_minus || !d->dg_plus || !d->omega)
+        return bad(FLOU_HIP_EINVAL, "missing operator arrays");
+    if (!d->elem_faceinds || !d->elem_facepos || !d->face_eleminds || !d->face_elempos)
+        return bad(FLOU_HIP_EINVAL, "missing connectivity arrays");
+    if (!d->elem_jac || !d->elem_metric || !d->elem_volume || !d->face_frames || !d->face_jac)
+        return bad(FLOU_HIP_EINVAL, "missing geometry arrays");
+
+    const int nd = d->nd, n = d->n, NF = 2 * nd;
+    h->nd = nd; h->n = n; h->nv = d->nvars; h->prec = d->precision;
+    h->ne = d->nelements; h->nf = d->nfaces; h->nbd = d->nboundaries;
+    h->npe = 1; h->npf = 1;
+    for (int k = 0; k < nd; ++k) h->npe *= n;
+    for (int k = 0; k < nd - 1; ++k) h->npf *= n;
+    h->ndofs = h->ne * h->npe;
+    h->eq = d->equation; h->vol = d->volume_op; h->tp = d->tp_flux;
+    h->savg = d->surface_avg; h->sdiss = d->surface_diss;
+    h->gamma = d->gamma; h->intensity = d->intensity;
+    for (int k = 0; k < 3; ++k) h->a[k] = d->equation == FLOU_HIP_ADVECTION ? d->velocity[k] : 0.0;
+    {
+        double a2 = h->a[0] * h->a[0];
+        for (int k = 1; k < nd; ++k) a2 = a2 + h->a[k] * h->a[k];
+        h->anorm = std::sqrt(a2);
+    }
+    h->typo = 0;
+
+    // connectivity checks and per element-face tables
+    const int64_t ne = h->ne, nf = h->nf;
+    if (d->face_orientation)
+        for (int64_t f = 0; f < nf; ++f)
+            if (d->face_orientation[f] != 0)
+                return bad(FLOU_HIP_EUNSUPPORTED, "face orientations other than 0 are not implemented");
+    std::vector<int> nb(ne * NF), finfo(ne * NF), fgi(ne * NF), fid(ne * NF);
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < NF; ++k) {
+            const int64_t f = d->elem_faceinds[e * NF + k];
+            const int s = d->elem_facepos[e * NF + k];
+            if (f < 0 || f >= nf || (s != 0 && s != 1)) return bad(FLOU_HIP_EINVAL, "bad elem_faceinds/elem_facepos");
+            if (d->face_eleminds[f * 2 + s] != e || d->face_elempos[f * 2 + s] != k)
+                return bad(FLOU_HIP_EINVAL, "element and face connectivity disagree");
+            const int64_t other = d->face_eleminds[f * 2 + 1 - s];
+            int info = s;
+            if (other >= 0) {
+                if (other >= ne) return bad(FLOU_HIP_EINVAL, "bad face_eleminds");
+                const int kp = d->face_elempos[f * 2 + 1 - s];
+                if (kp < 0 || kp >= NF) return bad(FLOU_HIP_EINVAL, "bad face_elempos");
+                info |= kp << 1;
+            } else {
+                if (s != 0) return bad(FLOU_HIP_EINVAL, "boundary faces must have the element on side 1");
+                const int bd = d->face_boundary ? d->face_boundary[f] : -1;
+                if (bd < 0 || bd >= h->nbd) return bad(FLOU_HIP_EINVAL, "boundary face without a valid boundary index");
+                info |= (bd + 1) << 4;
+            }
+            nb[e * NF + k] = (int)other;
+            finfo[e * NF + k] = info;
+            fid[e * NF + k] = (int)f;
+        }
+
+    // operators
+    std::vector<double> ops(ops_size(n));
+    std::memcpy(&ops[ops_D(n)], d->D, n * n * sizeof(double));
+    std::memcpy(&ops[ops_Ds(n)], d->Ds, n * n * sizeof(double));
+    std::memcpy(&ops[ops_Dsh(n)], d->Dsharp, n * n * sizeof(double));
+    std::memcpy(&ops[ops_lm(n)], d->l_minus, n * sizeof(double));
+    std::memcpy(&ops[ops_lp(n)], d->l_plus, n * sizeof(double));
+    std::memcpy(&ops[ops_gm(n)], d->dg_minus, n * sizeof(double));
+    std::memcpy(&ops[ops_gp(n)], d->dg_plus, n * sizeof(double));
+    std::memcpy(&ops[ops_w(n)], d->omega, n * sizeof(double));
+    h->gather = 1;
+    for (int j = 0; j < n; ++j) {
+        if (d->l_minus[j] != (j == 0 ? 1.0 : 0.0)) h->gather = 0;
+        if (d->l_plus[j] != (j == n - 1 ? 1.0 : 0.0)) h->gather = 0;
+    }
+
+    // geometry: detect the uniform (all elements alike) case
+    const int NG = nd * nd + 1;
+    bool uniform = true;
+    for (int64_t e = 1; e < ne && uniform; ++e) {
+        if (d->elem_jac[e] != d->elem_jac[0]) uniform = false;
+        for (int c = 0; c < nd * nd; ++c)
+            if (d->elem_metric[e * nd * nd + c] != d->elem_metric[c]) uniform = false;
+    }
+    std::vector<double> eg;
+    if (uniform) {
+        eg.resize(NG);
+        for (int c = 0; c < nd * nd; ++c) eg[c] = d->elem_metric[c];
+        eg[nd * nd] = d->elem_jac[0];
+        h->egeom_stride = 0;
+    } else {
+        eg.resize(ne * NG);
+        for (int64_t e = 0; e < ne; ++e) {
+            for (int c = 0; c < nd * nd; ++c) eg[e * NG + c] = d->elem_metric[e * nd * nd + c];
+            eg[e * NG + nd * nd] = d->elem_jac[e];
+        }
+        h->egeom_stride = NG;
+    }
+    for (int64_t e = 0; e < ne; ++e)
+        if (!(d->elem_jac[e] > 0.0)) return bad(FLOU_HIP_EINVAL, "element Jacobians must be positive");
+    const int FG = 3 * nd + 1;
+    // faces: frames + jac per master position when every face with that position agrees
+    std::vector<int> rep(NF, -1);
+    bool funiform = true;
+    for (int64_t f = 0; f < nf && funiform; ++f) {
+        const int pos = d->face_elempos[f * 2];
+        if (pos < 0 || pos >= NF) return bad(FLOU_HIP_EINVAL, "bad master face position");
+        if (rep[pos] < 0) { rep[pos] = (int)f; continue; }
+        const int64_t r = rep[pos];
+        if (d->face_jac[f] != d->face_jac[r]) funiform = false;
+        for (int c = 0; c < 3 * nd; ++c)
+            if (d->face_frames[f * 3 * nd + c] != d->face_frames[r * 3 * nd + c]) funiform = false;
+    }
+    std::vector<double> fg;
+    if (funiform) {
+        fg.assign(NF * FG, 0.0);
+        for (int p = 0; p < NF; ++p) {
+            if (rep[p] < 0) continue;
+            for (int c = 0; c < 3 * nd; ++c) fg[p * FG + c] = d->face_frames[(int64_t)rep[p] * 3 * nd + c];
+            fg[p * FG + 3 * nd] = d->face_jac[rep[p]];
+        }
+    } else {
+        fg.resize(nf * FG);
+        for (int64_t f = 0; f < nf; ++f) {
+            for (int c = 0; c < 3 * nd; ++c) fg[f * FG + c] = d->face_frames[f * 3 * nd + c];
+            fg[f * FG + 3 * nd] = d->face_jac[f];
+        }
+    }
+    for (int64_t e = 0; e < ne; ++e)
+        for (int k = 0; k < NF; ++k) {
+            const int64_t f = fid[e * NF + k];
+            fgi[e * NF + k] = funiform ? d->face_elempos[f * 2] : (int)f;
+        }
+
+    // reductions: Δx per element and quadrature weights (FP64)
+    std::vector<double> dxe(ne), jac(ne), w(h->npe);
+    for (int64_t e = 0; e < ne; ++e) {
+        double dx = d->elem_volume[e] / h->npe;
+        dxe[e] = nd == 1 ? dx : (nd == 2 ? std::sqrt(dx) : std::cbrt(dx));
+        jac[e] = d->elem_jac[e];
+    }
+    for (int i = 0; i < h->npe; ++i) {
+        int idx = i;
+        double wi[3];
+        for (int k = 0; k < nd; ++k) { wi[k] = d->omega[idx % n]; idx /= n; }
+        double x = wi[0];
+        for (int k = 1; k < nd; ++k) x = x * wi[k];
+        w[i] = x;
+    }
+
+    // kernels
+    if (h->prec == FLOU_HIP_FP64) {
+        h->l64 = nd == 1 ? plain_f64_1d(n, h->eq, h->vol, h->tp)
+                         : (nd == 2 ? plain_f64_2d(n, h->eq, h->vol, h->tp) : plain_f64_3d(n, h->eq, h->vol, h->tp));
+        if (!h->l64) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
+    } else {
+        h->l32 = nd == 1 ? plain_f32_1d(n, h->eq, h->vol, h->tp)
+                         : (nd == 2 ? plain_f32_2d(n, h->eq, h->vol, h->tp) : plain_f32_3d(n, h->eq, h->vol, h->tp));
+        if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
+    }
+
+    // device uploads
+    if (hipGetDevice(&h->device) != hipSuccess) return bad(FLOU_HIP_EDEVICE, "no HIP device");
+    int rc = 0;
+    rc |= upload_real(h, &h->d_ops, ops);
+    rc |= upload(h, (void**)&h->d_nb, nb);
+    rc |= upload(h, (void**)&h->d_finfo, finfo);
+    rc |= upload(h, (void**)&h->d_fgi, fgi);
+    rc |= upload(h, (void**)&h->d_fid, fid);
+    rc |= upload_real(h, &h->d_egeom, eg);
+    rc |= upload_real(h, &h->d_fgeom, fg);
+    rc |= upload(h, (void**)&h->d_dxe, dxe);
+    rc |= upload(h, (void**)&h->d_jac, jac);
+    rc |= upload(h, (void**)&h->d_w, w);
+    h->bctype.assign(std::max(h->nbd, 1), FLOU_HIP_BC_OUTFLOW);
+    h->bcconst.assign(std::max(h->nbd, 1) * h->nv, 0.0);
+    rc |= upload(h, (void**)&h->d_bctype, h->bctype);
+    rc |= upload_real(h, &h->d_bcconst, h->bcconst);
+    if (rc) return bad(FLOU_HIP_ENOMEM, "device upload failed: " + h->err);
+    if (hipMalloc((void**)&h->d_err, sizeof(int)) != hipSuccess ||
+        hipMalloc((void**)&h->d_partial, reduce_max_blocks() * sizeof(double)) != hipSuccess ||
+        hipMalloc((void**)&h->d_out, 4 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void**)&h->h_out, 4 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return bad(FLOU_HIP_ENOMEM, "device allocation failed");
+    if (hipMemset(h->d_err, 0, sizeof(int)) != hipSuccess) return bad(FLOU_HIP_EDEVICE, "hipMemset failed");
+    *out = h;
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_destroy(flou_hip_handle* h) {
+    if (!h) return FLOU_HIP_EINVAL;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    else (void)hipDeviceSynchronize();
+    free_all(h);
+    delete h;
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_set_stream(flou_hip_handle* h, void* stream) {
+    if (!h) return FLOU_HIP_EINVAL;
+    h->stream = (hipStream_t)stream;
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value) {
+    if (!h) return FLOU_HIP_EINVAL;
+    if (which == 0) { h->typo = value ? 1 : 0; return FLOU_HIP_OK; }
+    return fail(h, FLOU_HIP_EINVAL, "unknown option");
+}
+
+int flou_hip_set_bc(flou_hip_handle* h, int32_t ib, int32_t type, const double* data) {
+    if (!h) return FLOU_HIP_EINVAL;
+    if (ib < 0 || ib >= h->nbd) return fail(h, FLOU_HIP_EINVAL, "boundary index out of range");
+    if (type < FLOU_HIP_BC_DIRICHLET || type > FLOU_HIP_BC_SLIP) return fail(h, FLOU_HIP_EINVAL, "unknown BC type");
+    if (h->eq == FLOU_HIP_ADVECTION && (type == FLOU_HIP_BC_SLIP))
+        return fail(h, FLOU_HIP_EINVAL, "slip walls need the Euler equations");
+    if ((type == FLOU_HIP_BC_DIRICHLET || type == FLOU_HIP_BC_INFLOW) && !data)
+        return fail(h, FLOU_HIP_EINVAL, "BC data missing");
+    h->bctype[ib] = type;
+    if (type == FLOU_HIP_BC_INFLOW)
+        for (int v = 0; v < h->nv; ++v) h->bcconst[ib * h->nv + v] = data[v];
+    if (type == FLOU_HIP_BC_DIRICHLET) {
+        // `data` covers every face; copy only the faces of this boundary
+        const int64_t per = (int64_t)h->npf * h->nv;
+        if (h->bcstate.empty()) h->bcstate.assign(h->nf * per, 0.0);
+        std::vector<int> fid(h->ne * 2 * h->nd), finfo(h->ne * 2 * h->nd), nb(h->ne * 2 * h->nd);
+        HIPCHECK(h, hipMemcpy(fid.data(), h->d_fid, fid.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(h, hipMemcpy(finfo.data(), h->d_finfo, finfo.size() * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(h, hipMemcpy(nb.data(), h->d_nb, nb.size() * sizeof(int), hipMemcpyDeviceToHost));
+        for (size_t s = 0; s < fid.size(); ++s)
+            if (nb[s] < 0 && finfo_bd(finfo[s]) == ib)
+                for (int64_t q = 0; q < per; ++q) h->bcstate[fid[s] * per + q] = data[fid[s] * per + q];
+        if (h->d_bcstate) { (void)hipFree(h->d_bcstate); h->d_bcstate = nullptr; }
+        if (upload_real(h, &h->d_bcstate, h->bcstate)) return FLOU_HIP_ENOMEM;
+    }
+    if (h->d_bctype) HIPCHECK(h, hipMemcpy(h->d_bctype, h->bctype.data(), h->bctype.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (h->d_bcconst) { (void)hipFree(h->d_bcconst); h->d_bcconst = nullptr; }
+    if (upload_real(h, &h->d_bcconst, h->bcconst)) return FLOU_HIP_ENOMEM;
+    return FLOU_HIP_OK;
+}
+
+int64_t flou_hip_ndofs(const flou_hip_handle* h) { return h ? h->ndofs : -1; }
+
+size_t flou_hip_state_bytes(const flou_hip_handle* h) {
+    return h ? (size_t)h->ndofs * h->nv * bytes_of(h) : 0;
+}
+
+int flou_hip_alloc(flou_hip_handle* h, void** dptr, size_t bytes) {
+    if (!h || !dptr) return FLOU_HIP_EINVAL;
+    HIPCHECK(h, hipMalloc(dptr, bytes));
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_free(flou_hip_handle* h, void* dptr) {
+    if (!h) return FLOU_HIP_EINVAL;
+    HIPCHECK(h, hipFree(dptr));
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_upload(flou_hip_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h) return FLOU_HIP_EINVAL;
+    HIPCHECK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_download(flou_hip_handle* h, void* dst, const void* src, size_t bytes) {
+    if (!h) return FLOU_HIP_EINVAL;
+    HIPCHECK(h, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_synchronize(flou_hip_handle* h) {
+    if (!h) return FLOU_HIP_EINVAL;
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_rhs(flou_hip_handle* h, void* dQ, const void* Q, double t) {
+    (void)t;  // the periodic/time-independent BCs of this path do not depend on t
+    if (!h || !dQ || !Q) return FLOU_HIP_EINVAL;
+    if (overlaps(dQ, Q, flou_hip_state_bytes(h))) return fail(h, FLOU_HIP_EINVAL, "dQ and Q overlap");
+    return launch_stage(h, MODE_RHS, 0, 0.0, 0.0, 0.0, Q, nullptr, nullptr, dQ);
+}
+
+int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double dt, double t,
+                      const void* u_in, const void* u_base, void* reg, void* u_out) {
+    (void)t;
+    if (!h || !u_in || !u_out) return FLOU_HIP_EINVAL;
+    const size_t bytes = flou_hip_state_bytes(h);
+    if (overlaps(u_in, u_out, bytes)) return fail(h, FLOU_HIP_EINVAL, "u_out overlaps u_in");
+    if (scheme == FLOU_HIP_SSPRK33) {
+        if (stage < 0 || stage > 2) return fail(h, FLOU_HIP_EINVAL, "SSPRK33 has stages 0..2");
+        if (stage > 0 && !u_base) return fail(h, FLOU_HIP_EINVAL, "u_base missing");
+        if (stage == 1 && overlaps(u_base, u_out, bytes)) return fail(h, FLOU_HIP_EINVAL, "u_out overlaps u_base");
+        const int mode = stage == 0 ? MODE_SSP1 : (stage == 1 ? MODE_SSP2 : MODE_SSP3);
+        return launch_stage(h, mode, 0, dt, 0.0, 0.0, u_in, u_base, nullptr, u_out);
+    }
+    double A, B;
+    if (!tableau(scheme, stage, A, B)) return fail(h, FLOU_HIP_EINVAL, "unknown scheme or stage");
+    if (!reg) return fail(h, FLOU_HIP_EINVAL, "2N schemes need the register buffer");
+    if (overlaps(reg, u_in, bytes) || overlaps(reg, u_out, bytes))
+        return fail(h, FLOU_HIP_EINVAL, "register overlaps a state buffer");
+    return launch_stage(h, MODE_2N, stage == 0, dt, A, B, u_in, nullptr, reg, u_out);
+}
+
+int flou_hip_max_dt(flou_hip_handle* h, const void* Q, double cfl, double* out) {
+    if (!h || !Q || !out) return FLOU_HIP_EINVAL;
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64)
+        e = launch_max_dt<double>((const double*)Q, h->ndofs, h->npe, h->nd, h->eq, h->gamma, h->anorm,
+                                  cfl, h->d_dxe, h->d_partial, h->d_out, h->d_err, h->stream);
+    else
+        e = launch_max_dt<float>((const float*)Q, h->ndofs, h->npe, h->nd, h->eq, h->gamma, h->anorm,
+                                 cfl, h->d_dxe, h->d_partial, h->d_out, h->d_err, h->stream);
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, hipGetErrorString(e));
+    HIPCHECK(h, hipMemcpyAsync(h->h_out, h->d_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    *out = h->h_out[0];
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_integrate(flou_hip_handle* h, const void* Q, int32_t ivar, double* out) {
+    if (!h || !Q || !out) return FLOU_HIP_EINVAL;
+    if (ivar < 0 || ivar >= h->nv) return fail(h, FLOU_HIP_EINVAL, "ivar out of range");
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64)
+        e = launch_integrate<double>((const double*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->npe,
+                                     h->d_jac, h->d_w, h->d_partial, h->d_out, h->stream);
+    else
+        e = launch_integrate<float>((const float*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->npe,
+                                    h->d_jac, h->d_w, h->d_partial, h->d_out, h->stream);
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, hipGetErrorString(e));
+    HIPCHECK(h, hipMemcpyAsync(h->h_out, h->d_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    *out = h->h_out[0];
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_check(flou_hip_handle* h) {
+    if (!h) return FLOU_HIP_EINVAL;
+    int flag = 0;
+    HIPCHECK(h, hipMemcpyAsync(&flag, h->d_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    if (flag) {
+        HIPCHECK(h, hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream));
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        return fail(h, FLOU_HIP_EDOMAIN,
+                    (flag & ERR_SQRT_NEG) ? "DomainError: sqrt of a negative number (non-physical state)"
+                                          : "DomainError: log of a negative number (non-physical state)");
+    }
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_timeintegrate(flou_hip_handle* h, int32_t scheme, void* Q, double dt, int64_t nsteps,
+                           double cfl, double* t_inout) {
+    if (!h || !Q || nsteps < 0) return FLOU_HIP_EINVAL;
+    if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
+    if (!(dt > 0.0)) return fail(h, FLOU_HIP_EINVAL, "dt must be positive");
+    const size_t bytes = flou_hip_state_bytes(h);
+    for (int i = 0; i < 2; ++i)
+        if (!h->d_work[i]) HIPCHECK(h, hipMalloc(&h->d_work[i], bytes));
+    double t = t_inout ? *t_inout : 0.0;
+    // 2N schemes ping-pong the state between Q and work[0] (the register lives in
+    // work[1]); `cur` carries over from step to step and is copied back once at the end.
+    void* cur = Q;
+    void* oth = h->d_work[0];
+    int rc = 0;
+    for (int64_t step = 0; step < nsteps && !rc; ++step) {
+        double dts = dt;
+        if (cfl > 0.0) {
+            double m;
+            rc = flou_hip_max_dt(h, cur, cfl, &m);
+            if (rc) break;
+            dts = std::min(m, dt);
+        }
+        if (scheme == FLOU_HIP_SSPRK33) {
+            rc = flou_hip_rk_stage(h, scheme, 0, dts, t, Q, nullptr, nullptr, h->d_work[0]);
+            if (!rc) rc = flou_hip_rk_stage(h, scheme, 1, dts, t, h->d_work[0], Q, nullptr, h->d_work[1]);
+            if (!rc) rc = flou_hip_rk_stage(h, scheme, 2, dts, t, h->d_work[1], Q, nullptr, Q);
+        } else {
+            for (int s = 0; s < 5 && !rc; ++s) {
+                rc = flou_hip_rk_stage(h, scheme, s, dts, t, cur, nullptr, h->d_work[1], oth);
+                std::swap(cur, oth);
+            }
+        }
+        t += dts;
+    }
+    if (rc) return rc;
+    if (cur != Q) HIPCHECK(h, hipMemcpyAsync(Q, cur, bytes, hipMemcpyDeviceToDevice, h->stream));
+    if (t_inout) *t_inout = t;
+    return flou_hip_check(h);
+}
+
+int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void* Qbar) {
+    (void)Q; (void)dQbar; (void)Qbar;
+    return fail(h, FLOU_HIP_EUNSUPPORTED, "adjoint not built yet");
+}
+
+}  // extern "C"

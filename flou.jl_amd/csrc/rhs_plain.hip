@@ -1,0 +1,344 @@
+// Plain fused element kernel: one pass per element block computes Flou's rhs! for the
+// element and applies the mass matrix and, optionally, an RK stage combination.
+//
+//   Hyperbolic.jl:31-69  fill! -> project2faces! -> volume -> applyBCs! ->
+//                        interface_fluxes! -> surface -> apply_massmatrix!
+//
+// Arithmetic follows the reference term by term (see physics_plain.hpp); the FP64
+// translation units are built with -ffp-contract=off so that every value is rounded where
+// Julia rounds it.  Per element the kernel:
+//   1. stages the element's nodal block (all variables) in LDS, coalesced per variable;
+//   2. caches per-node primitives (Euler) and, for the strong form, the contravariant
+//      volume fluxes in LDS;
+//   3. evaluates the numerical flux on every node of its 2d faces from its own trace
+//      (LDS) and the neighbour's trace (read from HBM/L2), on the master side's frame,
+//      so both elements of a face obtain the same bits Flou's face loop stores in
+//      Fn.sides[1] / Fn.sides[2] (Interfaces.jl:111-136);
+//   4. accumulates volume terms (directions 1..d, columns in order), the FR lift
+//      (OpDivergence.jl:42-100) and divides by J, one thread per node.
+// Compile with -DFLOUHIP_PREC_F64 or -DFLOUHIP_PREC_F32 and -DFLOUHIP_ND=1|2|3.
+#include "common.hpp"
+#include "physics_plain.hpp"
+
+namespace flouhip {
+
+template <typename T, int ND, int N, int EQ, int VOL, int TP>
+__global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a) {
+    using C = Cfg<ND, N>;
+    constexpr int NPE = C::NPE, NPF = C::NPF, NF = C::NF, EPB = C::EPB, TPB = C::TPB;
+    constexpr int NV = EQ == EQ_ADVECTION ? 1 : ND + 2;
+    constexpr int NP = EQ == EQ_EULER ? NV + 1 : 1;
+    constexpr int NFT = VOL == VOL_STRONG ? ND * NV : 1;
+    constexpr int NOPS = ops_size(N);
+    constexpr int NG = ND * ND + 1;
+
+    __shared__ T sOps[NOPS];
+    __shared__ T sG[EPB][NG];
+    __shared__ T sQ[EPB][NV][NPE];
+    __shared__ T sP[EPB][NP][NPE];
+    __shared__ T sFt[EPB][NFT][NPE];
+    __shared__ T sFn[EPB][NF][NV][NPF];
+
+    const int tid = threadIdx.x;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int e0 = blk * EPB;
+    const int64_t Nd = a.ndofs;
+    const T avel[3] = {a.a0, a.a1, a.a2};
+
+    for (int i = tid; i < NOPS; i += TPB) sOps[i] = a.ops[i];
+    for (int i = tid; i < EPB * NG; i += TPB) {
+        const int le = i / NG, c = i % NG, e = e0 + le;
+        if (e < a.ne) sG[le][c] = a.egeom[(int64_t)e * a.egeom_stride + c];
+    }
+    for (int i = tid; i < NV * EPB * NPE; i += TPB) {
+        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+        const int le = j / NPE, nd = j % NPE;
+        if (e0 + le < a.ne) sQ[le][v][nd] = a.Q[v * Nd + (int64_t)e0 * NPE + j];
+    }
+    __syncthreads();
+
+    const int le = tid / NPE, node = tid % NPE, e = e0 + le;
+    const bool valid = e < a.ne;
+    if (valid) {
+        T q[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) q[v] = sQ[le][v][node];
+        if constexpr (EQ == EQ_EULER) {
+            T P[NV];
+            plain::cons2prim<T, ND>(q, a.gamma, P);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) sP[le][v][node] = P[v];
+            sP[le][NV][node] = P[0] / (T(2) * P[ND + 1]);
+        }
+        if constexpr (VOL == VOL_STRONG) {
+#pragma unroll
+            for (int dir = 0; dir < ND; ++dir) {
+                T Ft[NV];
+                plain::contravariant_flux<T, ND, EQ>(q, &sG[le][0], dir, a.gamma, avel, Ft);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sFt[le][dir * NV + v][node] = Ft[v];
+            }
+        }
+    }
+
+    // Numerical fluxes on the element's faces (interface_fluxes!, applyBCs!)
+    for (int it = tid; it < EPB * NF * NPF; it += TPB) {
+        const int le2 = it / (NF * NPF), r = it % (NF * NPF);
+        const int k = r / NPF, f = r % NPF, e2 = e0 + le2;
+        if (e2 >= a.ne) continue;
+        const int dir = k >> 1, lr = k & 1;
+        T own[NV], ext[NV];
+        if (a.gather) {
+            const int nn = line_node<ND, N>(dir, f, lr ? N - 1 : 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) own[v] = sQ[le2][v][nn];
+        } else {
+            const T* l = &sOps[lr ? ops_lp(N) : ops_lm(N)];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                T s = T(0);
+                for (int p = 0; p < N; ++p) s = s + l[p] * sQ[le2][v][line_node<ND, N>(dir, f, p)];
+                own[v] = s;
+            }
+        }
+        const int slot = e2 * NF + k;
+        const int info = a.finfo[slot];
+        const int nbe = a.nb[slot];
+        const T* g = a.fgeom + (int64_t)a.fgi[slot] * (3 * ND + 1);
+        if (nbe >= 0) {
+            const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
+            const T* Qn = a.Q + (int64_t)nbe * NPE;
+            if (a.gather) {
+                const int nn = line_node<ND, N>(ndir, f, nlr ? N - 1 : 0);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = Qn[v * Nd + nn];
+            } else {
+                const T* l = &sOps[nlr ? ops_lp(N) : ops_lm(N)];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    T s = T(0);
+                    for (int p = 0; p < N; ++p) s = s + l[p] * Qn[v * Nd + line_node<ND, N>(ndir, f, p)];
+                    ext[v] = s;
+                }
+            }
+        } else {
+            const int bd = finfo_bd(info);
+            const int type = a.bctype[bd];
+            if (type == BC_DIRICHLET) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = a.bcstate[((int64_t)a.fid[slot] * NPF + f) * NV + v];
+            } else if (type == BC_INFLOW) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = a.bcconst[bd * NV + v];
+            } else if (type == BC_OUTFLOW || EQ == EQ_ADVECTION) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = own[v];
+            } else {
+                if constexpr (EQ == EQ_EULER) {
+                    T R[NV];
+                    plain::rotate2face<T, ND>(own, g, R);
+                    R[1] = -R[1];
+                    plain::rotate2phys<T, ND>(R, g, ext);
+                }
+            }
+        }
+        const bool slave = finfo_side(info);
+        T Ql[NV], Qr[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Ql[v] = slave ? ext[v] : own[v];
+            Qr[v] = slave ? own[v] : ext[v];
+        }
+        T F[NV];
+        if constexpr (EQ == EQ_ADVECTION) {
+            T an = avel[0] * g[0];
+#pragma unroll
+            for (int kk = 1; kk < ND; ++kk) an = an + avel[kk] * g[kk];
+            F[0] = (an * (Ql[0] + Qr[0])) / T(2);
+            if (a.sdiss == DISS_LXF) F[0] = F[0] + ((fabs(an) * (Ql[0] - Qr[0])) / T(2)) * a.intensity;
+        } else {
+            T Rl[NV], Rr[NV], Fr[NV];
+            plain::rotate2face<T, ND>(Ql, g, Rl);
+            plain::rotate2face<T, ND>(Qr, g, Rr);
+            plain::euler_numericalflux<T, ND>(Rl, Rr, a.savg, a.sdiss, a.typo, a.intensity,
+                                              a.gamma, a.errflag, Fr);
+            plain::rotate2phys<T, ND>(Fr, g, F);
+        }
+        const T jac = g[3 * ND];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const T x = F[v] * jac;
+            sFn[le2][k][v][f] = slave ? -x : x;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    // Volume terms, one thread per node, in the reference's summation order
+    T acc[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) acc[v] = T(0);
+    const T* M = &sG[le][0];
+    const T J = sG[le][ND * ND];
+#pragma unroll
+    for (int dir = 0; dir < ND; ++dir) {
+        int f, ii;
+        node_line<ND, N>(node, dir, f, ii);
+        if constexpr (VOL == VOL_STRONG) {
+            const T* Ds = &sOps[ops_Ds(N)];
+#pragma unroll
+            for (int kk = 0; kk < N; ++kk) {
+                const int nk = line_node<ND, N>(dir, f, kk);
+                const T Dik = Ds[ii * N + kk];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + Dik * (-sFt[le][dir * NV + v][nk]);
+            }
+        } else {
+            const T* Dsh = &sOps[ops_Dsh(N)];
+            T qi[NV], Pi[NV], bi = T(0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) qi[v] = sQ[le][v][node];
+            if constexpr (EQ == EQ_EULER) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Pi[v] = sP[le][v][node];
+                bi = sP[le][NV][node];
+            }
+            T Fd[NV];
+            plain::contravariant_flux<T, ND, EQ>(qi, M, dir, a.gamma, avel, Fd);
+            for (int j = 0; j < N; ++j) {
+                const int nj = line_node<ND, N>(dir, f, j);
+                const T Dij = Dsh[ii * N + j];
+                T F[NV];
+                if (j == ii) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) F[v] = Fd[v];
+                } else {
+                    T qj[NV], Pj[NV], bj = T(0);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) qj[v] = sQ[le][v][nj];
+                    if constexpr (EQ == EQ_EULER) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Pj[v] = sP[le][v][nj];
+                        bj = sP[le][NV][nj];
+                    }
+                    // F♯ entries are twopointflux(Q_lower position, Q_higher position)
+                    if (j > ii)
+                        plain::twopointflux<T, ND, EQ, TP>(qi, qj, Pi, Pj, bi, bj, M + ND * dir,
+                                                           M + ND * dir, a.gamma, avel, a.errflag, F);
+                    else
+                        plain::twopointflux<T, ND, EQ, TP>(qj, qi, Pj, Pi, bj, bi, M + ND * dir,
+                                                           M + ND * dir, a.gamma, avel, a.errflag, F);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] - Dij * F[v];
+            }
+        }
+    }
+
+    // FR lift: dQ[line[ii]] -= ∂gl[ii]*Fn_left + ∂gr[ii]*Fn_right, direction by direction
+    const T* gm = &sOps[ops_gm(N)];
+    const T* gp = &sOps[ops_gp(N)];
+#pragma unroll
+    for (int dir = 0; dir < ND; ++dir) {
+        int f, ii;
+        node_line<ND, N>(node, dir, f, ii);
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            acc[v] = acc[v] - (gm[ii] * sFn[le][2 * dir][v][f] + gp[ii] * sFn[le][2 * dir + 1][v][f]);
+    }
+
+    // Mass matrix (ldiv! by Diagonal(J)) and the stage epilogue
+    const int64_t gi = (int64_t)e * NPE + node;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const T kv = acc[v] / J;
+        const int64_t idx = v * Nd + gi;
+        const T uin = sQ[le][v][node];
+        switch (a.mode) {
+            case MODE_RHS:
+                a.out[idx] = kv;
+                break;
+            case MODE_2N: {
+                const T t = a.first ? a.dt * kv : a.A * a.reg[idx] + a.dt * kv;
+                a.reg[idx] = t;
+                a.out[idx] = uin + a.B * t;
+                break;
+            }
+            case MODE_SSP1:
+                a.out[idx] = uin + a.dt * kv;
+                break;
+            case MODE_SSP2:
+                a.out[idx] = ((T(3) * a.ubase[idx] + uin) + a.dt * kv) / T(4);
+                break;
+            default:
+                a.out[idx] = ((a.ubase[idx] + T(2) * uin) + (T(2) * a.dt) * kv) / T(3);
+                break;
+        }
+    }
+}
+
+template <typename T, int ND, int N, int EQ, int VOL, int TP>
+static hipError_t launch_plain(const RhsArgs<T>& a, hipStream_t s) {
+    using C = Cfg<ND, N>;
+    const int nblocks = (a.ne + C::EPB - 1) / C::EPB;
+    hipLaunchKernelGGL((k_plain<T, ND, N, EQ, VOL, TP>), dim3(nblocks), dim3(C::TPB), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int ND, int N>
+static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
+    if (eq == EQ_ADVECTION)
+        return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0>
+                                 : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0>;
+    if (vol == VOL_STRONG) return &launch_plain<T, ND, N, EQ_EULER, VOL_STRONG, 0>;
+    switch (tp) {
+        case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG>;
+        case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR>;
+        case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER>;
+        default: return nullptr;
+    }
+}
+
+template <typename T, int ND>
+static hipError_t (*pick_plain_n(int n, int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
+    switch (n) {
+        case 2: return pick_plain<T, ND, 2>(eq, vol, tp);
+        case 3: return pick_plain<T, ND, 3>(eq, vol, tp);
+        case 4: return pick_plain<T, ND, 4>(eq, vol, tp);
+        case 5: return pick_plain<T, ND, 5>(eq, vol, tp);
+        case 6: return pick_plain<T, ND, 6>(eq, vol, tp);
+        case 7: return pick_plain<T, ND, 7>(eq, vol, tp);
+        case 8: return pick_plain<T, ND, 8>(eq, vol, tp);
+        default: return nullptr;
+    }
+}
+
+#if defined(FLOUHIP_PREC_F64)
+#define FLOUHIP_SCALAR double
+#define FLOUHIP_PTAG f64
+#elif defined(FLOUHIP_PREC_F32)
+#define FLOUHIP_SCALAR float
+#define FLOUHIP_PTAG f32
+#else
+#error "define FLOUHIP_PREC_F64 or FLOUHIP_PREC_F32"
+#endif
+
+#define FLOUHIP_CAT3(a, b, c) a##_##b##_##c
+#define FLOUHIP_NAME(p, d) FLOUHIP_CAT3(plain, p, d)
+
+#if FLOUHIP_ND == 1
+#define FLOUHIP_DTAG 1d
+#elif FLOUHIP_ND == 2
+#define FLOUHIP_DTAG 2d
+#elif FLOUHIP_ND == 3
+#define FLOUHIP_DTAG 3d
+#else
+#error "define FLOUHIP_ND"
+#endif
+
+hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp))(
+    const RhsArgs<FLOUHIP_SCALAR>&, hipStream_t) {
+    return pick_plain_n<FLOUHIP_SCALAR, FLOUHIP_ND>(n, eq, vol, tp);
+}
+
+}  // namespace flouhip

@@ -1,0 +1,33 @@
+"""flouhip -- MI355X (gfx950) DGSEM hot path of Flou.jl behind a C ABI.
+
+Host-side mirror of the Flou types a user composes to reach rhs! / timeintegrate; every
+kernel runs in libflouhip.so (flou.jl_amd/libflouhip.so, see include/flouhip.h).  Importing
+this package fails with ImportError when the library is missing: there is no CPU path.
+"""
+from . import _capi
+from ._capi import DomainError
+from .disc import (CarpenterKennedy2N54, EquationConfig, MultielementDisc, ORK256, SSPRK33,
+                   get_max_dt, integrate, rhs_, rk_stage, timeintegrate)
+from .equations import (ChandrasekharAverage, EulerEquation, EulerInflowBC, EulerOutflowBC,
+                        EulerSlipBC, GenericBC, KennedyGruber, LinearAdvection, LxF,
+                        MatrixDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
+                        nvariables, pressure, soundvelocity, vars_cons2prim, vars_prim2cons)
+from .mesh import CartesianMesh, apply_periodicBCs
+from .stdregions import DGSEMrec, LagrangeBasis, StdHex, StdQuad, StdSegment
+
+LIB_PATH = _capi.LIB_PATH
+
+
+def version():
+    return _capi.lib.flou_hip_version().decode()
+
+
+__all__ = [
+    "CarpenterKennedy2N54", "CartesianMesh", "ChandrasekharAverage", "DGSEMrec", "DomainError",
+    "EquationConfig", "EulerEquation", "EulerInflowBC", "EulerOutflowBC", "EulerSlipBC",
+    "GenericBC", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
+    "MultielementDisc", "ORK256", "SSPRK33", "SplitDivOperator", "StdAverage", "StdHex",
+    "StdQuad", "StdSegment", "StrongDivOperator", "apply_periodicBCs", "get_max_dt",
+    "integrate", "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
+    "timeintegrate", "vars_cons2prim", "vars_prim2cons", "version",
+]

@@ -1,0 +1,280 @@
+"""MultielementDisc on the GPU and the entry points that replace Flou's hot path.
+
+    MultielementDisc(mesh, std, equation, operator, bcs; precision)  -> flou_hip_create
+    rhs_(dQ, Q, EquationConfig(disc, eq), t)   (FlouCommon.rhs!)        -> flou_hip_rhs
+    timeintegrate(Q0, disc, eq, solver, tf; dt, cfl)                   -> flou_hip_timeintegrate
+    get_max_dt(Q, disc, eq, cfl)                                       -> flou_hip_max_dt
+    integrate(Q, disc, ivar)                                           -> flou_hip_integrate
+
+State buffers are torch tensors on the GPU holding Flou's (ndofs, nvars) column-major
+matrix, i.e. a contiguous (nvars, ndofs) tensor (or its transpose view).  torch is only the
+device allocator and stream provider here; all arithmetic runs in libflouhip.
+"""
+import ctypes
+import time
+
+import numpy as np
+
+from . import _capi as C
+from .equations import EulerEquation, LinearAdvection, _surface_codes
+from .stdregions import StdHex, StdQuad, StdSegment
+
+try:
+    import torch
+except ImportError:  # pragma: no cover - torch is part of the runtime image
+    torch = None
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(ctypes.POINTER(t))
+
+
+class EquationConfig:
+    def __init__(self, disc, equation):
+        self.disc, self.equation = disc, equation
+
+
+class MultielementDisc:
+    """GPU MultielementDisc.  `precision` is "fp64" (plain form) or "fp32" (FP32 state and
+    bulk arithmetic; dt and integrals stay FP64)."""
+
+    def __init__(self, mesh, std, equation, operator, bcs=None, precision="fp64"):
+        bcs = dict(bcs or {})
+        nd = mesh.nd
+        if std.ND != nd or equation.nd != nd:
+            raise ValueError("mesh, standard element and equation dimensions differ")
+        if len(bcs) != mesh.nboundaries():
+            raise ValueError("The number of BCs does not match the number of boundaries.")
+        if operator.code == C.SPLIT and not std.basis.hasboundaries:
+            raise NotImplementedError(
+                "the split form on Gauss nodes (subgrid entropy projection) is not on the GPU path")
+        self.mesh, self.std, self.equation, self.operator = mesh, std, equation, operator
+        self.precision = precision
+        self.dtype = {"fp64": np.float64, "fp32": np.float32}[precision]
+        self.nd, self.n = nd, std.n
+        self.nvars = equation.nvariables
+        self.nelements = mesh.nelements
+        self.npe = std.n ** nd
+        self.ndofs = self.nelements * self.npe
+
+        d = C.Desc()
+        d.nd, d.n, d.nvars = nd, std.n, self.nvars
+        d.precision = C.FP64 if precision == "fp64" else C.FP32
+        d.nelements, d.nfaces = mesh.nelements, mesh.nfaces
+        d.nboundaries = mesh.nboundaries()
+        if isinstance(equation, EulerEquation):
+            d.equation, d.gamma = C.EULER, equation.γ
+        elif isinstance(equation, LinearAdvection):
+            d.equation = C.ADVECTION
+            for k, a in enumerate(equation.a):
+                d.velocity[k] = a
+        else:
+            raise ValueError("unsupported equation")
+        d.volume_op = operator.code
+        d.tp_flux = operator.tpflux.tp_code if operator.code == C.SPLIT else C.TP_STDAVG
+        d.surface_avg, d.surface_diss, d.intensity = _surface_codes(operator.numflux)
+
+        keep = []  # arrays referenced by the descriptor must outlive create()
+
+        def arr(x, dt):
+            a = np.ascontiguousarray(x, dtype=dt)
+            keep.append(a)
+            return a
+
+        D = arr(std.D, np.float64)
+        d.D = _ptr(D, ctypes.c_double)
+        d.Ds = _ptr(arr(std.Ds, np.float64), ctypes.c_double)
+        d.Dsharp = _ptr(arr(std.Dsharp, np.float64), ctypes.c_double)
+        d.l_minus = _ptr(arr(std.l[0], np.float64), ctypes.c_double)
+        d.l_plus = _ptr(arr(std.l[1], np.float64), ctypes.c_double)
+        d.dg_minus = _ptr(arr(std.dg[0], np.float64), ctypes.c_double)
+        d.dg_plus = _ptr(arr(std.dg[1], np.float64), ctypes.c_double)
+        d.omega = _ptr(arr(std.ω1, np.float64), ctypes.c_double)
+        d.elem_faceinds = _ptr(arr(mesh.elem_faces, np.int64), ctypes.c_int64)
+        d.elem_facepos = _ptr(arr(mesh.elem_side, np.int8), ctypes.c_int8)
+        d.face_eleminds = _ptr(arr(mesh.face_elems, np.int64), ctypes.c_int64)
+        d.face_elempos = _ptr(arr(mesh.face_pos, np.int8), ctypes.c_int8)
+        d.face_orientation = None
+        d.face_boundary = _ptr(arr(mesh.face_boundary, np.int32), ctypes.c_int32)
+        ejac, emet, evol = mesh.element_geometry(std)
+        ffr, fjac = mesh.face_geometry()
+        d.geometry = C.GEOM_AFFINE
+        d.elem_jac = _ptr(arr(ejac, np.float64), ctypes.c_double)
+        d.elem_metric = _ptr(arr(emet, np.float64), ctypes.c_double)
+        d.elem_volume = _ptr(arr(evol, np.float64), ctypes.c_double)
+        d.face_frames = _ptr(arr(ffr, np.float64), ctypes.c_double)
+        d.face_jac = _ptr(arr(fjac, np.float64), ctypes.c_double)
+        self.elem_jac, self.elem_volume = ejac, evol
+
+        h = ctypes.c_void_p()
+        C.check(C.lib.flou_hip_create(ctypes.byref(d), ctypes.byref(h)), None, "MultielementDisc")
+        self.h = h
+        self._set_bcs(bcs)
+        self._sync_stream()
+
+    # ------------------------------------------------------------------ lifecycle
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h is not None and h.value and C is not None and C.lib is not None:
+            C.lib.flou_hip_destroy(h)
+            self.h = None
+
+    def _sync_stream(self):
+        if torch is not None and torch.cuda.is_available():
+            s = torch.cuda.current_stream().cuda_stream
+            C.check(C.lib.flou_hip_set_stream(self.h, ctypes.c_void_p(s)), self.h)
+
+    def set_option(self, which, value):
+        C.check(C.lib.flou_hip_set_option(self.h, which, value), self.h)
+
+    def _set_bcs(self, bcs):
+        mesh = self.mesh
+        for name, bc in bcs.items():
+            orig = int(name) - 1
+            if orig not in mesh.bdalive:
+                raise ValueError(f"unknown boundary {name}")
+            ib = mesh.bdalive.index(orig)
+            if bc.code == C.BC_DIRICHLET:
+                fx = mesh.face_node_coords(self.std)
+                npf = fx.shape[1]
+                data = np.zeros((mesh.nfaces, npf, self.nvars))
+                for f in np.nonzero(mesh.face_boundary == ib)[0]:
+                    for i in range(npf):
+                        data[f, i] = np.asarray(bc.Qext(None, fx[f, i], None, 0.0, self.equation),
+                                                dtype=np.float64)
+                C.check(C.lib.flou_hip_set_bc(self.h, ib, bc.code, _ptr(np.ascontiguousarray(data), ctypes.c_double)), self.h)
+            elif bc.code == C.BC_INFLOW:
+                q = np.ascontiguousarray(bc.Qext, dtype=np.float64)
+                C.check(C.lib.flou_hip_set_bc(self.h, ib, bc.code, _ptr(q, ctypes.c_double)), self.h)
+            else:
+                C.check(C.lib.flou_hip_set_bc(self.h, ib, bc.code, None), self.h)
+
+    # ------------------------------------------------------------------ state helpers
+    def torch_dtype(self):
+        return torch.float64 if self.precision == "fp64" else torch.float32
+
+    def new_state(self):
+        """Uninitialised device state, a (nvars, ndofs) tensor (Flou's Q.data transposed)."""
+        return torch.empty((self.nvars, self.ndofs), dtype=self.torch_dtype(), device="cuda")
+
+    def to_device(self, Q):
+        """Host (ndofs, nvars) array -> device state tensor."""
+        Q = np.asarray(Q)
+        if Q.shape != (self.ndofs, self.nvars):
+            raise ValueError(f"state must be ({self.ndofs}, {self.nvars})")
+        return torch.from_numpy(np.ascontiguousarray(Q.T.astype(self.dtype))).to("cuda")
+
+    def to_host(self, Qd):
+        torch.cuda.synchronize()
+        return Qd.detach().to("cpu").numpy().T.copy()
+
+    def _dptr(self, t):
+        if not isinstance(t, torch.Tensor) or not t.is_cuda:
+            raise TypeError("state buffers must be CUDA (HIP) torch tensors")
+        if t.dtype != self.torch_dtype():
+            raise TypeError(f"state dtype must be {self.torch_dtype()}")
+        if t.numel() != self.ndofs * self.nvars:
+            raise ValueError("state has the wrong size")
+        if t.shape == (self.nvars, self.ndofs) and t.is_contiguous():
+            return ctypes.c_void_p(t.data_ptr())
+        if t.shape == (self.ndofs, self.nvars) and t.stride() == (1, self.ndofs):
+            return ctypes.c_void_p(t.data_ptr())
+        raise ValueError("state must be a contiguous (nvars, ndofs) tensor or its transpose")
+
+    def check(self):
+        """Synchronise and raise DomainError if a kernel met a non-physical state."""
+        C.check(C.lib.flou_hip_check(self.h), self.h)
+
+
+def rhs_(dQ, Q, config, t=0.0):
+    """FlouCommon.rhs!(dQ, Q, EquationConfig(disc, equation), t) on the GPU."""
+    disc = config.disc
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rhs(disc.h, disc._dptr(dQ), disc._dptr(Q), float(t)), disc.h, "rhs!")
+    return None
+
+
+def rk_stage(disc, scheme, stage, dt, u_in, u_out, u_base=None, reg=None, t=0.0):
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rk_stage(disc.h, scheme, stage, float(dt), float(t), disc._dptr(u_in),
+                                    disc._dptr(u_base) if u_base is not None else None,
+                                    disc._dptr(reg) if reg is not None else None,
+                                    disc._dptr(u_out)), disc.h, "rk_stage")
+
+
+class SSPRK33:
+    code = C.SSPRK33
+    nstages = 3
+
+
+class CarpenterKennedy2N54:
+    code = C.CK2N54
+    nstages = 5
+
+    def __init__(self, williamson_condition=False):
+        self.williamson_condition = williamson_condition
+
+
+class ORK256:
+    code = C.ORK256
+    nstages = 5
+
+    def __init__(self, williamson_condition=False):
+        self.williamson_condition = williamson_condition
+
+
+def _solver_code(solver):
+    if isinstance(solver, type):
+        solver = solver()
+    return solver.code
+
+
+def timeintegrate(Q0, disc, equation, solver, tfinal, dt, cfl=None, t0=0.0):
+    """timeintegrate(Q0, disc, equation, solver, tf; dt, adaptive=false) -- FlouTime.jl:34-56.
+
+    Integrates in place on the device tensor Q0 with fixed steps of `dt` (the last step is
+    shortened to land on tfinal, as OrdinaryDiffEq does).  With `cfl`, every step uses
+    min(get_max_dt(Q, cfl), dt) (get_cfl_callback(cfl, dt), FlouTime.jl:92-104).
+    Returns (Q0, elapsed seconds).  Raises DomainError on a non-physical state
+    (the reference logs "Simulation crashed!")."""
+    code = _solver_code(solver)
+    disc._sync_stream()
+    ptr = disc._dptr(Q0)
+    t = ctypes.c_double(float(t0))
+    tic = time.perf_counter()
+    if cfl is None:
+        span = float(tfinal) - float(t0)
+        nfull = int(np.floor(span / dt * (1 + 1e-12)))
+        rest = span - nfull * dt
+        C.check(C.lib.flou_hip_timeintegrate(disc.h, code, ptr, float(dt), nfull, 0.0,
+                                             ctypes.byref(t)), disc.h, "timeintegrate")
+        if rest > 1e-12 * max(1.0, abs(float(tfinal))):
+            C.check(C.lib.flou_hip_timeintegrate(disc.h, code, ptr, float(rest), 1, 0.0,
+                                                 ctypes.byref(t)), disc.h, "timeintegrate")
+    else:
+        while t.value < float(tfinal) * (1 - 1e-14):
+            step = min(float(dt), float(tfinal) - t.value)
+            C.check(C.lib.flou_hip_timeintegrate(disc.h, code, ptr, step, 1, float(cfl),
+                                                 ctypes.byref(t)), disc.h, "timeintegrate")
+    torch.cuda.synchronize()
+    return Q0, time.perf_counter() - tic
+
+
+def get_max_dt(Q, disc, equation, cfl):
+    """get_max_dt(q, disc, eq, cfl) (MultielementDiscontinuous.jl:162-178), FP64."""
+    disc._sync_stream()
+    out = ctypes.c_double()
+    C.check(C.lib.flou_hip_max_dt(disc.h, disc._dptr(Q), float(cfl), ctypes.byref(out)), disc.h, "get_max_dt")
+    return out.value
+
+
+def integrate(Q, disc, ivar=0):
+    """integrate(f, disc, ivar) (MultielementDiscontinuous.jl:148-160), FP64; ivar 0-based."""
+    disc._sync_stream()
+    out = ctypes.c_double()
+    C.check(C.lib.flou_hip_integrate(disc.h, disc._dptr(Q), int(ivar), ctypes.byref(out)), disc.h, "integrate")
+    return out.value
+
+
+def std_for(nd):
+    return {1: StdSegment, 2: StdQuad, 3: StdHex}[nd]

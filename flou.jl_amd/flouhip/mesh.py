@@ -1,0 +1,343 @@
+"""Cartesian meshes, periodic rewiring and Cartesian geometry (host side, setup only).
+
+Mirrors CartesianMesh{ND,RT}(start, finish, nxyz) and apply_periodicBCs!
+(src/FlouCommon/CartesianMesh.jl:38-150, 152-500; src/FlouCommon/Mesh.jl:236-322) and the
+Cartesian PhysicalElement / PhysicalFace constructors (src/FlouSpatial/PhysicalRegions.jl:
+370-408, 541-696), vectorised with numpy so that 64^3-element meshes build in seconds.
+Numbering is the reference's, shifted to 0-based.
+"""
+import numpy as np
+
+LD = np.longdouble
+
+
+class CartesianMesh:
+    """CartesianMesh(ND, start, finish, nxyz)."""
+
+    def __init__(self, nd, start, finish, nxyz):
+        if nd not in (1, 2, 3):
+            raise ValueError("The mesh can only have 1, 2 or 3 dimensions.")
+        start = np.atleast_1d(np.asarray(start, dtype=np.float64))
+        finish = np.atleast_1d(np.asarray(finish, dtype=np.float64))
+        nxyz = tuple(int(x) for x in np.atleast_1d(nxyz))
+        if len(start) != nd:
+            raise ValueError(f"The `start` point must have {nd} coordinates.")
+        if len(finish) != nd:
+            raise ValueError(f"The `finish` point must have {nd} coordinates.")
+        if len(nxyz) != nd:
+            raise ValueError("The number of elements in all directions must be given.")
+        if not np.all(start < finish):
+            raise ValueError("All components of `start` must be lower than those of `finish`.")
+        self.nd = nd
+        self.start, self.finish, self.nelements_dir = start, finish, nxyz
+        self.Δx = tuple((finish[k] - start[k]) / nxyz[k] for k in range(nd))
+        # vertex coordinates: range(start, finish, n+1), one rounding from extended precision
+        self.xyz = [np.array([LD(start[k]) + (LD(finish[k]) - LD(start[k])) * i / nxyz[k]
+                              for i in range(nxyz[k] + 1)], dtype=LD).astype(np.float64)
+                    for k in range(nd)]
+        self._build()
+        self.bdnames = [str(i + 1) for i in range(2 * nd)]
+        self.periodic = {}
+
+    # ------------------------------------------------------------------ connectivity
+    def _build(self):
+        nd, n = self.nd, self.nelements_dir
+        if nd == 1:
+            nx, = n
+            i = np.arange(nx)
+            self.elem_faces = np.stack([i, i + 1], axis=1)
+            self.elem_side = np.stack([np.where(i == 0, 0, 1), np.zeros(nx, int)], axis=1)
+            nf = nx + 1
+            f = np.arange(nf)
+            fe = np.stack([f - 1, f], axis=1)
+            fp = np.tile([1, 0], (nf, 1))
+            fe[0] = [0, -1]; fp[0] = [0, -1]
+            fe[-1] = [nx - 1, -1]; fp[-1] = [1, -1]
+            self.face_elems, self.face_pos = fe, fp
+            self.bdfaces = [np.array([0]), np.array([nf - 1])]
+            self.face_vertices = f[:, None]
+        elif nd == 2:
+            nx, ny = n
+            npx, npy = nx + 1, ny + 1
+            jj, ii = np.meshgrid(np.arange(ny), np.arange(nx), indexing="ij")
+            ii, jj = ii.ravel(), jj.ravel()
+            fx = lambda i, j: j * npx + i
+            fy = lambda i, j: npx * ny + j * nx + i
+            self.elem_faces = np.stack([fx(ii, jj), fx(ii + 1, jj), fy(ii, jj), fy(ii, jj + 1)], axis=1)
+            self.elem_side = np.stack([np.where(ii == 0, 0, 1), np.zeros_like(ii),
+                                       np.where(jj == 0, 0, 1), np.zeros_like(ii)], axis=1)
+            el = lambda i, j: j * nx + i
+            nfx, nfy = npx * ny, nx * npy
+            fe = np.zeros((nfx + nfy, 2), dtype=np.int64)
+            fp = np.zeros((nfx + nfy, 2), dtype=np.int64)
+            J, I = np.meshgrid(np.arange(ny), np.arange(npx), indexing="ij")
+            I, J = I.ravel(), J.ravel()
+            fe[:nfx, 0] = np.where(I == 0, el(0, J), el(np.maximum(I - 1, 0), J))
+            fe[:nfx, 1] = np.where((I == 0) | (I == nx), -1, el(np.minimum(I, nx - 1), J))
+            fp[:nfx, 0] = np.where(I == 0, 0, 1)
+            fp[:nfx, 1] = np.where((I == 0) | (I == nx), -1, 0)
+            J, I = np.meshgrid(np.arange(npy), np.arange(nx), indexing="ij")
+            I, J = I.ravel(), J.ravel()
+            fe[nfx:, 0] = np.where(J == 0, el(I, 0), el(I, np.maximum(J - 1, 0)))
+            fe[nfx:, 1] = np.where((J == 0) | (J == ny), -1, el(I, np.minimum(J, ny - 1)))
+            fp[nfx:, 0] = np.where(J == 0, 2, 3)
+            fp[nfx:, 1] = np.where((J == 0) | (J == ny), -1, 2)
+            self.face_elems, self.face_pos = fe, fp
+            allf = np.arange(nfx + nfy)
+            Ix = allf[:nfx] % npx
+            Jy = (allf[nfx:] - nfx) // nx
+            self.bdfaces = [allf[:nfx][Ix == 0], allf[:nfx][Ix == nx],
+                            allf[nfx:][Jy == 0], allf[nfx:][Jy == ny]]
+            # face vertex lists (for face-node coordinates)
+            v = lambda i, j: j * npx + i
+            fv = np.zeros((nfx + nfy, 2), dtype=np.int64)
+            J, I = np.meshgrid(np.arange(ny), np.arange(npx), indexing="ij")
+            fv[:nfx] = np.stack([v(I.ravel(), J.ravel()), v(I.ravel(), J.ravel() + 1)], axis=1)
+            J, I = np.meshgrid(np.arange(npy), np.arange(nx), indexing="ij")
+            fv[nfx:] = np.stack([v(I.ravel(), J.ravel()), v(I.ravel() + 1, J.ravel())], axis=1)
+            self.face_vertices = fv
+        else:
+            nx, ny, nz = n
+            npx, npy, npz = nx + 1, ny + 1, nz + 1
+            kk, jj, ii = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+            ii, jj, kk = ii.ravel(), jj.ravel(), kk.ravel()
+            nfx, nfy, nfz = npx * ny * nz, nx * npy * nz, nx * ny * npz
+            fx = lambda i, j, k: k * npx * ny + j * npx + i
+            fy = lambda i, j, k: nfx + k * nx * npy + j * nx + i
+            fz = lambda i, j, k: nfx + nfy + k * nx * ny + j * nx + i
+            self.elem_faces = np.stack([fx(ii, jj, kk), fx(ii + 1, jj, kk), fy(ii, jj, kk),
+                                        fy(ii, jj + 1, kk), fz(ii, jj, kk), fz(ii, jj, kk + 1)], axis=1)
+            z = np.zeros_like(ii)
+            self.elem_side = np.stack([np.where(ii == 0, 0, 1), z, np.where(jj == 0, 0, 1), z,
+                                       np.where(kk == 0, 0, 1), z], axis=1)
+            el = lambda i, j, k: k * nx * ny + j * nx + i
+            nf = nfx + nfy + nfz
+            fe = np.zeros((nf, 2), dtype=np.int64)
+            fp = np.zeros((nf, 2), dtype=np.int64)
+            bd = [None] * 6
+            # X faces
+            K, J, I = [a.ravel() for a in np.meshgrid(np.arange(nz), np.arange(ny), np.arange(npx), indexing="ij")]
+            s = slice(0, nfx)
+            fe[s, 0] = np.where(I == 0, el(0, J, K), el(np.maximum(I - 1, 0), J, K))
+            fe[s, 1] = np.where((I == 0) | (I == nx), -1, el(np.minimum(I, nx - 1), J, K))
+            fp[s, 0] = np.where(I == 0, 0, 1)
+            fp[s, 1] = np.where((I == 0) | (I == nx), -1, 0)
+            idx = np.arange(nfx)
+            bd[0], bd[1] = idx[I == 0], idx[I == nx]
+            # Y faces
+            K, J, I = [a.ravel() for a in np.meshgrid(np.arange(nz), np.arange(npy), np.arange(nx), indexing="ij")]
+            s = slice(nfx, nfx + nfy)
+            fe[s, 0] = np.where(J == 0, el(I, 0, K), el(I, np.maximum(J - 1, 0), K))
+            fe[s, 1] = np.where((J == 0) | (J == ny), -1, el(I, np.minimum(J, ny - 1), K))
+            fp[s, 0] = np.where(J == 0, 2, 3)
+            fp[s, 1] = np.where((J == 0) | (J == ny), -1, 2)
+            idx = np.arange(nfx, nfx + nfy)
+            bd[2], bd[3] = idx[J == 0], idx[J == ny]
+            # Z faces
+            K, J, I = [a.ravel() for a in np.meshgrid(np.arange(npz), np.arange(ny), np.arange(nx), indexing="ij")]
+            s = slice(nfx + nfy, nf)
+            fe[s, 0] = np.where(K == 0, el(I, J, 0), el(I, J, np.maximum(K - 1, 0)))
+            fe[s, 1] = np.where((K == 0) | (K == nz), -1, el(I, J, np.minimum(K, nz - 1)))
+            fp[s, 0] = np.where(K == 0, 4, 5)
+            fp[s, 1] = np.where((K == 0) | (K == nz), -1, 4)
+            idx = np.arange(nfx + nfy, nf)
+            bd[4], bd[5] = idx[K == 0], idx[K == nz]
+            self.face_elems, self.face_pos, self.bdfaces = fe, fp, bd
+            self.face_vertices = None  # 3-D face-node coordinates are computed on demand
+        self.face_elems = self.face_elems.astype(np.int64)
+        self.face_pos = self.face_pos.astype(np.int64)
+        self.elem_faces = self.elem_faces.astype(np.int64)
+        self.elem_side = self.elem_side.astype(np.int64)
+        self.nelements = self.elem_faces.shape[0]
+        self.nfaces = self.face_elems.shape[0]
+        self.face_boundary = np.full(self.nfaces, -1, dtype=np.int64)
+        for b, fl in enumerate(self.bdfaces):
+            self.face_boundary[fl] = b
+        self.bdalive = list(range(2 * self.nd))  # original ids of the remaining boundaries
+        self._orig_face = np.arange(self.nfaces)
+
+    def nboundaries(self):
+        return len(self.bdalive)
+
+    # ------------------------------------------------------------------ periodicity
+    def apply_periodicBCs(self, *pairs):
+        """apply_periodicBCs!(mesh, "1" => "2", ...): pairs of boundary names ("1", "2")."""
+        todo = []
+        for p in pairs:
+            try:
+                b1, b2 = int(p[0]), int(p[1])
+            except (TypeError, ValueError):
+                raise ValueError("Boundary IDs must be integers.")
+            nb = 2 * self.nd
+            if not (1 <= b1 <= nb and 1 <= b2 <= nb):
+                raise ValueError(f"Boundary IDs must be between 1 and {nb}.")
+            if not (b2 % 2 == 0 and b1 + 1 == b2):
+                raise ValueError(f"Boundaries {p[0]} and {p[1]} cannot be made periodic.")
+            if (b1 - 1) not in self.bdalive or (b2 - 1) not in self.bdalive:
+                raise ValueError(f"Boundaries {p[0]} and {p[1]} cannot be periodic.")
+            todo.append((b1 - 1, b2 - 1))
+        fe, fp = self.face_elems, self.face_pos
+        delete = []
+        for b1, b2 in todo:
+            f1 = self.bdfaces[self.bdalive.index(b1)]
+            f2 = self.bdfaces[self.bdalive.index(b2)]
+            e2 = fe[f2, 0]
+            p2 = fp[f2, 0]
+            fe[f1, 1] = e2
+            fp[f1, 1] = p2
+            self.elem_faces[e2, p2] = f1
+            self.elem_side[e2, p2] = 1
+            delete.append(f2)
+            self.periodic[str(b1 + 1)] = str(b2 + 1)
+        for b1, b2 in todo:
+            i1 = self.bdalive.index(b1)
+            del self.bdfaces[i1], self.bdalive[i1]
+            i2 = self.bdalive.index(b2)
+            del self.bdfaces[i2], self.bdalive[i2]
+        if not delete:
+            return self
+        dele = np.sort(np.concatenate(delete))
+        keep = np.ones(self.nfaces, dtype=bool)
+        keep[dele] = False
+        newidx = np.cumsum(keep) - 1
+        self.face_elems = fe[keep]
+        self.face_pos = fp[keep]
+        self._orig_face = self._orig_face[keep]
+        self.nfaces = int(keep.sum())
+        self.bdfaces = [newidx[b] for b in self.bdfaces]
+        self.face_boundary = np.full(self.nfaces, -1, dtype=np.int64)
+        for b, fl in enumerate(self.bdfaces):
+            self.face_boundary[fl] = b
+        # Mesh.jl:308-320: element face indices from the (renumbered) face table
+        f = np.arange(self.nfaces)
+        self.elem_faces[self.face_elems[:, 0], self.face_pos[:, 0]] = f
+        sl = self.face_elems[:, 1] >= 0
+        self.elem_faces[self.face_elems[sl, 1], self.face_pos[sl, 1]] = f[sl]
+        return self
+
+    # ------------------------------------------------------------------ geometry
+    def element_geometry(self, std):
+        """Per-element J, metric (column-major nd x nd) and volume (PhysicalRegions.jl:370-408)."""
+        nd, dx = self.nd, self.Δx
+        pdx = dx[0]
+        for k in range(1, nd):
+            pdx = pdx * dx[k]
+        jac = pdx / (2 ** nd)
+        M = np.zeros((nd, nd))
+        if nd == 1:
+            M[0, 0] = 1.0
+        elif nd == 2:
+            M[0, 0], M[1, 1] = dx[1] / 2, dx[0] / 2
+        else:
+            M[0, 0], M[1, 1], M[2, 2] = dx[1] * dx[2] / 4, dx[0] * dx[2] / 4, dx[0] * dx[1] / 4
+        vol = 0.0
+        for w in std.ω:
+            vol = vol + jac * w
+        ne = self.nelements
+        return (np.full(ne, jac), np.tile(M.T.ravel(), (ne, 1)), np.full(ne, vol))
+
+    def face_geometry(self):
+        """Per-face frames (n, t, b) and surface J from the master position."""
+        nd, dx = self.nd, self.Δx
+        frames = np.zeros((2 * nd, 3, nd))
+        fjac = np.zeros(2 * nd)
+        if nd == 1:
+            frames[0, 0, 0], frames[1, 0, 0] = -1, 1
+            fjac[:] = 1.0
+        elif nd == 2:
+            frames[0, 0] = [-1, 0]; frames[0, 1] = [0, -1]
+            frames[1, 0] = [1, 0]; frames[1, 1] = [0, 1]
+            frames[2, 0] = [0, -1]; frames[2, 1] = [1, 0]
+            frames[3, 0] = [0, 1]; frames[3, 1] = [-1, 0]
+            fjac[0:2], fjac[2:4] = dx[1] / 2, dx[0] / 2
+        else:
+            frames[0] = [[-1, 0, 0], [0, -1, 0], [0, 0, 1]]
+            frames[1] = [[1, 0, 0], [0, 1, 0], [0, 0, 1]]
+            frames[2] = [[0, -1, 0], [0, 0, -1], [1, 0, 0]]
+            frames[3] = [[0, 1, 0], [0, 0, 1], [1, 0, 0]]
+            frames[4] = [[0, 0, -1], [-1, 0, 0], [0, 1, 0]]
+            frames[5] = [[0, 0, 1], [1, 0, 0], [0, 1, 0]]
+            fjac[0:2] = dx[1] * dx[2] / 4
+            fjac[2:4] = dx[0] * dx[2] / 4
+            fjac[4:6] = dx[0] * dx[1] / 4
+        mp = self.face_pos[:, 0]
+        return frames[mp], fjac[mp]
+
+    def node_coords(self, std):
+        """Coordinates of every solution node, (ndofs, nd) (Mesh.jl:380-446)."""
+        nd, n = self.nd, std.n
+        r = (std.basis.ξ + 1.0) / 2.0
+        ne = self.nelements
+        nel = self.nelements_dir
+        eidx = np.arange(ne)
+        ei = [eidx % nel[0]]
+        if nd >= 2:
+            ei.append((eidx // nel[0]) % nel[1])
+        if nd == 3:
+            ei.append(eidx // (nel[0] * nel[1]))
+        # node tensor indices, x fastest
+        nidx = np.arange(n ** nd)
+        ni = [nidx % n, (nidx // n) % n, nidx // (n * n)][:nd]
+        X = np.zeros((ne, n ** nd, nd))
+        for c in range(nd):
+            lo = self.xyz[c][ei[c]][:, None]
+            hi = self.xyz[c][ei[c] + 1][:, None]
+            R = [r[ni[k]][None, :] for k in range(nd)]
+            if nd == 1:
+                X[:, :, c] = lo * (1.0 - R[0]) + hi * R[0]
+            elif nd == 2:
+                V = [lo, hi, hi, lo] if c == 0 else [lo, lo, hi, hi]
+                X[:, :, c] = ((V[0] * (1 - R[0]) * (1 - R[1]) + V[1] * R[0] * (1 - R[1]))
+                              + V[2] * R[0] * R[1]) + V[3] * (1 - R[0]) * R[1]
+            else:
+                if c == 0:
+                    V = [lo, hi, hi, lo, lo, hi, hi, lo]
+                elif c == 1:
+                    V = [lo, lo, hi, hi, lo, lo, hi, hi]
+                else:
+                    V = [lo, lo, lo, lo, hi, hi, hi, hi]
+                x = V[0] * (1 - R[0]) * (1 - R[1]) * (1 - R[2])
+                x = x + V[1] * R[0] * (1 - R[1]) * (1 - R[2])
+                x = x + V[2] * R[0] * R[1] * (1 - R[2])
+                x = x + V[3] * (1 - R[0]) * R[1] * (1 - R[2])
+                x = x + V[4] * (1 - R[0]) * (1 - R[1]) * R[2]
+                x = x + V[5] * R[0] * (1 - R[1]) * R[2]
+                x = x + V[6] * R[0] * R[1] * R[2]
+                x = x + V[7] * (1 - R[0]) * R[1] * R[2]
+                X[:, :, c] = x
+        return X.reshape(ne * n ** nd, nd)
+
+    def face_node_coords(self, std):
+        """Coordinates of the nodes of every face, (nfaces, n^(nd-1), nd)."""
+        nd, n = self.nd, std.n
+        npf = n ** (nd - 1)
+        out = np.zeros((self.nfaces, npf, nd))
+        # a face node coincides with the corresponding trace node of the master element
+        X = self.node_coords(std).reshape(self.nelements, n ** nd, nd)
+        e = self.face_elems[:, 0]
+        pos = self.face_pos[:, 0]
+        for f_ in range(npf):
+            for p in range(2 * nd):
+                sel = pos == p
+                if not np.any(sel):
+                    continue
+                node = _line_node(nd, n, p // 2, f_, 0 if p % 2 == 0 else n - 1)
+                out[sel, f_] = X[e[sel], node]
+        return out
+
+
+def _line_node(nd, n, d, f, p):
+    if nd == 1:
+        return p
+    if nd == 2:
+        return p + n * f if d == 0 else f + n * p
+    fi, fj = f % n, f // n
+    if d == 0:
+        return p + n * (fi + n * fj)
+    if d == 1:
+        return fi + n * (p + n * fj)
+    return fi + n * (fj + n * p)
+
+
+def apply_periodicBCs(mesh, *pairs):
+    return mesh.apply_periodicBCs(*pairs)

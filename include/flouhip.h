@@ -1,0 +1,212 @@
+/*
+ * flouhip.h -- C ABI of libflouhip.so, the MI355X (gfx950) implementation of Flou.jl's
+ * DGSEM right-hand side, its explicit Runge-Kutta stage update, the CFL time-step
+ * reduction, the conservation integral and the adjoint (vector-Jacobian products) of the
+ * rhs and of the RK stage.
+ *
+ * Conventions
+ *  - Plain C types only; device buffers are passed as raw pointers to HIP device memory.
+ *  - State layout is Flou's ODE state matrix: Q is (ndofs, nvars), column-major
+ *    (variable-major SoA), rows grouped per element (row = e*n^d + i, element nodes
+ *    x-fastest).  Element numbering and face numbering are those of the host mesh.
+ *    References: src/FlouSpatial/DofHandler.jl:53-68, src/FlouSpatial/Containers.jl:19-151.
+ *  - The scalar type of every state buffer is the handle's precision: double for
+ *    FLOU_HIP_FP64, float for FLOU_HIP_FP32.  Reductions always return FP64.
+ *  - Every call that launches work is asynchronous on the handle's stream; outputs are
+ *    valid once the stream is synchronised.  A handle is not thread safe.
+ *  - Return codes: 0 on success, negative FLOU_HIP_E* on failure; flou_hip_last_error()
+ *    gives the message.
+ */
+#ifndef FLOUHIP_H
+#define FLOUHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct flou_hip_handle flou_hip_handle;
+
+/* Status codes */
+enum {
+    FLOU_HIP_OK = 0,
+    FLOU_HIP_EINVAL = -1,       /* invalid argument (Julia: ArgumentError) */
+    FLOU_HIP_EUNSUPPORTED = -2, /* valid in Flou but not implemented on the GPU path */
+    FLOU_HIP_EDEVICE = -3,      /* HIP runtime error */
+    FLOU_HIP_ENOMEM = -4,       /* device allocation failed */
+    FLOU_HIP_EDOMAIN = -5       /* non-physical state met (Julia: DomainError) */
+};
+
+/* Precision of the state and of the bulk arithmetic */
+enum { FLOU_HIP_FP64 = 0, FLOU_HIP_FP32 = 1 };
+
+/* Equations: LinearAdvection{ND} and EulerEquation{ND} */
+enum { FLOU_HIP_ADVECTION = 0, FLOU_HIP_EULER = 1 };
+
+/* Volume operators: StrongDivOperator, SplitDivOperator */
+enum { FLOU_HIP_STRONG = 0, FLOU_HIP_SPLIT = 1 };
+
+/* Two-point volume fluxes: StdAverage, ChandrasekharAverage, Kennedy-Gruber */
+enum { FLOU_HIP_TP_STDAVG = 0, FLOU_HIP_TP_CHANDRASEKHAR = 1, FLOU_HIP_TP_KENNEDY_GRUBER = 2 };
+
+/* Surface numerical flux = average + dissipation:
+ *   StdAverage()                      -> (AVG_STD, DISS_NONE)
+ *   LxF(StdAverage(), c)              -> (AVG_STD, DISS_LXF), intensity c
+ *   MatrixDissipation(Chandrasekhar(), c) -> (AVG_CHANDRASEKHAR, DISS_MATRIX), intensity c */
+enum { FLOU_HIP_AVG_STD = 0, FLOU_HIP_AVG_CHANDRASEKHAR = 1 };
+enum { FLOU_HIP_DISS_NONE = 0, FLOU_HIP_DISS_LXF = 1, FLOU_HIP_DISS_MATRIX = 2 };
+
+/* Boundary conditions (applyBC!, src/FlouSpatial/Interfaces.jl:25-49;
+ * src/FlouSpatial/Equations/Euler.jl:69-94) */
+enum {
+    FLOU_HIP_BC_DIRICHLET = 0, /* GenericBC with a time-independent Qext(x): per face-node state */
+    FLOU_HIP_BC_INFLOW = 1,    /* EulerInflowBC(Qext) */
+    FLOU_HIP_BC_OUTFLOW = 2,   /* EulerOutflowBC() */
+    FLOU_HIP_BC_SLIP = 3       /* EulerSlipBC() */
+};
+
+/* Explicit RK schemes (OrdinaryDiffEq 6.49.1 perform_step! forms) */
+enum { FLOU_HIP_SSPRK33 = 0, FLOU_HIP_CK2N54 = 1, FLOU_HIP_ORK256 = 2 };
+
+/* Geometry storage */
+enum {
+    FLOU_HIP_GEOM_AFFINE = 0, /* metric, J constant per element; frames, jac constant per face */
+    FLOU_HIP_GEOM_CURVED = 1  /* metric, J per node; frames, jac per face node */
+};
+
+/*
+ * Flattened MultielementDisc + equation + operator
+ * (src/FlouSpatial/MultielementDiscontinuous.jl:16-92).  All indices are 0-based.
+ * The library copies everything it needs; the caller may free the arrays after create.
+ */
+typedef struct flou_hip_desc {
+    int32_t nd;         /* spatial dimension, 1..3 */
+    int32_t n;          /* nodes per direction (polynomial order + 1), 2..8 */
+    int32_t nvars;      /* 1 (advection) or nd + 2 (Euler) */
+    int32_t precision;  /* FLOU_HIP_FP64 | FLOU_HIP_FP32 */
+    int64_t nelements;
+    int64_t nfaces;
+    int32_t nboundaries;
+
+    int32_t equation;   /* FLOU_HIP_ADVECTION | FLOU_HIP_EULER */
+    double gamma;       /* EulerEquation.γ */
+    double velocity[3]; /* LinearAdvection.a */
+
+    int32_t volume_op;  /* FLOU_HIP_STRONG | FLOU_HIP_SPLIT */
+    int32_t tp_flux;    /* two-point flux of SplitDivOperator */
+    int32_t surface_avg;
+    int32_t surface_diss;
+    double intensity;
+
+    /* 1-D operators of the standard element (StdSegment.jl:76-89), FP64.
+     * Matrices row-major n x n: M[i*n + j] = M_ij. */
+    const double* D;
+    const double* Ds;
+    const double* Dsharp;
+    const double* l_minus;   /* std.l[1]: interpolation at ξ = -1 */
+    const double* l_plus;    /* std.l[2] */
+    const double* dg_minus;  /* std.∂g[1] */
+    const double* dg_plus;   /* std.∂g[2] */
+    const double* omega;     /* std.ω (1-D) */
+
+    /* Connectivity (src/FlouCommon/Mesh.jl:19-113, after apply_periodicBCs!) */
+    const int64_t* elem_faceinds;  /* nelements * 2nd: mesh.elements[ie].faceinds - 1 */
+    const int8_t* elem_facepos;    /* nelements * 2nd: side of the element on that face (0 = master) */
+    const int64_t* face_eleminds;  /* nfaces * 2: eleminds - 1 (-1: boundary exterior) */
+    const int8_t* face_elempos;    /* nfaces * 2: elempos - 1 (-1: boundary exterior) */
+    const uint8_t* face_orientation; /* nfaces, NULL = all 0 */
+    const int32_t* face_boundary;  /* nfaces: boundary index of boundary faces, else -1 */
+
+    /* Geometry (src/FlouSpatial/PhysicalRegions.jl:306-364, 485-535) */
+    int32_t geometry;          /* FLOU_HIP_GEOM_AFFINE | FLOU_HIP_GEOM_CURVED */
+    const double* elem_jac;    /* AFFINE: nelements; CURVED: ndofs */
+    const double* elem_metric; /* AFFINE: nelements*nd*nd; CURVED: ndofs*nd*nd.  Each block
+                                  column-major: m[r + nd*c] = metric[r, c] (Ja^c in column c) */
+    const double* elem_volume; /* nelements */
+    const double* face_frames; /* AFFINE: nfaces*3*nd; CURVED: nfacedofs*3*nd; (n, t, b) */
+    const double* face_jac;    /* AFFINE: nfaces; CURVED: nfacedofs */
+} flou_hip_desc;
+
+/* Library version string. */
+const char* flou_hip_version(void);
+
+/* Replaces MultielementDisc construction for the GPU (MultielementDiscontinuous.jl:29-92)
+ * and HyperbolicCache / construct_cache (Equations/Hyperbolic.jl:16-29): validates the
+ * description, uploads operators, connectivity and geometry, and builds the per-element
+ * face tables.  Binds the handle to the current device and the default stream. */
+int flou_hip_create(const flou_hip_desc* desc, flou_hip_handle** out);
+int flou_hip_destroy(flou_hip_handle* h);
+const char* flou_hip_last_error(const flou_hip_handle* h);
+/* Message of the last failed flou_hip_create on this thread (the constructor ArgumentErrors
+ * of MultielementDiscontinuous.jl:42-44 and OpDivergence.jl:211-214 map here). */
+const char* flou_hip_create_error(void);
+
+/* Options.  0: reproduce the 3-D ChandrasekharAverage surface-flux term of
+ * src/FlouSpatial/Equations/Euler.jl:216, which squares wl where wr is meant (default 0:
+ * the corrected flux). */
+int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value);
+
+/* Stream the handle launches on (hipStream_t; NULL = default stream). */
+int flou_hip_set_stream(flou_hip_handle* h, void* stream);
+
+/* Boundary condition of boundary `ib` (disc.bcs, MultielementDiscontinuous.jl:40-51).
+ * DIRICHLET: `data` = nfaces * n^(nd-1) * nvars host doubles (only faces of boundary ib are
+ * read); INFLOW: `data` = nvars host doubles; OUTFLOW, SLIP: data ignored. */
+int flou_hip_set_bc(flou_hip_handle* h, int32_t ib, int32_t type, const double* data);
+
+/* Number of DOFs (nodes) and bytes of one state buffer. */
+int64_t flou_hip_ndofs(const flou_hip_handle* h);
+size_t flou_hip_state_bytes(const flou_hip_handle* h);
+
+/* Device state helpers for hosts without their own device allocator (the Julia binding). */
+int flou_hip_alloc(flou_hip_handle* h, void** dptr, size_t bytes);
+int flou_hip_free(flou_hip_handle* h, void* dptr);
+int flou_hip_upload(flou_hip_handle* h, void* dst_device, const void* src_host, size_t bytes);
+int flou_hip_download(flou_hip_handle* h, void* dst_host, const void* src_device, size_t bytes);
+int flou_hip_synchronize(flou_hip_handle* h);
+
+/* rhs!(dQ, Q, EquationConfig(disc, eq), t) -- src/FlouSpatial/Equations/Hyperbolic.jl:31-69.
+ * dQ is fully overwritten; Q is read only.  dQ and Q must not overlap. */
+int flou_hip_rhs(flou_hip_handle* h, void* dQ, const void* Q, double t);
+
+/* One fused RK stage = rhs of the stage input + the stage combination
+ * (OrdinaryDiffEq perform_step!, called from FlouTime.timeintegrate, FlouTime.jl:34-56).
+ *   SSPRK33, stage 0: u_out = u_in + dt*L(u_in)                          (u_base unused)
+ *            stage 1: u_out = (3*u_base + u_in + dt*L(u_in)) / 4
+ *            stage 2: u_out = (u_base + 2*u_in + 2*dt*L(u_in)) / 3       (u_out may be u_base)
+ *   CK2N54 / ORK256, stage s: reg = A_s*reg + dt*L(u_in) (reg = dt*L(u_in) at s = 0);
+ *            u_out = u_in + B_s*reg                                       (u_base unused)
+ * u_out must not overlap u_in (neighbouring elements read u_in). */
+int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double dt, double t,
+                      const void* u_in, const void* u_base, void* reg, void* u_out);
+
+/* Full fixed-dt time integration of nsteps steps in place on Q, with handle-owned work
+ * buffers (timeintegrate(Q0, disc, eq, solver, tf; dt, adaptive=false), FlouTime.jl:34-56).
+ * If cfl > 0, dt is recomputed before every step as min(get_max_dt(Q, cfl), dt) (the
+ * get_cfl_callback pattern, FlouTime.jl:92-104) and `dt` acts as Δtmax.
+ * Returns FLOU_HIP_EDOMAIN if a non-physical state was met ("Simulation crashed!"). */
+int flou_hip_timeintegrate(flou_hip_handle* h, int32_t scheme, void* Q, double dt,
+                           int64_t nsteps, double cfl, double* t_inout);
+
+/* get_max_dt(Q, disc, eq, cfl) -- MultielementDiscontinuous.jl:162-178 (FP64 result). */
+int flou_hip_max_dt(flou_hip_handle* h, const void* Q, double cfl, double* out);
+
+/* integrate(Q, disc, ivar) -- MultielementDiscontinuous.jl:148-160 (FP64 result); ivar
+ * 0-based. */
+int flou_hip_integrate(flou_hip_handle* h, const void* Q, int32_t ivar, double* out);
+
+/* Synchronises, returns FLOU_HIP_EDOMAIN if a kernel met a non-physical state since the
+ * last call (sqrt or log of a negative number in the reference), and clears the flag. */
+int flou_hip_check(flou_hip_handle* h);
+
+/* Adjoint of rhs!: Qbar += (d rhs / dQ)^T dQbar, evaluated at Q.  No reference counterpart
+ * (Flou has no AD rule); it backs the rrule of rhs!. */
+int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void* Qbar);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FLOUHIP_H */

@@ -683,4 +683,46 @@ int oracle_rk_run_jvp(oracle_handle* h, int scheme, double* u, double* du, doubl
     return 0;
 }
 
+// Point-wise exports for closed-form checks of the flux formulas.
+double oracle_logmean(double a, double b) { return logarithmic_mean(a, b); }
+
+int oracle_twopoint(oracle_handle* h, const double* Q1, const double* Q2, const double* Ja1,
+                    const double* Ja2, double* F) {
+    try {
+        twopointflux(h->d, Q1, Q2, Ja1, Ja2, F);
+    } catch (std::domain_error& e) {
+        h->err = e.what();
+        return -5;
+    }
+    return 0;
+}
+
+// Numerical flux through a face with frame (n, t, b): rotate, flux, rotate back (no jac).
+int oracle_numflux(oracle_handle* h, const double* Ql, const double* Qr, const double* frame,
+                   double* F) {
+    const Disc& d = h->d;
+    try {
+        if (d.eq == EQ_ADVECTION) {
+            double an = d.a[0] * frame[0];
+            for (int k = 1; k < d.nd; ++k) an = an + d.a[k] * frame[k];
+            F[0] = (an * (Ql[0] + Qr[0])) / 2.0;
+            if (d.sdiss == DISS_LXF) F[0] = F[0] + ((std::fabs(an) * (Ql[0] - Qr[0])) / 2.0) * d.intensity;
+        } else {
+            double Rl[5], Rr[5], Fr[5];
+            rotate2face(d.nd, Ql, frame, Rl);
+            rotate2face(d.nd, Qr, frame, Rr);
+            euler_numericalflux(d, Rl, Rr, Fr);
+            rotate2phys(d.nd, Fr, frame, F);
+        }
+    } catch (std::domain_error& e) {
+        h->err = e.what();
+        return -5;
+    }
+    return 0;
+}
+
+void oracle_cons2entropy(int nd, double gamma, const double* Q, double* W) {
+    cons2entropy(Q, nd, gamma, W);
+}
+
 }  // extern "C"

@@ -15,6 +15,7 @@
 #pragma once
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -544,9 +545,13 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
         }
     }
 
+    // Domain errors (sqrt/log of a negative) cannot leave an OpenMP region: record and
+    // rethrow after the loop.
+    std::atomic<int> domain_failed(0);
+
     // volume_contribution! (OpDivergence.jl:109-160, 200-282)
 #pragma omp parallel for schedule(static)
-    for (long e = 0; e < ne; ++e) {
+    for (long e = 0; e < ne; ++e) try {
         const long base = e * npe;
         const double* M = &d.e_metric[(size_t)e * nd * nd];
         std::vector<T> Qe((size_t)npe * nv), Ft((size_t)npe * nd * nv), acc((size_t)npe * nv);
@@ -602,7 +607,10 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
         }
         for (int i = 0; i < npe; ++i)
             for (int v = 0; v < nv; ++v) dQ[(size_t)v * N + base + i] = acc[i * nv + v];
+    } catch (std::domain_error&) {
+        domain_failed = 1;
     }
+    if (domain_failed) throw std::domain_error("DomainError in the volume term");
 
     // applyBCs! (Interfaces.jl:25-49)
     for (long face = 0; face < nf; ++face) {
@@ -612,7 +620,7 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
 
     // interface_fluxes! (Interfaces.jl:111-136); Cartesian orientation is 0 (j == i)
 #pragma omp parallel for schedule(static)
-    for (long face = 0; face < nf; ++face) {
+    for (long face = 0; face < nf; ++face) try {
         const double* fr = &d.f_frame[(size_t)face * 3 * nd];
         const double jac = d.f_jac[face];
         for (int f = 0; f < npf; ++f) {
@@ -640,7 +648,10 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
                 F2[v] = -F1[v];
             }
         }
+    } catch (std::domain_error&) {
+        domain_failed = 1;
     }
+    if (domain_failed) throw std::domain_error("DomainError in the interface fluxes");
 
     // surface_contribution! (OpDivergence.jl:42-100)
 #pragma omp parallel for schedule(static)

@@ -1,0 +1,40 @@
+"""Shared builders for the tests: the same discretisation on the oracle and on the GPU."""
+import numpy as np
+
+import oracle_lib as O
+
+GAMMA = 1.4
+
+
+def random_euler_state(X, nd, seed, amp=0.1, smooth=True):
+    """Seeded positive state (ndofs, nd+2): ρ ≈ 1 ± amp, |u| ≲ amp, p ≈ 1 ± amp."""
+    rng = np.random.default_rng(seed)
+    nn = X.shape[0]
+    prim = np.zeros((nn, nd + 2))
+    if smooth:
+        k = rng.uniform(1, 3, size=(nd + 2, nd))
+        ph = rng.uniform(0, 2 * np.pi, size=(nd + 2,))
+        for v in range(nd + 2):
+            prim[:, v] = np.sin(X @ k[v] + ph[v])
+        prim += 0.1 * rng.standard_normal(prim.shape) * amp
+    else:
+        prim = rng.uniform(-1, 1, size=(nn, nd + 2))
+    prim[:, 0] = 1.0 + amp * prim[:, 0]
+    prim[:, 1:nd + 1] *= amp
+    prim[:, nd + 1] = 1.0 + amp * prim[:, nd + 1]
+    Q = np.zeros_like(prim)
+    Q[:, 0] = prim[:, 0]
+    u2 = prim[:, 1] * prim[:, 1]
+    for k in range(1, nd):
+        u2 = u2 + prim[:, 1 + k] * prim[:, 1 + k]
+    for k in range(nd):
+        Q[:, 1 + k] = prim[:, 0] * prim[:, 1 + k]
+    Q[:, nd + 1] = prim[:, nd + 1] / (GAMMA - 1) + prim[:, 0] * u2 / 2
+    return np.asfortranarray(Q)
+
+
+def oracle_for(nd, n, nel, start, finish, periodic, eq, eqparams, volop, tpflux, savg, sdiss,
+               intensity=1.0, nodes="GLL"):
+    ops = O.build_ops(n, nodes)
+    return O.OracleDisc(nd, n, nel, start, finish, periodic, ops, eq, eqparams, volop, tpflux,
+                        savg, sdiss, intensity)

@@ -56,6 +56,11 @@ def lib():
         L.oracle_rhs_jvp.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_rk_run_jvp.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp,
                                         ctypes.c_double, ctypes.c_int]
+        L.oracle_logmean.restype = ctypes.c_double
+        L.oracle_logmean.argtypes = [ctypes.c_double, ctypes.c_double]
+        L.oracle_twopoint.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_numflux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_cons2entropy.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_last_error.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -167,6 +172,18 @@ class OracleDisc:
         self._check(lib().oracle_rhs_jvp(self.h, _p(Q), _p(dr), _p(out), _p(dout)))
         return out, dout
 
+    def twopoint(self, Q1, Q2, Ja1, Ja2):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (Q1, Q2, Ja1, Ja2)]
+        F = np.zeros(self.nv)
+        self._check(lib().oracle_twopoint(self.h, *[_p(x) for x in a], _p(F)))
+        return F
+
+    def numflux(self, Ql, Qr, frame):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (Ql, Qr, frame)]
+        F = np.zeros(self.nv)
+        self._check(lib().oracle_numflux(self.h, *[_p(x) for x in a], _p(F)))
+        return F
+
     def max_dt(self, Q, cfl):
         Q = np.asfortranarray(Q, dtype=np.float64)
         return lib().oracle_max_dt(self.h, _p(Q), float(cfl))
@@ -185,6 +202,17 @@ class OracleDisc:
         du = np.array(dQ, dtype=np.float64, order="F", copy=True)
         self._check(lib().oracle_rk_run_jvp(self.h, scheme, _p(u), _p(du), float(dt), int(nsteps)))
         return u, du
+
+
+def logmean(a, b):
+    return lib().oracle_logmean(float(a), float(b))
+
+
+def cons2entropy(nd, gamma, Q):
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    W = np.zeros(nd + 2)
+    lib().oracle_cons2entropy(nd, float(gamma), _p(Q), _p(W))
+    return W
 
 
 def tableau_2n(scheme):

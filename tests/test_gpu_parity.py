@@ -1,0 +1,275 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+FP64 plain form: same arithmetic as the oracle term by term, so the bar is 1e-13 relative
+to the field's max norm, and most cases must be bit-identical (reported per case).
+FP32 chosen form: error against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import flouhip as F
+import oracle_lib as O
+from cases import GAMMA, oracle_for, random_euler_state
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+SURF = {
+    "lxf": (lambda: F.LxF(F.StdAverage(), 1.0), O.AVG_STD, O.DISS_LXF),
+    "avg": (lambda: F.StdAverage(), O.AVG_STD, O.DISS_NONE),
+    "matrix": (lambda: F.MatrixDissipation(F.ChandrasekharAverage(), 1.0), O.AVG_CHANDRASEKHAR, O.DISS_MATRIX),
+}
+TP = {"std": (F.StdAverage, O.TP_STDAVG), "ch": (F.ChandrasekharAverage, O.TP_CHANDRASEKHAR),
+      "kg": (F.KennedyGruber, O.TP_KENNEDY_GRUBER)}
+
+
+def build(nd, n, nel, vol, tp="std", surf="lxf", periodic=None, nodes="GLL", eq="euler",
+          precision="fp64", velocity=None, finish=None):
+    periodic = [1] * nd if periodic is None else periodic
+    start = [0.0] * nd
+    finish = finish or [1.0, 1.5, 2.0][:nd]
+    b = F.LagrangeBasis(nodes, n)
+    nv = nd + 2 if eq == "euler" else 1
+    std = [None, F.StdSegment, F.StdQuad, F.StdHex][nd](b, F.DGSEMrec(b), nv)
+    mesh = F.CartesianMesh(nd, start, finish, nel)
+    mesh.apply_periodicBCs(*[(str(2 * k + 1), str(2 * k + 2)) for k in range(nd) if periodic[k]])
+    sf, savg, sdiss = SURF[surf]
+    if eq == "euler":
+        equation = F.EulerEquation(nd, GAMMA)
+        eqc, eqp = O.EQ_EULER, [GAMMA]
+    else:
+        velocity = velocity or [1.0, 0.5, -0.25][:nd]
+        equation = F.LinearAdvection(*velocity)
+        eqc, eqp = O.EQ_ADVECTION, velocity
+    if vol == "strong":
+        op = F.StrongDivOperator(sf())
+        volc, tpc = O.VOL_STRONG, 0
+    else:
+        op = F.SplitDivOperator(TP[tp][0](), sf())
+        volc, tpc = O.VOL_SPLIT, TP[tp][1]
+    return mesh, std, equation, op, oracle_for(nd, n, nel, start, finish, periodic, eqc, eqp, volc,
+                                               tpc, savg, sdiss, 1.0, nodes=nodes)
+
+
+def rel_err(a, b):
+    scale = np.max(np.abs(b), axis=0)
+    scale[scale == 0] = 1
+    return np.max(np.abs(a - b) / scale)
+
+
+RHS_CASES = [
+    # nd, n, nel, vol, tp, surf, eq, nodes
+    (1, 4, [7], "strong", "std", "lxf", "euler", "GLL"),
+    (1, 6, [5], "split", "ch", "lxf", "euler", "GLL"),
+    (1, 4, [6], "split", "ch", "matrix", "euler", "GLL"),
+    (2, 3, [4, 3], "strong", "std", "lxf", "euler", "GLL"),
+    (2, 8, [3, 2], "split", "kg", "lxf", "euler", "GLL"),
+    (2, 5, [3, 4], "split", "ch", "avg", "euler", "GLL"),
+    (2, 4, [3, 3], "split", "std", "lxf", "euler", "GLL"),
+    (2, 4, [3, 3], "split", "ch", "matrix", "euler", "GLL"),
+    (3, 2, [3, 2, 2], "strong", "std", "lxf", "euler", "GLL"),
+    (3, 6, [2, 3, 2], "split", "ch", "lxf", "euler", "GLL"),
+    (3, 5, [2, 2, 2], "strong", "std", "lxf", "euler", "GLL"),
+    (3, 7, [2, 2, 1], "split", "kg", "lxf", "euler", "GLL"),
+    (3, 8, [1, 2, 2], "split", "std", "avg", "euler", "GLL"),
+    (3, 4, [2, 2, 2], "split", "ch", "matrix", "euler", "GLL"),
+    (2, 5, [4, 4], "strong", "std", "lxf", "adv", "GLL"),
+    (2, 5, [4, 4], "split", "std", "lxf", "adv", "GLL"),
+    (2, 5, [4, 3], "strong", "std", "lxf", "adv", "GL"),
+    (3, 4, [2, 2, 2], "strong", "std", "lxf", "adv", "GL"),
+    (3, 3, [2, 2, 2], "strong", "std", "lxf", "euler", "GL"),
+]
+
+
+def _state(disc, orc, nd, eq, seed, amp=0.1):
+    X, _ = orc.coords()
+    if eq == "euler":
+        return random_euler_state(X, nd, seed=seed, amp=amp)
+    rng = np.random.default_rng(seed)
+    return np.asfortranarray(np.sin(X @ rng.uniform(1, 3, nd))[:, None] + 0.01 * rng.standard_normal((X.shape[0], 1)))
+
+
+@pytest.mark.parametrize("case", RHS_CASES, ids=lambda c: "-".join(str(x) for x in c))
+def test_rhs_fp64_matches_oracle(case):
+    nd, n, nel, vol, tp, surf, eq, nodes = case
+    mesh, std, equation, op, orc = build(nd, n, nel, vol, tp, surf, nodes=nodes, eq=eq)
+    disc = F.MultielementDisc(mesh, std, equation, op, precision="fp64")
+    Q = _state(disc, orc, nd, eq, seed=17, amp=0.3)
+    ref = orc.rhs(Q)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation), 0.0)
+    disc.check()
+    got = disc.to_host(dQd)
+    same = np.mean(got == ref)
+    print(f"{case}: bit-identical fraction {same:.4f}, max rel err {rel_err(got, ref):.2e}")
+    assert rel_err(got, ref) <= 1e-13
+    assert same >= 0.99
+
+
+def test_rhs_strong_form_is_bit_identical():
+    """No log is evaluated on this path and sqrt/div are correctly rounded on both sides:
+    the GPU must reproduce the oracle exactly."""
+    mesh, std, equation, op, orc = build(3, 5, [3, 2, 2], "strong", surf="lxf")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = _state(disc, orc, 3, "euler", seed=3)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+    assert np.array_equal(disc.to_host(dQd), orc.rhs(Q))
+
+
+@pytest.mark.parametrize("solver,scheme", [(F.SSPRK33, O.RK_SSPRK33), (F.CarpenterKennedy2N54, O.RK_CK2N54),
+                                           (F.ORK256, O.RK_ORK256)])
+def test_fused_rk_steps_match_oracle(solver, scheme):
+    mesh, std, equation, op, orc = build(2, 5, [4, 3], "split", "ch", "lxf")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = _state(disc, orc, 2, "euler", seed=5)
+    dt, steps = 2e-3, 4
+    ref = orc.rk_run(scheme, Q, dt, steps)
+    Qd = disc.to_device(Q)
+    F.timeintegrate(Qd, disc, equation, solver(), steps * dt, dt=dt)
+    got = disc.to_host(Qd)
+    print(f"{solver.__name__}: bit-identical {np.mean(got == ref):.4f}, rel err {rel_err(got - Q, ref - Q):.2e}")
+    assert rel_err(got - Q, ref - Q) <= 1e-12
+
+
+def test_rk_stage_api_and_aliasing():
+    mesh, std, equation, op, orc = build(2, 4, [3, 3], "strong")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = disc.to_device(_state(disc, orc, 2, "euler", seed=2))
+    out, reg = disc.new_state(), disc.new_state()
+    with pytest.raises(ValueError):
+        F.rk_stage(disc, F.CarpenterKennedy2N54.code, 0, 1e-3, Q, Q, reg=reg)
+    with pytest.raises(ValueError):
+        F.rk_stage(disc, F.CarpenterKennedy2N54.code, 0, 1e-3, Q, out)  # no register
+    F.rk_stage(disc, F.CarpenterKennedy2N54.code, 0, 1e-3, Q, out, reg=reg)
+    # stage 0 of a 2N scheme: reg = dt*L(Q), out = Q + B1*reg
+    dQ = disc.new_state()
+    F.rhs_(dQ, Q, F.EquationConfig(disc, equation))
+    A, B, _ = O.tableau_2n(O.RK_CK2N54)
+    r = disc.to_host(reg)
+    assert np.array_equal(r, 1e-3 * disc.to_host(dQ))
+    assert np.array_equal(disc.to_host(out), disc.to_host(Q) + B[0] * r)
+
+
+def test_max_dt_and_integrate():
+    mesh, std, equation, op, orc = build(3, 4, [3, 2, 2], "split", "ch")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = _state(disc, orc, 3, "euler", seed=9)
+    Qd = disc.to_device(Q)
+    assert F.get_max_dt(Qd, disc, equation, 0.3) == orc.max_dt(Q, 0.3)
+    for v in range(5):
+        assert F.integrate(Qd, disc, v) == pytest.approx(orc.integrate(Q, v), rel=1e-14, abs=1e-15)
+
+
+def test_timeintegrate_with_cfl_matches_oracle_dt():
+    mesh, std, equation, op, orc = build(2, 4, [4, 4], "split", "ch")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = _state(disc, orc, 2, "euler", seed=21)
+    dt = orc.max_dt(Q, 0.2)
+    ref = orc.rk_run(O.RK_SSPRK33, Q, dt, 1)
+    Qd = disc.to_device(Q)
+    F.timeintegrate(Qd, disc, equation, F.SSPRK33(), dt, dt=1.0, cfl=0.2)
+    assert rel_err(disc.to_host(Qd) - Q, ref - Q) <= 1e-12
+
+
+def test_sod_tube_on_gpu_matches_reference_golden():
+    """SodTube1D (test/tests.jl:90-134) through the product API -> runtests.jl:37-38."""
+    g = GAMMA
+    eq = F.EulerEquation(1, g)
+    b = F.LagrangeBasis("GLL", 4)
+    std = F.StdSegment(b, F.DGSEMrec(b), 3)
+    mesh = F.CartesianMesh(1, [0.0], [1.0], [20])
+
+    def Qext(_, x, __, ___, eq_):
+        P = (1.0, 0.0, 100.0) if x[0] < 0.5 else (0.125, 0.0, 10.0)
+        return F.vars_prim2cons(np.array(P), eq_)
+
+    bcs = {"1": F.GenericBC(Qext), "2": F.GenericBC(Qext)}
+    op = F.SplitDivOperator(F.MatrixDissipation(F.ChandrasekharAverage(), 1.0))
+    disc = F.MultielementDisc(mesh, std, eq, op, bcs)
+    X = mesh.node_coords(std)
+    Q = np.array([Qext(None, x, None, 0.0, eq) for x in X])
+    Qd = disc.to_device(Q)
+    F.timeintegrate(Qd, disc, eq, F.ORK256(williamson_condition=False), 0.018, dt=1e-4)
+    u = disc.to_host(Qd)
+    assert u.min() == pytest.approx(-0.1662939230897596, rel=1e-7)
+    assert u.max() == pytest.approx(254.90504152149907, rel=1e-7)
+    print("Sod GPU min/max rel dev:", abs(u.min() + 0.1662939230897596) / 0.1662939230897596,
+          abs(u.max() - 254.90504152149907) / 254.90504152149907)
+
+
+def test_advection_2d_full_period_gauss_nodes():
+    """Advection2D (test/tests.jl:53-88) on the GPU: Gauss nodes use the projection path."""
+    eq = F.LinearAdvection(3.0, 4.0)
+    b = F.LagrangeBasis("GL", 5)
+    std = F.StdQuad(b, F.DGSEMrec(b), 1)
+    mesh = F.CartesianMesh(2, (0, 0), (1.5, 2), (20, 10)).apply_periodicBCs(("1", "2"), ("3", "4"))
+    disc = F.MultielementDisc(mesh, std, eq, F.StrongDivOperator(F.LxF(F.StdAverage(), 1.0)))
+    X = mesh.node_coords(std)
+    Q0 = np.exp(-(X[:, 0] - 0.75) ** 2 / (2 * 0.2 ** 2) - (X[:, 1] - 1.0) ** 2 / (2 * 0.2 ** 2))[:, None]
+    Qd = disc.to_device(Q0)
+    F.timeintegrate(Qd, disc, eq, F.ORK256(), 0.5, dt=1e-3)
+    u = disc.to_host(Qd)
+    assert np.linalg.norm(u - Q0) <= 1e-3 * max(np.linalg.norm(u), np.linalg.norm(Q0))
+
+
+def test_fp32_rhs_error_against_fp64_oracle():
+    mesh, std, equation, op, orc = build(3, 6, [2, 2, 2], "split", "ch", finish=[2 * np.pi] * 3)
+    disc = F.MultielementDisc(mesh, std, equation, op, precision="fp32")
+    Q = _state(disc, orc, 3, "euler", seed=4)
+    ref = orc.rhs(Q)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+    got = disc.to_host(dQd).astype(np.float64)
+    err = rel_err(got, ref)
+    print(f"FP32 rhs max rel err vs FP64 oracle: {err:.2e}")
+    assert err < 1e-4
+
+
+def test_domain_error_on_negative_pressure():
+    mesh, std, equation, op, orc = build(2, 4, [3, 2], "strong")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Q = _state(disc, orc, 2, "euler", seed=1)
+    Q[0, 3] = -1.0
+    with pytest.raises(ArithmeticError):
+        orc.rhs(Q)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+    with pytest.raises(F.DomainError):
+        disc.check()
+    disc.check()  # the flag is cleared
+
+
+def test_rhs_is_deterministic():
+    mesh, std, equation, op, orc = build(3, 6, [2, 2, 2], "split", "ch")
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    Qd = disc.to_device(_state(disc, orc, 3, "euler", seed=8))
+    a, b = disc.new_state(), disc.new_state()
+    cfg = F.EquationConfig(disc, equation)
+    F.rhs_(a, Qd, cfg)
+    F.rhs_(b, Qd, cfg)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+
+
+def test_non_periodic_boundaries_match_oracle():
+    """Inflow / outflow / slip boundaries on a 2-D box (Equations/Euler.jl:69-94)."""
+    mesh, std, equation, op, orc = build(2, 4, [3, 3], "split", "ch", periodic=[0, 0])
+    qin = np.array([1.1, 0.3, 0.0, 2.7])
+    bcs = {"1": F.EulerInflowBC(qin), "2": F.EulerOutflowBC(), "3": F.EulerSlipBC(), "4": F.EulerSlipBC()}
+    disc = F.MultielementDisc(mesh, std, equation, op, bcs)
+    orc.set_bc(0, O.BC_INFLOW, qin)
+    orc.set_bc(1, O.BC_OUTFLOW)
+    orc.set_bc(2, O.BC_SLIP)
+    orc.set_bc(3, O.BC_SLIP)
+    Q = _state(disc, orc, 2, "euler", seed=12)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+    got, ref = disc.to_host(dQd), orc.rhs(Q)
+    assert rel_err(got, ref) <= 1e-13
