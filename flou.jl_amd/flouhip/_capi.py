@@ -78,11 +78,29 @@ SYMBOLS = {
 }
 
 
+def _share_torch_hip_runtime():
+    """Load torch's HIP runtime first.
+
+    torch ships its own libamdhip64.so (same soname, libamdhip64.so.7, as /opt/rocm's).  A
+    process must hold a single HIP/HSA runtime, so when torch provides the device buffers
+    libflouhip must bind to torch's copy: loading it globally before libflouhip makes the
+    dynamic linker resolve libflouhip's DT_NEEDED entry to it."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        return
+    libdir = os.path.join(os.path.dirname(torch.__file__), "lib")
+    hip = os.path.join(libdir, "libamdhip64.so")
+    if os.path.exists(hip):
+        ctypes.CDLL(hip, mode=ctypes.RTLD_GLOBAL)
+
+
 def _load():
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"libflouhip.so not found at {LIB_PATH}; build it with "
             "`python flou.jl_amd/build_lib.py` (there is no CPU fallback)")
+    _share_torch_hip_runtime()
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SYMBOLS.items():
         fn = getattr(lib, name)

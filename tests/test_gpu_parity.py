@@ -96,16 +96,21 @@ def test_rhs_fp64_matches_oracle(case):
     mesh, std, equation, op, orc = build(nd, n, nel, vol, tp, surf, nodes=nodes, eq=eq)
     disc = F.MultielementDisc(mesh, std, equation, op, precision="fp64")
     Q = _state(disc, orc, nd, eq, seed=17, amp=0.3)
-    ref = orc.rhs(Q)
-    Qd = disc.to_device(Q)
-    dQd = disc.new_state()
-    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation), 0.0)
-    disc.check()
-    got = disc.to_host(dQd)
-    same = np.mean(got == ref)
-    print(f"{case}: bit-identical fraction {same:.4f}, max rel err {rel_err(got, ref):.2e}")
-    assert rel_err(got, ref) <= 1e-13
-    assert same >= 0.99
+    for typo in ((0, 1) if surf == "matrix" and nd == 3 else (0,)):
+        # 3-D ChandrasekharAverage surface flux with and without the Euler.jl:216 typo
+        orc.set_option(0, typo)
+        disc.set_option(0, typo)
+        ref = orc.rhs(Q)
+        Qd = disc.to_device(Q)
+        dQd = disc.new_state()
+        F.rhs_(dQd, Qd, F.EquationConfig(disc, equation), 0.0)
+        disc.check()
+        got = disc.to_host(dQd)
+        same = np.mean(got == ref)
+        print(f"{case} typo={typo}: bit-identical fraction {same:.4f}, max rel err {rel_err(got, ref):.2e}")
+        assert rel_err(got, ref) <= 1e-13
+        # only the log branch of logarithmic_mean (libm vs device log, 1 ulp) may differ
+        assert same >= (0.9 if (tp == "ch" or surf == "matrix") else 1.0)
 
 
 def test_rhs_strong_form_is_bit_identical():
