@@ -12,6 +12,7 @@
 
 #include "../../include/flouhip.h"
 #include "common.hpp"
+#include "fast.hpp"
 #include "reduce.hpp"
 
 using namespace flouhip;
@@ -51,6 +52,13 @@ struct flou_hip_handle {
     // kernels
     RhsLauncherF64 l64 = nullptr;
     RhsLauncherF32 l32 = nullptr;
+    // FP32 fast path (3-D split form on axis-aligned affine elements)
+    FastLauncher lfast = nullptr;
+    int force_plain = 0;
+    void* d_fegeom = nullptr;
+    int fegeom_stride = 0;
+    void* d_fdsharp = nullptr;
+    float lift_m = 0.0f, lift_p = 0.0f, fdiss = 0.0f;
     std::string err;
 };
 
@@ -87,7 +95,8 @@ int upload_real(flou_hip_handle* h, void** dptr, const std::vector<double>& v) {
 void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
-                    h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2]};
+                    h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
+                    h->d_fegeom, h->d_fdsharp};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -162,6 +171,34 @@ int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, d
         a.dt = dt; a.A = A; a.B = B;
         a.mode = mode; a.first = first;
         e = h->l64(a, h->stream);
+    } else if (h->lfast && !h->force_plain) {
+        FastArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.Q = (const float*)u_in;
+        a.out = (float*)out;
+        a.ubase = (const float*)ubase;
+        a.reg = (float*)reg;
+        a.dt = (float)dt; a.A = (float)A; a.B = (float)B;
+        a.mode = mode; a.first = first;
+        a.ndofs = h->ndofs;
+        a.ne = (int)h->ne;
+        a.nb = h->d_nb;
+        a.finfo = h->d_finfo;
+        a.fgi = h->d_fgi;
+        a.fid = h->d_fid;
+        a.egeom = (const float*)h->d_fegeom;
+        a.egeom_stride = h->fegeom_stride;
+        a.fgeom = (const float*)h->d_fgeom;
+        a.dsharp = (const float*)h->d_fdsharp;
+        a.lift_m = h->lift_m;
+        a.lift_p = h->lift_p;
+        a.bcstate = (const float*)h->d_bcstate;
+        a.bcconst = (const float*)h->d_bcconst;
+        a.bctype = h->d_bctype;
+        a.gamma = (float)h->gamma;
+        a.diss = h->fdiss;
+        a.errflag = h->d_err;
+        e = h->lfast(a, h->stream);
     } else {
         RhsArgs<float> a = make_args<float>(h);
         a.Q = (const float*)u_in;
@@ -391,8 +428,59 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
 
+    // FP32 fast path: 3-D split-form Euler with the StdAverage/LxF surface flux on
+    // axis-aligned affine elements (diagonal metric, frames n = ±e_d), GLL traces
+    // (l = e_1/e_n and ∂g supported on the end nodes up to 1e-12) and a vanishing D♯
+    // diagonal.  Anything else runs the plain kernel in FP32.
+    std::vector<float> fe, fds;
+    if (h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
+        h->savg == FLOU_HIP_AVG_STD && h->sdiss != FLOU_HIP_DISS_MATRIX && n >= 4) {
+        bool ok = true;
+        double dmax = 0.0;
+        for (int i = 0; i < n * n; ++i) dmax = std::max(dmax, std::fabs(d->Dsharp[i]));
+        for (int j = 0; j < n; ++j) {
+            if (std::fabs(d->l_minus[j] - (j == 0 ? 1.0 : 0.0)) > 1e-12) ok = false;
+            if (std::fabs(d->l_plus[j] - (j == n - 1 ? 1.0 : 0.0)) > 1e-12) ok = false;
+            if (j > 0 && std::fabs(d->dg_minus[j]) > 1e-12 * std::fabs(d->dg_minus[0])) ok = false;
+            if (j < n - 1 && std::fabs(d->dg_plus[j]) > 1e-12 * std::fabs(d->dg_plus[n - 1])) ok = false;
+            if (std::fabs(d->Dsharp[j * n + j]) > 1e-10 * dmax) ok = false;
+        }
+        for (int64_t e = 0; e < ne && ok; ++e)
+            for (int r = 0; r < 3; ++r)
+                for (int c = 0; c < 3; ++c)
+                    if (r != c && d->elem_metric[e * 9 + r + 3 * c] != 0.0) ok = false;
+        for (int64_t f = 0; f < nf && ok; ++f) {
+            const int dd = d->face_elempos[f * 2] >> 1;
+            for (int c = 0; c < 3; ++c) {
+                const double nc = d->face_frames[f * 9 + c];
+                if (c == dd ? std::fabs(nc) != 1.0 : nc != 0.0) ok = false;
+            }
+        }
+        if (ok) h->lfast = fast3d_launcher(n, h->tp);
+        if (h->lfast) {
+            const int64_t cnt = uniform ? 1 : ne;
+            fe.assign(cnt * 8, 0.0f);
+            for (int64_t e = 0; e < cnt; ++e) {
+                const double J = d->elem_jac[e];
+                for (int c = 0; c < 3; ++c) fe[e * 8 + c] = (float)(d->elem_metric[e * 9 + c * 4] / J);
+                fe[e * 8 + 3] = (float)(1.0 / J);
+            }
+            h->fegeom_stride = uniform ? 0 : 8;
+            fds.resize(n * n);
+            for (int i = 0; i < n * n; ++i) fds[i] = (float)d->Dsharp[i];
+            h->lift_m = (float)d->dg_minus[0];
+            h->lift_p = (float)d->dg_plus[n - 1];
+            h->fdiss = h->sdiss == FLOU_HIP_DISS_LXF ? (float)h->intensity : 0.0f;
+        }
+    }
+
     // device uploads
     if (hipGetDevice(&h->device) != hipSuccess) return bad(FLOU_HIP_EDEVICE, "no HIP device");
+    if (h->lfast) {
+        int rcf = upload(h, &h->d_fegeom, fe);
+        rcf |= upload(h, &h->d_fdsharp, fds);
+        if (rcf) return bad(FLOU_HIP_ENOMEM, "device upload failed: " + h->err);
+    }
     int rc = 0;
     rc |= upload_real(h, &h->d_ops, ops);
     rc |= upload(h, (void**)&h->d_nb, nb);
@@ -437,6 +525,8 @@ int flou_hip_set_stream(flou_hip_handle* h, void* stream) {
 int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value) {
     if (!h) return FLOU_HIP_EINVAL;
     if (which == 0) { h->typo = value ? 1 : 0; return FLOU_HIP_OK; }
+    if (which == 1) { h->force_plain = value ? 1 : 0; return FLOU_HIP_OK; }
+    if (which == 2) return h->lfast ? 1 : 0;  // query: does FP32 run the fast kernel?
     return fail(h, FLOU_HIP_EINVAL, "unknown option");
 }
 

@@ -127,6 +127,10 @@ class MultielementDisc:
     def set_option(self, which, value):
         C.check(C.lib.flou_hip_set_option(self.h, which, value), self.h)
 
+    def uses_fast_kernel(self):
+        """True when this (FP32) discretisation runs the fast fused kernel."""
+        return C.lib.flou_hip_set_option(self.h, 2, 0) == 1
+
     def _set_bcs(self, bcs):
         mesh = self.mesh
         for name, bc in bcs.items():

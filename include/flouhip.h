@@ -143,9 +143,13 @@ const char* flou_hip_last_error(const flou_hip_handle* h);
  * of MultielementDiscontinuous.jl:42-44 and OpDivergence.jl:211-214 map here). */
 const char* flou_hip_create_error(void);
 
-/* Options.  0: reproduce the 3-D ChandrasekharAverage surface-flux term of
- * src/FlouSpatial/Equations/Euler.jl:216, which squares wl where wr is meant (default 0:
- * the corrected flux). */
+/* Options.
+ *  0: reproduce the 3-D ChandrasekharAverage surface-flux term of
+ *     src/FlouSpatial/Equations/Euler.jl:216, which squares wl where wr is meant (default 0:
+ *     the corrected flux).
+ *  1: FP32 handles: 1 = always run the plain (reference-order) kernel, 0 = use the fast
+ *     FP32 kernel where it applies (default).
+ *  2: query, returns 1 if this FP32 handle has the fast kernel, else 0 (value ignored). */
 int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value);
 
 /* Stream the handle launches on (hipStream_t; NULL = default stream). */

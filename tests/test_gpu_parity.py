@@ -221,18 +221,59 @@ def test_advection_2d_full_period_gauss_nodes():
     assert np.linalg.norm(u - Q0) <= 1e-3 * max(np.linalg.norm(u), np.linalg.norm(Q0))
 
 
-def test_fp32_rhs_error_against_fp64_oracle():
-    mesh, std, equation, op, orc = build(3, 6, [2, 2, 2], "split", "ch", finish=[2 * np.pi] * 3)
-    disc = F.MultielementDisc(mesh, std, equation, op, precision="fp32")
-    Q = _state(disc, orc, 3, "euler", seed=4)
+FP32_CASES = [
+    # n, tp, surf, periodic, amp
+    (6, "ch", "lxf", [1, 1, 1], 0.1),
+    (6, "ch", "lxf", [1, 1, 1], 0.5),  # large jumps: log branch of the log-mean
+    (5, "kg", "lxf", [1, 1, 1], 0.1),
+    (7, "std", "avg", [1, 1, 1], 0.1),
+    (4, "ch", "lxf", [1, 0, 1], 0.1),  # slip walls on y
+    (8, "ch", "lxf", [1, 1, 1], 0.1),
+]
+
+
+@pytest.mark.parametrize("case", FP32_CASES, ids=lambda c: "-".join(str(x) for x in c))
+def test_fp32_rhs_error_against_fp64_oracle(case):
+    """FP32 chosen form (fast kernel) and FP32 plain kernel against the FP64 oracle."""
+    n, tp, surf, periodic, amp = case
+    mesh, std, equation, op, orc = build(3, n, [2, 2, 2], "split", tp, surf, periodic=periodic,
+                                         finish=[2 * np.pi] * 3)
+    bcs = {}
+    if not all(periodic):
+        bcs = {"3": F.EulerSlipBC(), "4": F.EulerSlipBC()}
+        orc.set_bc(0, O.BC_SLIP)
+        orc.set_bc(1, O.BC_SLIP)
+    disc = F.MultielementDisc(mesh, std, equation, op, bcs, precision="fp32")
+    assert disc.uses_fast_kernel()
+    Q = _state(disc, orc, 3, "euler", seed=4, amp=amp)
     ref = orc.rhs(Q)
     Qd = disc.to_device(Q)
-    dQd = disc.new_state()
-    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
-    got = disc.to_host(dQd).astype(np.float64)
-    err = rel_err(got, ref)
-    print(f"FP32 rhs max rel err vs FP64 oracle: {err:.2e}")
-    assert err < 1e-4
+    for plain in (0, 1):
+        disc.set_option(1, plain)
+        dQd = disc.new_state()
+        F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+        disc.check()
+        got = disc.to_host(dQd).astype(np.float64)
+        err = rel_err(got, ref)
+        print(f"FP32 {'plain' if plain else 'fast'} rhs {case}: max rel err vs FP64 oracle {err:.2e}")
+        assert err < 2e-5
+
+
+def test_fp32_fused_ck2n54_against_fp64_oracle():
+    mesh, std, equation, op, orc = build(3, 6, [3, 2, 2], "split", "ch", finish=[2 * np.pi] * 3)
+    disc = F.MultielementDisc(mesh, std, equation, op, precision="fp32")
+    Q = _state(disc, orc, 3, "euler", seed=6)
+    dt, steps = 1e-2, 5
+    ref = orc.rk_run(O.RK_CK2N54, Q, dt, steps)
+    Qd = disc.to_device(Q)
+    F.timeintegrate(Qd, disc, equation, F.CarpenterKennedy2N54(), steps * dt, dt=dt)
+    got = disc.to_host(Qd).astype(np.float64)
+    err = np.linalg.norm(got[:, 0] - ref[:, 0]) / np.linalg.norm(ref[:, 0])
+    print(f"FP32 fused CK2N54 x{steps}: density rel L2 err vs FP64 oracle {err:.2e}")
+    assert err < 1e-6
+    # conservation of mass in FP32 (faces exactly antisymmetric): FP64 integrals
+    m0 = orc.integrate(Q, 0)
+    assert abs(F.integrate(Qd, disc, 0) - m0) < 1e-5 * abs(m0)
 
 
 def test_domain_error_on_negative_pressure():
