@@ -152,6 +152,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
                     help="time one form only (profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 3 plain copies of 1 GiB (FP32), the reference kernel for "
+                         "scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -161,6 +164,13 @@ def main():
     import flouhip as F
 
     torch.cuda.set_device(0)
+    if args.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # an elementwise kernel: 1 GiB read + 1 GiB write
+        torch.cuda.synchronize()
+        del src, dst
     # initial state and dt (FP64)
     disc64, eq, std = build_disc("fp64")
     Q64 = taylor_green_state(disc64, std)

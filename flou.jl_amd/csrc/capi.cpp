@@ -462,7 +462,11 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             fe.assign(cnt * 8, 0.0f);
             for (int64_t e = 0; e < cnt; ++e) {
                 const double J = d->elem_jac[e];
-                for (int c = 0; c < 3; ++c) fe[e * 8 + c] = (float)(d->elem_metric[e * 9 + c * 4] / J);
+                for (int c = 0; c < 3; ++c) {
+                    const double ja = d->elem_metric[e * 9 + c * 4];
+                    fe[e * 8 + c] = (float)(ja / J);   // Ja_cc / J
+                    fe[e * 8 + 4 + c] = (float)(1.0 / ja);  // lift scale 1 / Ja_cc
+                }
                 fe[e * 8 + 3] = (float)(1.0 / J);
             }
             h->fegeom_stride = uniform ? 0 : 8;

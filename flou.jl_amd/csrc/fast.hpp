@@ -19,7 +19,7 @@ struct FastArgs {
     const int* finfo;   // ne*6 (see common.hpp)
     const int* fgi;     // ne*6 index into fgeom
     const int* fid;     // ne*6 mesh face index
-    const float* egeom; // per element: Ja_xx/J, Ja_yy/J, Ja_zz/J, 1/J, 4 unused
+    const float* egeom; // per element: Ja_xx/J, Ja_yy/J, Ja_zz/J, 1/J, 1/Ja_xx, 1/Ja_yy, 1/Ja_zz, -
     int egeom_stride;   // 0 when every element is alike, else 8
     const float* fgeom; // per entry: frame (n, t, b) then surface J (10 floats)
     const float* dsharp;// D♯, row-major n x n
