@@ -196,6 +196,7 @@ int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, d
         a.bcconst = (const float*)h->d_bcconst;
         a.bctype = h->d_bctype;
         a.gamma = (float)h->gamma;
+        a.inv_gm1 = (float)(1.0 / (h->gamma - 1.0));
         a.diss = h->fdiss;
         a.errflag = h->d_err;
         e = h->lfast(a, h->stream);
@@ -431,10 +432,12 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // FP32 fast path: 3-D split-form Euler with the StdAverage/LxF surface flux on
     // axis-aligned affine elements (diagonal metric, frames n = ±e_d), GLL traces
     // (l = e_1/e_n and ∂g supported on the end nodes up to 1e-12) and a vanishing D♯
-    // diagonal.  Anything else runs the plain kernel in FP32.
+    // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable).  Anything
+    // else runs the plain kernel in FP32.
     std::vector<float> fe, fds;
     if (h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
-        h->savg == FLOU_HIP_AVG_STD && h->sdiss != FLOU_HIP_DISS_MATRIX && n >= 4) {
+        h->savg == FLOU_HIP_AVG_STD && h->sdiss != FLOU_HIP_DISS_MATRIX && n >= 4 &&
+        h->ndofs < (int64_t(1) << 30)) {
         bool ok = true;
         double dmax = 0.0;
         for (int i = 0; i < n * n; ++i) dmax = std::max(dmax, std::fabs(d->Dsharp[i]));

@@ -7,7 +7,7 @@
 namespace flouhip {
 
 struct FastArgs {
-    const float* Q;     // stage input, (ndofs, 5) column-major
+    const float* Q;     // stage input, (ndofs, 5) column-major, ndofs < 2^30
     float* out;
     const float* ubase;
     float* reg;
@@ -27,7 +27,7 @@ struct FastArgs {
     const float* bcstate;
     const float* bcconst;
     const int* bctype;
-    float gamma;
+    float gamma, inv_gm1;
     float diss;         // LxF intensity (0 for the StdAverage surface flux)
     int* errflag;
 };
