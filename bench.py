@@ -158,7 +158,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+              "(one GPU per run; multi-GPU sharding is not part of this build)", file=sys.stderr)
         sys.exit(2)
     import torch
     import flouhip as F

@@ -57,22 +57,21 @@ struct FCfg {
     static constexpr int NFN = EPB * 6 * NPF;                        // face nodes per block
     static constexpr int NPT = (NN + TPB - 1) / TPB;                 // nodes per thread
     static constexpr int FPT = (NFN + TPB - 1) / TPB;                // face nodes per thread
-    // LDS layout in floats.  Q: [5][NN]; P4: [NN] float4; P2: [NN] float2;
+    // LDS layout in floats.  Q: [5][NN]; P: [6][NN] two-point flux inputs;
     // X: [EPB][6][5][NPF] exterior traces; C: [5][NN]; G: [EPB][4]; F: [EPB][6] float2.
-    // Row sums of direction 0 reuse X, direction 1 reuses P4, direction 2 uses C.
+    // Row sums of direction 0 reuse X, direction 1 reuses P, direction 2 uses C.
+    // Arrays are indexed by block node, so lanes on consecutive lines hit distinct banks.
     static constexpr int OFF_Q = 0;
-    static constexpr int OFF_P4 = r4(5 * NN);
-    static constexpr int OFF_P2 = OFF_P4 + 4 * NN;
-    static constexpr int OFF_X = r4(OFF_P2 + 2 * NN);
+    static constexpr int OFF_P = r4(5 * NN);
+    static constexpr int OFF_X = r4(OFF_P + 6 * NN);
     static constexpr int SZ_X = 30 * EPB * NPF > 5 * NN ? 30 * EPB * NPF : 5 * NN;
     static constexpr int OFF_C = r4(OFF_X + SZ_X);
     static constexpr int OFF_G = r4(OFF_C + 5 * NN);
     static constexpr int OFF_F = OFF_G + 4 * EPB;
     static constexpr int LDS = OFF_F + 12 * EPB;
-    static_assert(5 * NN <= 6 * NN, "direction-1 row sums must fit in the flux inputs");
 };
 
-// Node inputs (6 floats, P4 = slots 0-3, P2 = slots 4-5):
+// Node inputs (6 slots):
 //   Chandrasekhar:  ρ/2, u/2, v/2, w/2 | β = ρ/(2p), 2/ρ
 //   Kennedy-Gruber: ρ/2, u/2, v/2, w/2 | p/2, e/2
 //   StdAverage:     ρ,   u,   v,   w   | p,   ρe
@@ -203,20 +202,18 @@ struct LineAddr {
     int m[3];
 };
 
-template <int N>
-__device__ __forceinline__ void load_line(const float* __restrict__ P4, const float2* __restrict__ P2,
-                                          const LineAddr& L, float (&P)[N][6]) {
+// flux inputs of the N nodes of a line, line-frame order (slot arrays of NN floats)
+template <int N, int NN>
+__device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr& L, float (&P)[N][6]) {
 #pragma unroll
     for (int p = 0; p < N; ++p) {
-        const int nd = L.base + p * L.stride;
-        const float* q = P4 + 4 * nd;
+        const float* q = sP + L.base + p * L.stride;
         P[p][0] = q[0];
-        P[p][1] = q[L.m[0]];
-        P[p][2] = q[L.m[1]];
-        P[p][3] = q[L.m[2]];
-        const float2 r = P2[nd];
-        P[p][4] = r.x;
-        P[p][5] = r.y;
+        P[p][1] = q[L.m[0] * NN];
+        P[p][2] = q[L.m[1] * NN];
+        P[p][3] = q[L.m[2] * NN];
+        P[p][4] = q[4 * NN];
+        P[p][5] = q[5 * NN];
     }
 }
 
@@ -298,8 +295,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
     float* const sQ = sm + C::OFF_Q;
-    float* const sP4 = sm + C::OFF_P4;
-    float2* const sP2 = reinterpret_cast<float2*>(sm + C::OFF_P2);
+    float* const sP = sm + C::OFF_P;
     float* const sX = sm + C::OFF_X;
     float* const sC = sm + C::OFF_C;
     float* const sG = sm + C::OFF_G;
@@ -405,20 +401,18 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f))) err |= ERR_LOG_NEG;
 #pragma unroll
         for (int c = 0; c < 5; ++c) sQ[c * NN + j] = q[r][c];
-        float4 p4;
-        float2 p2;
+        float in[6];
         if constexpr (TP == TP_CHANDRASEKHAR) {
-            p4 = make_float4(0.5f * rho, 0.5f * u, 0.5f * v, 0.5f * w);
-            p2 = make_float2(0.5f * rho * rcp(p), 2.0f * ir);
+            in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
+            in[4] = 0.5f * rho * rcp(p); in[5] = 2.0f * ir;
         } else if constexpr (TP == TP_KENNEDY_GRUBER) {
-            p4 = make_float4(0.5f * rho, 0.5f * u, 0.5f * v, 0.5f * w);
-            p2 = make_float2(0.5f * p, 0.5f * q[r][4] * ir);
+            in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
+            in[4] = 0.5f * p; in[5] = 0.5f * q[r][4] * ir;
         } else {
-            p4 = make_float4(rho, u, v, w);
-            p2 = make_float2(p, q[r][4]);
+            in[0] = rho; in[1] = u; in[2] = v; in[3] = w; in[4] = p; in[5] = q[r][4];
         }
-        reinterpret_cast<float4*>(sP4)[j] = p4;
-        sP2[j] = p2;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) sP[c * NN + j] = in[c];
     }
 #pragma unroll
     for (int r = 0; r < FPT; ++r) {
@@ -451,7 +445,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
     float acc[N][5];  // set by the first pair of each row
     {
         float P[N][6];
-        load_line<N>(sP4, sP2, L, P);
+        load_line<N, NN>(sP, L, P);
         if constexpr (TP == TP_CHANDRASEKHAR) {
             float amax = P[0][0], amin = P[0][0], bmax = P[0][4], bmin = P[0][4];
 #pragma unroll
@@ -496,7 +490,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         for (int r = 0; r < NPT; ++r) xv[r][v] = bload(rx, (uint32_t)(tid + r * TPB) * 4u);
     }
     __syncthreads();  // flux inputs and exterior traces are dead from here
-    if (line_ok) store_line<N, NN>(ldir == 0 ? sX : (ldir == 1 ? sP4 : sC), L, acc);
+    if (line_ok) store_line<N, NN>(ldir == 0 ? sX : (ldir == 1 ? sP : sC), L, acc);
     __syncthreads();
 
     // ---- per node: Σ_d (Ja_dd/J) row sums and the stage epilogue -----------------------
@@ -510,7 +504,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         const float s0 = sG[le * 4 + 0], s1 = sG[le * 4 + 1], s2 = sG[le * 4 + 2];
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-            kv[r][v] = fmaf(s0, sX[v * NN + jj], fmaf(s1, sP4[v * NN + jj], s2 * sC[v * NN + jj]));
+            kv[r][v] = fmaf(s0, sX[v * NN + jj], fmaf(s1, sP[v * NN + jj], s2 * sC[v * NN + jj]));
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
