@@ -106,14 +106,30 @@ def time_form(precision, Q64, dt, steps, warmup):
     gpu_ms = t0.elapsed_time(t1)
     rho = cur[0].to(torch.float64).clone()
     ndofs = disc.ndofs
+    fast = disc.uses_fast_kernel()
     del bufs, Q, cur, disc
     torch.cuda.empty_cache()
-    return dict(wall_s=wall, gpu_ms=gpu_ms, ndofs=ndofs, rho=rho)
+    return dict(wall_s=wall, gpu_ms=gpu_ms, ndofs=ndofs, rho=rho, fast=fast)
 
 
-def cpu_baseline(nel=24, steps=2):
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fast3d_6_chandrasekhar_64cube.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the fast FP32 kernel on this workload, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary (line 'traffic_bytes_per_launch: X')."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(nel=32, steps=6):
     """Oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on a 24^3 sample of the same
-    workload, CK2N54 steps, DOF-updates/s per stage."""
+    workload (about 10 s of CPU work), CK2N54 steps, DOF-updates/s per stage."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
@@ -139,7 +155,7 @@ def cpu_baseline(nel=24, steps=2):
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": d.ndofs * 5 * steps / t, "unit": "DOF-updates/s", "cores": cores,
             "kind": "port",
-            "sample": f"oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on 24^3 elements, N=5 "
+            "sample": f"oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on {nel}^3 elements, N=5 "
                       f"({d.ndofs} nodes), {steps} CK2N54 steps = {5 * steps} stages, {t:.1f} s"}
 
 
@@ -196,9 +212,17 @@ def main():
     ms_step = r["gpu_ms"] / args.steps
     value = nn * stages / (r["gpu_ms"] / 1e3)
     s_bytes = 4 if main_form == "fp32" else 8
-    bytes_per_launch = 4 * 5 * s_bytes * nn  # read u_in, read/write register, write u_out
+    # CK2N54 stage: read u_in, write register, write u_out; stages 2-5 also read the register
+    # -> 3 + 4 + 4 + 4 + 4 arrays of nn x 5 values per step
+    bytes_per_launch = (3 + 4 * 4) / 5 * 5 * s_bytes * nn
     launch_ms = r["gpu_ms"] / stages
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+    if main_form == "fp32" and r["fast"]:
+        kname = "k_fast3d<6, CHANDRASEKHAR> (FP32 fused rhs + CK2N54 stage)"
+        traffic = measured_traffic()
+    else:
+        kname = f"k_plain<{'float' if main_form == 'fp32' else 'double'},3,6,EULER,SPLIT,CHANDRASEKHAR> (fused rhs + CK2N54 stage)"
+        traffic = None
     out = {
         "metric": "DOF-updates/s per RK stage, 3D Euler N=5, FP32 vs FP64 GPU; adjoint/forward ratio",
         "value": value,
@@ -217,9 +241,8 @@ def main():
                                "CarpenterKennedy2N54 fused stages, fixed dt (CFL 0.3)",
                    "form": main_form, "nodes": nn, "stages_per_step": 5, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
-                     "kernel": "k_plain<float,3,6,EULER,SPLIT,CHANDRASEKHAR> (fused rhs + 2N stage)",
-                     "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms},
+                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "kernel": kname, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms},
     }
     if "fp64" in res and "fp32" in res:
         import torch as _t
