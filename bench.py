@@ -128,7 +128,7 @@ def measured_traffic():
 
 
 def cpu_baseline(nel=32, steps=6):
-    """Oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on a 24^3 sample of the same
+    """Oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on a 32^3 sample of the same
     workload (about 10 s of CPU work), CK2N54 steps, DOF-updates/s per stage."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "tests"))
