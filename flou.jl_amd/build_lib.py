@@ -32,6 +32,8 @@ def units():
         for nd in (1, 2, 3):
             u.append((f"rhs_plain_{prec.lower()}_{nd}d.o", "rhs_plain.hip",
                       EXACT + [f"-DFLOUHIP_PREC_{prec}", f"-DFLOUHIP_ND={nd}"]))
+            u.append((f"vjp_plain_{prec.lower()}_{nd}d.o", "vjp_plain.hip",
+                      [f"-DFLOUHIP_PREC_{prec}", f"-DFLOUHIP_ND={nd}"]))
     return u
 
 

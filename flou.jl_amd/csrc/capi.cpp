@@ -52,6 +52,9 @@ struct flou_hip_handle {
     // kernels
     RhsLauncherF64 l64 = nullptr;
     RhsLauncherF32 l32 = nullptr;
+    VjpLauncherF64 v64 = nullptr;
+    VjpLauncherF32 v32 = nullptr;
+    std::string vjp_why;  // why the adjoint is unavailable for this handle
     // FP32 fast path (3-D split form on axis-aligned affine elements)
     FastLauncher lfast = nullptr;
     int force_plain = 0;
@@ -157,6 +160,38 @@ RhsArgs<T> make_args(flou_hip_handle* h) {
     a.typo = h->typo;
     a.errflag = h->d_err;
     return a;
+}
+
+// One adjoint pass (see VjpArgs): o0 = a0·P0 + a1·P1 + (d rhs/dQ)^T(c·Y) at Q,
+// o1 = b0·o0 + b1·P1 + b2·P2.  o0/o1 may alias P0/P1/P2 (point-wise) but not Y or Q.
+int launch_vjp(flou_hip_handle* h, const void* Q, const void* Y, double c, const void* P0, double a0,
+               const void* P1, double a1, void* o0, const void* P2, double b0, double b1, double b2,
+               void* o1) {
+    if (!h->v64 && !h->v32) return fail(h, FLOU_HIP_EUNSUPPORTED, h->vjp_why);
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64) {
+        VjpArgs<double> a;
+        std::memset(&a, 0, sizeof(a));
+        a.r = make_args<double>(h);
+        a.r.Q = (const double*)Q;
+        a.Y = (const double*)Y; a.c = c;
+        a.P0 = (const double*)P0; a.P1 = (const double*)P1; a.P2 = (const double*)P2;
+        a.a0 = a0; a.a1 = a1; a.b0 = b0; a.b1 = b1; a.b2 = b2;
+        a.o0 = (double*)o0; a.o1 = (double*)o1;
+        e = h->v64(a, h->stream);
+    } else {
+        VjpArgs<float> a;
+        std::memset(&a, 0, sizeof(a));
+        a.r = make_args<float>(h);
+        a.r.Q = (const float*)Q;
+        a.Y = (const float*)Y; a.c = (float)c;
+        a.P0 = (const float*)P0; a.P1 = (const float*)P1; a.P2 = (const float*)P2;
+        a.a0 = (float)a0; a.a1 = (float)a1; a.b0 = (float)b0; a.b1 = (float)b1; a.b2 = (float)b2;
+        a.o0 = (float*)o0; a.o1 = (float*)o1;
+        e = h->v32(a, h->stream);
+    }
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, std::string("adjoint kernel launch: ") + hipGetErrorString(e));
+    return FLOU_HIP_OK;
 }
 
 int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, double B,
@@ -428,6 +463,19 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                          : (nd == 2 ? plain_f32_2d(n, h->eq, h->vol, h->tp) : plain_f32_3d(n, h->eq, h->vol, h->tp));
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
+    // adjoint kernels: StdAverage surface average with or without LxF dissipation
+    if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
+        h->vjp_why = "adjoint: only the StdAverage surface average is differentiated";
+    } else if (h->sdiss == FLOU_HIP_DISS_MATRIX) {
+        h->vjp_why = "adjoint: MatrixDissipation is not differentiated";
+    } else if (h->prec == FLOU_HIP_FP64) {
+        h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp)
+                         : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp) : vjp_f64_3d(n, h->eq, h->vol, h->tp));
+    } else {
+        h->v32 = nd == 1 ? vjp_f32_1d(n, h->eq, h->vol, h->tp)
+                         : (nd == 2 ? vjp_f32_2d(n, h->eq, h->vol, h->tp) : vjp_f32_3d(n, h->eq, h->vol, h->tp));
+    }
+    if (!h->v64 && !h->v32 && h->vjp_why.empty()) h->vjp_why = "adjoint: no kernel for this combination";
 
     // FP32 fast path: 3-D split-form Euler with the StdAverage/LxF surface flux on
     // axis-aligned affine elements (diagonal metric, frames n = ±e_d), GLL traces
@@ -723,8 +771,76 @@ int flou_hip_timeintegrate(flou_hip_handle* h, int32_t scheme, void* Q, double d
 }
 
 int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void* Qbar) {
-    (void)Q; (void)dQbar; (void)Qbar;
-    return fail(h, FLOU_HIP_EUNSUPPORTED, "adjoint not built yet");
+    if (!h || !Q || !dQbar || !Qbar) return FLOU_HIP_EINVAL;
+    const size_t bytes = flou_hip_state_bytes(h);
+    if (overlaps(Qbar, dQbar, bytes) || overlaps(Qbar, Q, bytes))
+        return fail(h, FLOU_HIP_EINVAL, "Qbar must not overlap Q or dQbar");
+    return launch_vjp(h, Q, dQbar, 1.0, Qbar, 1.0, nullptr, 0.0, Qbar, nullptr, 0.0, 0.0, 0.0, nullptr);
+}
+
+int flou_hip_rk_step_record(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                            void* const* stages, void* u_next) {
+    if (!h || !stages || !u_next) return FLOU_HIP_EINVAL;
+    const int S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
+    if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
+    for (int s = 0; s < S; ++s)
+        if (!stages[s]) return fail(h, FLOU_HIP_EINVAL, "stage buffer missing");
+    int rc = 0;
+    if (scheme == FLOU_HIP_SSPRK33) {
+        rc = flou_hip_rk_stage(h, scheme, 0, dt, t, stages[0], nullptr, nullptr, stages[1]);
+        if (!rc) rc = flou_hip_rk_stage(h, scheme, 1, dt, t, stages[1], stages[0], nullptr, stages[2]);
+        if (!rc) rc = flou_hip_rk_stage(h, scheme, 2, dt, t, stages[2], stages[0], nullptr, u_next);
+    } else {
+        if (!h->d_work[0]) HIPCHECK(h, hipMalloc(&h->d_work[0], flou_hip_state_bytes(h)));
+        for (int s = 0; s < S && !rc; ++s)
+            rc = flou_hip_rk_stage(h, scheme, s, dt, t, stages[s], nullptr, h->d_work[0],
+                                   s + 1 < S ? stages[s + 1] : u_next);
+    }
+    return rc;
+}
+
+int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                         const void* const* stages, void* ubar) {
+    (void)t;
+    if (!h || !stages || !ubar) return FLOU_HIP_EINVAL;
+    if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
+    const int S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
+    for (int s = 0; s < S; ++s)
+        if (!stages[s]) return fail(h, FLOU_HIP_EINVAL, "stage buffer missing");
+    const size_t bytes = flou_hip_state_bytes(h);
+    for (int i = 0; i < 3; ++i)
+        if (!h->d_work[i]) HIPCHECK(h, hipMalloc(&h->d_work[i], bytes));
+    void* X = h->d_work[0];
+    void* Y2 = h->d_work[1];
+    void* Z = h->d_work[2];
+    int rc;
+    if (scheme == FLOU_HIP_SSPRK33) {
+        // u1 = u + dt L(u); u2 = (3u + u1 + dt L(u1))/4; u3 = (u + 2u2 + 2dt L(u2))/3
+        // s=2: ū2 = ⅔ū3 + L'(u2)ᵀ(⅔dt ū3) -> X, acc = ⅓ū3 -> Z
+        rc = launch_vjp(h, stages[2], ubar, 2.0 * dt / 3.0, ubar, 2.0 / 3.0, nullptr, 0.0, X,
+                        ubar, 0.0, 0.0, 1.0 / 3.0, Z);
+        // s=1: ū1 = ¼ū2 + L'(u1)ᵀ(¼dt ū2) -> Y2, acc += ¾ū2
+        if (!rc) rc = launch_vjp(h, stages[1], X, dt / 4.0, X, 0.25, Z, 0.0, Y2, X, 0.0, 1.0, 0.75, Z);
+        // s=0: ū = ū1 + acc + L'(u)ᵀ(dt ū1) -> ubar
+        if (!rc) rc = launch_vjp(h, stages[0], Y2, dt, Y2, 1.0, Z, 1.0, ubar, nullptr, 0.0, 0.0, 0.0, nullptr);
+        return rc;
+    }
+    // 2N: reg_s = A_s reg_{s-1} + dt L(u_s), u_{s+1} = u_s + B_s reg_s.  Reverse with
+    // ḡ_s = B_s ū_{s+1} + A_{s+1} ḡ_{s+1}, ū_s = ū_{s+1} + L'(u_s)ᵀ(dt ḡ_s).
+    double A[5], B[5];
+    for (int s = 0; s < S; ++s) tableau(scheme, s, A[s], B[s]);
+    void* G[2] = {Y2, Z};
+    // s = S-1: ḡ = B ū (folded into the scale), result ū -> X, ḡ_{S-2} -> G[0]
+    rc = launch_vjp(h, stages[S - 1], ubar, dt * B[S - 1], ubar, 1.0, nullptr, 0.0, X,
+                    ubar, B[S - 2], 0.0, A[S - 1] * B[S - 1], G[0]);
+    int cur = 0;
+    for (int s = S - 2; s >= 1 && !rc; --s) {
+        rc = launch_vjp(h, stages[s], G[cur], dt, X, 1.0, nullptr, 0.0, X, G[cur], B[s - 1], 0.0, A[s],
+                        G[1 - cur]);
+        cur = 1 - cur;
+    }
+    if (!rc) rc = launch_vjp(h, stages[0], G[cur], dt, X, 1.0, nullptr, 0.0, ubar, nullptr, 0.0, 0.0, 0.0, nullptr);
+    return rc;
 }
 
 }  // extern "C"

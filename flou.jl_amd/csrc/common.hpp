@@ -119,6 +119,21 @@ __host__ __device__ inline void node_line(int node, int dir, int& f, int& p) {
     else { f = ix + N * iy; p = iz; }
 }
 
+// Adjoint pass (vjp_plain.hip): V = (d out/dQ)^T (c·Y) at r.Q, then
+//   o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2   (null P*: term absent; o1 optional)
+template <typename T>
+struct VjpArgs {
+    RhsArgs<T> r;
+    const T* Y;
+    T c;
+    const T* P0;
+    const T* P1;
+    const T* P2;
+    T a0, a1, b0, b1, b2;
+    T* o0;
+    T* o1;
+};
+
 typedef hipError_t (*RhsLauncherF64)(const RhsArgs<double>&, hipStream_t);
 typedef hipError_t (*RhsLauncherF32)(const RhsArgs<float>&, hipStream_t);
 
@@ -129,5 +144,14 @@ RhsLauncherF64 plain_f64_3d(int n, int eq, int vol, int tp);
 RhsLauncherF32 plain_f32_1d(int n, int eq, int vol, int tp);
 RhsLauncherF32 plain_f32_2d(int n, int eq, int vol, int tp);
 RhsLauncherF32 plain_f32_3d(int n, int eq, int vol, int tp);
+
+typedef hipError_t (*VjpLauncherF64)(const VjpArgs<double>&, hipStream_t);
+typedef hipError_t (*VjpLauncherF32)(const VjpArgs<float>&, hipStream_t);
+VjpLauncherF64 vjp_f64_1d(int n, int eq, int vol, int tp);
+VjpLauncherF64 vjp_f64_2d(int n, int eq, int vol, int tp);
+VjpLauncherF64 vjp_f64_3d(int n, int eq, int vol, int tp);
+VjpLauncherF32 vjp_f32_1d(int n, int eq, int vol, int tp);
+VjpLauncherF32 vjp_f32_2d(int n, int eq, int vol, int tp);
+VjpLauncherF32 vjp_f32_3d(int n, int eq, int vol, int tp);
 
 }  // namespace flouhip

@@ -1,0 +1,342 @@
+// Adjoint (vector-Jacobian product) of the rhs, fused with the reverse RK stage epilogue.
+//
+// Forward (rhs_plain.hip): out_i = (1/J) R_i with
+//   R_i = -Σ_d Σ_j D♯_ij F♯_d(Q_i, Q_j)      (split; strong: -Σ_j Ds_ij F̃_d(Q_j))
+//         -Σ_d [∂g⁻_ii Fn_{2d}(f) + ∂g⁺_ii Fn_{2d+1}(f)],  Fn = ±jac · F(trace_master, trace_slave)
+// (OpDivergence.jl:42-299, Interfaces.jl:25-136, MultielementDiscontinuous.jl:132-137).
+// Given a cotangent Y of out, one pass per element block computes
+//   V = (∂out/∂Q)^T (c·Y)
+// as a gather (no atomics):
+//   1. w = c·Y/J for the block's elements (LDS);
+//   2. per face node of the element's faces: the cotangent of the face flux,
+//      F̄ = -jac (σ_own Σ_i ∂g_i w_own,i + σ_nb Σ_i ∂g_i w_nb,i) (σ = +1 master, -1 slave,
+//      the neighbour's w read from HBM), pulled through the rotation and the numerical
+//      flux to the element's own trace, and stored in LDS;
+//   3. per node: the volume terms (each node gathers ∂/∂Q_i of every pair flux it is part
+//      of, with F̄_ij = -D♯_ij w_i - D♯_ji w_j) and the transposed trace projection;
+// then the epilogue  o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2  (o1 optional),
+// which expresses rhs_vjp (Qbar += V) and the reverse stages of SSPRK33 and 2N schemes.
+// Compile with -DFLOUHIP_PREC_F64 or -DFLOUHIP_PREC_F32 and -DFLOUHIP_ND=1|2|3.
+#include "common.hpp"
+#include "physics_plain.hpp"
+#include "adjoint.hpp"
+
+namespace flouhip {
+
+template <typename T, int ND, int N, int EQ, int VOL, int TP>
+__global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
+    using C = Cfg<ND, N>;
+    constexpr int NPE = C::NPE, NPF = C::NPF, NF = C::NF, EPB = C::EPB, TPB = C::TPB;
+    constexpr int NV = EQ == EQ_ADVECTION ? 1 : ND + 2;
+    constexpr int NOPS = ops_size(N);
+    constexpr int NG = ND * ND + 1;
+    const RhsArgs<T>& a = A.r;
+
+    __shared__ T sOps[NOPS];
+    __shared__ T sG[EPB][NG];
+    __shared__ T sQ[EPB][NV][NPE];
+    __shared__ T sW[EPB][NV][NPE];
+    __shared__ T sTb[EPB][NF][NV][NPF];
+
+    const int tid = threadIdx.x;
+    const int e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int64_t Nd = a.ndofs;
+    const T avel[3] = {a.a0, a.a1, a.a2};
+
+    for (int i = tid; i < NOPS; i += TPB) sOps[i] = a.ops[i];
+    for (int i = tid; i < EPB * NG; i += TPB) {
+        const int le = i / NG, c = i % NG, e = e0 + le;
+        if (e < a.ne) sG[le][c] = a.egeom[(int64_t)e * a.egeom_stride + c];
+    }
+    __syncthreads();
+    for (int i = tid; i < NV * EPB * NPE; i += TPB) {
+        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+        const int le = j / NPE, nd = j % NPE;
+        if (e0 + le < a.ne) {
+            const int64_t g = v * Nd + (int64_t)e0 * NPE + j;
+            sQ[le][v][nd] = a.Q[g];
+            sW[le][v][nd] = A.c * A.Y[g] / sG[le][ND * ND];
+        }
+    }
+    __syncthreads();
+
+    // ---- face nodes: cotangent of the own trace ----------------------------------------
+    for (int it = tid; it < EPB * NF * NPF; it += TPB) {
+        const int le = it / (NF * NPF), r = it % (NF * NPF);
+        const int k = r / NPF, f = r % NPF, e = e0 + le;
+        if (e >= a.ne) continue;
+        const int dir = k >> 1, lr = k & 1;
+        const T* lk = &sOps[lr ? ops_lp(N) : ops_lm(N)];
+        const T* gk = &sOps[lr ? ops_gp(N) : ops_gm(N)];
+        T own[NV], ext[NV], Lo[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            T s = T(0), t = T(0);
+            for (int p = 0; p < N; ++p) {
+                const int nn = line_node<ND, N>(dir, f, p);
+                s = s + lk[p] * sQ[le][v][nn];
+                t = t + gk[p] * sW[le][v][nn];
+            }
+            own[v] = a.gather ? sQ[le][v][line_node<ND, N>(dir, f, lr ? N - 1 : 0)] : s;
+            Lo[v] = t;
+        }
+        const int slot = e * NF + k;
+        const int info = a.finfo[slot];
+        const int nbe = a.nb[slot];
+        const bool slave = finfo_side(info);
+        const T* g = a.fgeom + (int64_t)a.fgi[slot] * (3 * ND + 1);
+        const T jac = g[3 * ND];
+        T Fb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Fb[v] = slave ? Lo[v] : -Lo[v];  // -σ_own L_own
+        int type = -1;
+        if (nbe >= 0) {
+            const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
+            const T* lq = &sOps[nlr ? ops_lp(N) : ops_lm(N)];
+            const T* gq = &sOps[nlr ? ops_gp(N) : ops_gm(N)];
+            const T Jn = a.egeom[(int64_t)nbe * a.egeom_stride + ND * ND];
+            const int64_t base = (int64_t)nbe * NPE;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                T s = T(0), t = T(0);
+                for (int p = 0; p < N; ++p) {
+                    const int64_t gi = v * Nd + base + line_node<ND, N>(ndir, f, p);
+                    s = s + lq[p] * a.Q[gi];
+                    t = t + gq[p] * A.Y[gi];
+                }
+                ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, f, nlr ? N - 1 : 0)] : s;
+                const T Ln = A.c * t / Jn;
+                Fb[v] = Fb[v] + (slave ? -Ln : Ln);  // -σ_nb L_nb, σ_nb = -σ_own
+            }
+        } else {
+            const int bd = finfo_bd(info);
+            type = a.bctype[bd];
+            if (type == BC_DIRICHLET) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = a.bcstate[((int64_t)a.fid[slot] * NPF + f) * NV + v];
+            } else if (type == BC_INFLOW) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = a.bcconst[bd * NV + v];
+            } else if (type == BC_OUTFLOW || EQ == EQ_ADVECTION) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = own[v];
+            } else {
+                if constexpr (EQ == EQ_EULER) {
+                    T R[NV];
+                    plain::rotate2face<T, ND>(own, g, R);
+                    R[1] = -R[1];
+                    plain::rotate2phys<T, ND>(R, g, ext);
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Fb[v] = Fb[v] * jac;
+        const T* Ql = slave ? ext : own;
+        const T* Qr = slave ? own : ext;
+        T Qlb[NV], Qrb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) { Qlb[v] = T(0); Qrb[v] = T(0); }
+        if constexpr (EQ == EQ_ADVECTION) {
+            T an = avel[0] * g[0];
+#pragma unroll
+            for (int kk = 1; kk < ND; ++kk) an = an + avel[kk] * g[kk];
+            const T diss = a.sdiss == DISS_LXF ? fabs(an) * a.intensity : T(0);
+            Qlb[0] = Fb[0] * (an + diss) / T(2);
+            Qrb[0] = Fb[0] * (an - diss) / T(2);
+        } else {
+            T Rl[NV], Rr[NV], Rb[NV], Rlb[NV], Rrb[NV];
+            plain::rotate2face<T, ND>(Ql, g, Rl);
+            plain::rotate2face<T, ND>(Qr, g, Rr);
+            plain::rotate2face<T, ND>(Fb, g, Rb);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) { Rlb[v] = T(0); Rrb[v] = T(0); }
+            adj::numflux_vjp<T, ND>(Rl, Rr, a.sdiss == DISS_LXF, a.intensity, a.gamma, Rb, Rlb, Rrb);
+            plain::rotate2phys<T, ND>(Rlb, g, Qlb);
+            plain::rotate2phys<T, ND>(Rrb, g, Qrb);
+        }
+        T tb[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) tb[v] = slave ? Qrb[v] : Qlb[v];
+        if (nbe < 0) {  // boundary: the exterior state is a function of the own trace
+            if (type == BC_OUTFLOW || (EQ == EQ_ADVECTION && type != BC_DIRICHLET && type != BC_INFLOW)) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) tb[v] = tb[v] + Qrb[v];
+            } else if (type == BC_SLIP) {
+                if constexpr (EQ == EQ_EULER) {
+                    T R[NV], B[NV];
+                    plain::rotate2face<T, ND>(Qrb, g, R);
+                    R[1] = -R[1];
+                    plain::rotate2phys<T, ND>(R, g, B);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) tb[v] = tb[v] + B[v];
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) sTb[le][k][v][f] = tb[v];
+    }
+    __syncthreads();
+
+    const int le = tid / NPE, node = tid % NPE, e = e0 + le;
+    if (e >= a.ne) return;
+
+    T qi[NV], wi[NV], V[NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        qi[v] = sQ[le][v][node];
+        wi[v] = sW[le][v][node];
+        V[v] = T(0);
+    }
+    const T* M = &sG[le][0];
+#pragma unroll
+    for (int dir = 0; dir < ND; ++dir) {
+        int f, ii;
+        node_line<ND, N>(node, dir, f, ii);
+        T n[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) n[k] = M[k + ND * dir];
+        if constexpr (VOL == VOL_STRONG) {
+            const T* Ds = &sOps[ops_Ds(N)];
+            T cb[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) cb[v] = T(0);
+            for (int kk = 0; kk < N; ++kk) {
+                const int nk = line_node<ND, N>(dir, f, kk);
+                const T Dki = Ds[kk * N + ii];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) cb[v] = cb[v] - Dki * sW[le][v][nk];
+            }
+            if constexpr (EQ == EQ_ADVECTION) {
+                T an = T(0);
+#pragma unroll
+                for (int k = 0; k < ND; ++k) an = an + avel[k] * n[k];
+                V[0] = V[0] + an * cb[0];
+            } else {
+                adj::flux_vjp<T, ND>(qi, n, a.gamma, cb, V);
+            }
+        } else {
+            const T* Dsh = &sOps[ops_Dsh(N)];
+            for (int j = 0; j < N; ++j) {
+                const int nj = line_node<ND, N>(dir, f, j);
+                T Gb[NV];
+                if (j == ii) {
+                    const T d = Dsh[ii * N + ii];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) Gb[v] = -d * wi[v];
+                    if constexpr (EQ == EQ_ADVECTION) {
+                        T an = T(0);
+#pragma unroll
+                        for (int k = 0; k < ND; ++k) an = an + avel[k] * n[k];
+                        V[0] = V[0] + an * Gb[0];
+                    } else {
+                        adj::flux_vjp<T, ND>(qi, n, a.gamma, Gb, V);
+                    }
+                } else {
+                    const T dij = Dsh[ii * N + j], dji = Dsh[j * N + ii];
+                    T qj[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        qj[v] = sQ[le][v][nj];
+                        Gb[v] = -dij * wi[v] - dji * sW[le][v][nj];
+                    }
+                    if constexpr (EQ == EQ_ADVECTION) {
+                        T an = T(0);
+#pragma unroll
+                        for (int k = 0; k < ND; ++k) an = an + avel[k] * n[k];
+                        V[0] = V[0] + an * Gb[0] / T(2);
+                    } else {
+                        adj::twopoint_vjp1<T, ND, TP>(qi, qj, n, a.gamma, Gb, V);
+                    }
+                }
+            }
+        }
+        // transposed trace projection of the two faces of this line
+        const T lmi = sOps[ops_lm(N) + ii], lpi = sOps[ops_lp(N) + ii];
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+            V[v] = V[v] + lmi * sTb[le][2 * dir][v][f] + lpi * sTb[le][2 * dir + 1][v][f];
+    }
+
+    // epilogue
+    const int64_t gi = (int64_t)e * NPE + node;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const int64_t idx = v * Nd + gi;
+        T o = V[v];
+        if (A.P0) o = o + A.a0 * A.P0[idx];
+        if (A.P1) o = o + A.a1 * A.P1[idx];
+        if (A.o1) {
+            T x = A.b0 * o;
+            if (A.P1) x = x + A.b1 * A.P1[idx];
+            if (A.P2) x = x + A.b2 * A.P2[idx];
+            A.o1[idx] = x;
+        }
+        A.o0[idx] = o;
+    }
+}
+
+template <typename T, int ND, int N, int EQ, int VOL, int TP>
+static hipError_t launch_vjp(const VjpArgs<T>& a, hipStream_t s) {
+    using C = Cfg<ND, N>;
+    const int nblocks = (a.r.ne + C::EPB - 1) / C::EPB;
+    hipLaunchKernelGGL((k_vjp<T, ND, N, EQ, VOL, TP>), dim3(nblocks), dim3(C::TPB), 0, s, a);
+    return hipGetLastError();
+}
+
+template <typename T, int ND, int N>
+static hipError_t (*pick_vjp(int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
+    if (eq == EQ_ADVECTION)
+        return vol == VOL_STRONG ? &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0>
+                                 : &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0>;
+    if (vol == VOL_STRONG) return &launch_vjp<T, ND, N, EQ_EULER, VOL_STRONG, 0>;
+    switch (tp) {
+        case TP_STDAVG: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG>;
+        case TP_CHANDRASEKHAR: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR>;
+        case TP_KENNEDY_GRUBER: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER>;
+        default: return nullptr;
+    }
+}
+
+template <typename T, int ND>
+static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
+    switch (n) {
+        case 2: return pick_vjp<T, ND, 2>(eq, vol, tp);
+        case 3: return pick_vjp<T, ND, 3>(eq, vol, tp);
+        case 4: return pick_vjp<T, ND, 4>(eq, vol, tp);
+        case 5: return pick_vjp<T, ND, 5>(eq, vol, tp);
+        case 6: return pick_vjp<T, ND, 6>(eq, vol, tp);
+        case 7: return pick_vjp<T, ND, 7>(eq, vol, tp);
+        case 8: return pick_vjp<T, ND, 8>(eq, vol, tp);
+        default: return nullptr;
+    }
+}
+
+#if defined(FLOUHIP_PREC_F64)
+#define FLOUHIP_SCALAR double
+#define FLOUHIP_PTAG f64
+#elif defined(FLOUHIP_PREC_F32)
+#define FLOUHIP_SCALAR float
+#define FLOUHIP_PTAG f32
+#else
+#error "define FLOUHIP_PREC_F64 or FLOUHIP_PREC_F32"
+#endif
+
+#define FLOUHIP_CAT3(a, b, c) a##_##b##_##c
+#define FLOUHIP_NAME(p, d) FLOUHIP_CAT3(vjp, p, d)
+
+#if FLOUHIP_ND == 1
+#define FLOUHIP_DTAG 1d
+#elif FLOUHIP_ND == 2
+#define FLOUHIP_DTAG 2d
+#elif FLOUHIP_ND == 3
+#define FLOUHIP_DTAG 3d
+#else
+#error "define FLOUHIP_ND"
+#endif
+
+hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp))(
+    const VjpArgs<FLOUHIP_SCALAR>&, hipStream_t) {
+    return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND>(n, eq, vol, tp);
+}
+
+}  // namespace flouhip

@@ -75,6 +75,10 @@ SYMBOLS = {
     "flou_hip_integrate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _d]),
     "flou_hip_check": (ctypes.c_int, [_vp]),
     "flou_hip_rhs_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
+    "flou_hip_rk_step_record": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                               ctypes.POINTER(_vp), _vp]),
+    "flou_hip_rk_step_vjp": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(_vp), _vp]),
 }
 
 

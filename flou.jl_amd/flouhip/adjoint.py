@@ -1,0 +1,158 @@
+"""Adjoint of rhs! and of the RK time loop, with checkpointed recomputation.
+
+Flou has no AD rules; the north-star binding puts a ChainRulesCore ``rrule`` around
+``rhs!`` and ``timeintegrate``.  Here the same rules are ``torch.autograd.Function``s and
+plain functions over the C-ABI (include/flouhip.h):
+
+* ``rhs_vjp``        -- Qbar += (d rhs/dQ)^T dQbar            (flou_hip_rhs_vjp)
+* ``rk_step_record`` -- one RK step keeping its stage inputs   (flou_hip_rk_step_record)
+* ``rk_step_vjp``    -- cotangent of u_n from that of u_{n+1}  (flou_hip_rk_step_vjp)
+* ``gradient``       -- reverse sweep over nsteps fixed-dt steps: checkpoints every
+  ``checkpoint_every`` steps during the forward run, then per segment one forward
+  recomputation that keeps the stage inputs, followed by the reverse steps.
+
+dt is a fixed schedule (no gradient through the CFL min).  All arithmetic runs in
+libflouhip; torch supplies device buffers.
+"""
+import ctypes
+
+import torch
+
+from . import _capi as C
+from .disc import _solver_code
+
+
+def _nstages(code):
+    return 3 if code == C.SSPRK33 else 5
+
+
+def rhs_vjp(Qbar, dQbar, Q, disc):
+    """Qbar += (d rhs!(·)/dQ at Q)^T dQbar (in place on Qbar)."""
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rhs_vjp(disc.h, disc._dptr(Q), disc._dptr(dQbar), disc._dptr(Qbar)),
+            disc.h, "rhs_vjp")
+    return Qbar
+
+
+def _ptr_array(disc, bufs):
+    arr = (ctypes.c_void_p * len(bufs))()
+    for i, b in enumerate(bufs):
+        arr[i] = disc._dptr(b).value
+    return arr
+
+
+def rk_step_record(disc, solver, dt, stages, u_next, t=0.0):
+    """One RK step from stages[0] = u_n; stages[1:] receive the stage inputs, u_next u_{n+1}."""
+    code = _solver_code(solver)
+    if len(stages) != _nstages(code):
+        raise ValueError(f"{_nstages(code)} stage buffers expected")
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rk_step_record(disc.h, code, float(dt), float(t), _ptr_array(disc, stages),
+                                          disc._dptr(u_next)), disc.h, "rk_step_record")
+
+
+def rk_step_vjp(disc, solver, dt, stages, ubar, t=0.0):
+    """In place: ubar (cotangent of u_{n+1}) -> cotangent of u_n, at the recorded stages."""
+    code = _solver_code(solver)
+    if len(stages) != _nstages(code):
+        raise ValueError(f"{_nstages(code)} stage buffers expected")
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rk_step_vjp(disc.h, code, float(dt), float(t), _ptr_array(disc, stages),
+                                       disc._dptr(ubar)), disc.h, "rk_step_vjp")
+    return ubar
+
+
+def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtype=None):
+    """Fixed-dt forward run from Q0 (not modified).  Returns (u_T, checkpoints) where
+    checkpoints[i] is the state at step i*checkpoint_every, stored as checkpoint_dtype
+    (default: the working precision)."""
+    code = _solver_code(solver)
+    K = checkpoint_every or nsteps or 1
+    cdt = checkpoint_dtype or disc.torch_dtype()
+    S = _nstages(code)
+    stages = [Q0.clone()] + [disc.new_state() for _ in range(S - 1)]
+    nxt = disc.new_state()
+    cps = []
+    for n in range(nsteps):
+        if n % K == 0:
+            cps.append(stages[0].to(cdt, copy=True))
+        rk_step_record(disc, solver, dt, stages, nxt)
+        stages[0], nxt = nxt, stages[0]
+    return stages[0], cps
+
+
+def gradient(disc, solver, Q0, dt, nsteps, seed, checkpoint_every=20, checkpoint_dtype=None,
+             return_state=False):
+    """Cotangent of the initial state for a loss of the final state.
+
+    seed(u_T) returns dL/du_T (a device state tensor).  Memory: nsteps/checkpoint_every
+    checkpoints plus checkpoint_every x (stages) states for the segment being reversed."""
+    code = _solver_code(solver)
+    S = _nstages(code)
+    K = max(1, int(checkpoint_every))
+    wdt = disc.torch_dtype()
+    uT, cps = forward(disc, solver, Q0, dt, nsteps, K, checkpoint_dtype)
+    ubar = seed(uT).to(wdt).contiguous()
+    nseg = len(cps)
+    seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
+    nxt = disc.new_state()
+    for sgi in reversed(range(nseg)):
+        n0 = sgi * K
+        nloc = min(K, nsteps - n0)
+        seg_bufs[0][0].copy_(cps[sgi])
+        for i in range(nloc):
+            dst = seg_bufs[i + 1][0] if i + 1 < nloc else nxt
+            rk_step_record(disc, solver, dt, seg_bufs[i], dst)
+        for i in reversed(range(nloc)):
+            rk_step_vjp(disc, solver, dt, seg_bufs[i], ubar)
+    return (ubar, uT) if return_state else ubar
+
+
+class Rhs(torch.autograd.Function):
+    """dQ = rhs!(Q) with the device adjoint as its backward (the rrule of rhs!)."""
+
+    @staticmethod
+    def forward(ctx, Q, disc):
+        dQ = torch.empty_like(Q)
+        disc._sync_stream()
+        C.check(C.lib.flou_hip_rhs(disc.h, disc._dptr(dQ), disc._dptr(Q), 0.0), disc.h, "rhs!")
+        ctx.save_for_backward(Q)
+        ctx.disc = disc
+        return dQ
+
+    @staticmethod
+    def backward(ctx, gdQ):
+        (Q,) = ctx.saved_tensors
+        Qbar = torch.zeros_like(Q)
+        rhs_vjp(Qbar, gdQ.contiguous(), Q, ctx.disc)
+        return Qbar, None
+
+
+class TimeIntegrate(torch.autograd.Function):
+    """u_T = timeintegrate(Q0) over nsteps fixed steps, backward by checkpointed reverse
+    sweep (the rrule of timeintegrate)."""
+
+    @staticmethod
+    def forward(ctx, Q0, disc, solver, dt, nsteps, checkpoint_every=20):
+        uT, cps = forward(disc, solver, Q0, dt, nsteps, checkpoint_every)
+        ctx.disc, ctx.solver, ctx.dt, ctx.nsteps, ctx.K = disc, solver, dt, nsteps, checkpoint_every
+        ctx.cps = cps
+        return uT
+
+    @staticmethod
+    def backward(ctx, guT):
+        disc, solver, dt, nsteps, K = ctx.disc, ctx.solver, ctx.dt, ctx.nsteps, ctx.K
+        code = _solver_code(solver)
+        S = _nstages(code)
+        ubar = guT.to(disc.torch_dtype()).contiguous().clone()
+        bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
+        nxt = disc.new_state()
+        for sgi in reversed(range(len(ctx.cps))):
+            n0 = sgi * K
+            nloc = min(K, nsteps - n0)
+            bufs[0][0].copy_(ctx.cps[sgi])
+            for i in range(nloc):
+                rk_step_record(disc, solver, dt, bufs[i], bufs[i + 1][0] if i + 1 < nloc else nxt)
+            for i in reversed(range(nloc)):
+                rk_step_vjp(disc, solver, dt, bufs[i], ubar)
+        return ubar, None, None, None, None, None

@@ -205,9 +205,26 @@ int flou_hip_integrate(flou_hip_handle* h, const void* Q, int32_t ivar, double* 
  * last call (sqrt or log of a negative number in the reference), and clears the flag. */
 int flou_hip_check(flou_hip_handle* h);
 
-/* Adjoint of rhs!: Qbar += (d rhs / dQ)^T dQbar, evaluated at Q.  No reference counterpart
- * (Flou has no AD rule); it backs the rrule of rhs!. */
+/* Adjoint of rhs! (Equations/Hyperbolic.jl:31-69): Qbar += (d rhs / dQ)^T dQbar, evaluated
+ * at Q.  Flou has no AD rule for rhs!; this backs the ChainRulesCore rrule of the binding.
+ * Supported: the StdAverage surface average with or without LxF dissipation (not
+ * MatrixDissipation or the Chandrasekhar surface average: FLOU_HIP_EUNSUPPORTED).
+ * Qbar must not overlap Q or dQbar. */
 int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void* Qbar);
+
+/* One RK step of FlouTime.timeintegrate (FlouTime.jl:34-56; the OrdinaryDiffEq
+ * perform_step! forms of flou_hip_rk_stage) that keeps the stage inputs for the adjoint:
+ * stages[0] = u_n (read), stages[1..S-1] receive the inputs of stages 1..S-1, u_next the
+ * result (S = 3 for SSPRK33, 5 for CK2N54 and ORK256).  All buffers distinct. */
+int flou_hip_rk_step_record(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                            void* const* stages, void* u_next);
+
+/* Adjoint of one RK step at fixed dt (no reference counterpart; the reverse sweep of the
+ * checkpointed rrule of timeintegrate): on entry ubar = cotangent of u_{n+1}, on return
+ * the cotangent of u_n.  stages[] as written by flou_hip_rk_step_record.  Uses three
+ * handle-owned work buffers; ubar must not overlap any stage buffer. */
+int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                         const void* const* stages, void* ubar);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,27 @@ def random_euler_state(X, nd, seed, amp=0.1, smooth=True):
     return np.asfortranarray(Q)
 
 
+def periodic_euler_state(X, nd, length, seed, amp=0.1, nmodes=4):
+    """Seeded smooth state that is periodic on [0, length]^nd: each primitive is a sum of
+    nmodes Fourier modes with integer wavenumbers 1..2 per axis and random phases."""
+    rng = np.random.default_rng(seed)
+    nn = X.shape[0]
+    prim = np.zeros((nn, nd + 2))
+    for v in range(nd + 2):
+        for _ in range(nmodes):
+            k = rng.integers(1, 3, size=nd) * (2 * np.pi / length)
+            prim[:, v] += np.sin(X @ k + rng.uniform(0, 2 * np.pi)) / nmodes
+    prim[:, 0] = 1.0 + amp * prim[:, 0]
+    prim[:, 1:nd + 1] *= 3 * amp
+    prim[:, nd + 1] = 1.0 + amp * prim[:, nd + 1]
+    Q = np.zeros_like(prim)
+    Q[:, 0] = prim[:, 0]
+    u2 = np.sum(prim[:, 1:nd + 1] ** 2, axis=1)
+    Q[:, 1:nd + 1] = prim[:, :1] * prim[:, 1:nd + 1]
+    Q[:, nd + 1] = prim[:, nd + 1] / (GAMMA - 1) + prim[:, 0] * u2 / 2
+    return np.asfortranarray(Q)
+
+
 def oracle_for(nd, n, nel, start, finish, periodic, eq, eqparams, volop, tpflux, savg, sdiss,
                intensity=1.0, nodes="GLL"):
     ops = O.build_ops(n, nodes)
