@@ -127,6 +127,78 @@ def measured_traffic():
     return None
 
 
+def adjoint_ratio(nel=128, n=6, nsteps=200, checkpoint_every=20, seed=(101, 202)):
+    """Adjoint workload: 2-D Euler on a periodic 128^2 quad mesh over [0,2]^2, N=5, FP64,
+    split Chandrasekhar + LxF(StdAverage), SSPRK33 at a fixed dt (CFL 0.15 of state A; 0.3
+    is beyond the SSPRK33 stability limit of this N=5 split form).
+    Loss ||ρ_A(T) - ρ_B(T)||²_{L2,Jω} at T = 200 steps; gradient w.r.t. the initial state
+    by the checkpointed reverse sweep (checkpoints every 20 steps, recompute in between).
+    Returns reverse-sweep time (recompute + VJP) / forward time over the same steps."""
+    import torch
+    import flouhip as F
+    from flouhip import adjoint as A
+    b = F.LagrangeBasis("GLL", n)
+    std = F.StdQuad(b, F.DGSEMrec(b), 4)
+    mesh = F.CartesianMesh(2, (0, 0), (2.0, 2.0), (nel, nel))
+    mesh.apply_periodicBCs(("1", "2"), ("3", "4"))
+    eq = F.EulerEquation(2, GAMMA)
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    disc = F.MultielementDisc(mesh, std, eq, op, precision="fp64")
+    xi = torch.tensor(std.basis.ξ, dtype=torch.float64, device="cuda")
+    e = torch.arange(disc.nelements, device="cuda")
+    a = torch.arange(n * n, device="cuda")
+    dx = 2.0 / nel
+    x = ((e % nel)[:, None] + (xi[a % n][None, :] + 1) / 2).reshape(-1) * dx
+    y = ((e // nel)[:, None] + (xi[a // n][None, :] + 1) / 2).reshape(-1) * dx
+
+    def state(sd):
+        g = torch.Generator(device="cpu")
+        g.manual_seed(sd)
+        prim = []
+        for v in range(4):
+            f = torch.zeros_like(x)
+            for _ in range(4):
+                kx, ky = (torch.randint(1, 3, (2,), generator=g) * math.pi).tolist()
+                ph = float(torch.rand(1, generator=g)) * 2 * math.pi
+                f += torch.sin(kx * x + ky * y + ph) / 4
+            prim.append(f)
+        rho, u, v, p = 1 + 0.1 * prim[0], 0.3 * prim[1], 0.3 * prim[2], 1 + 0.1 * prim[3]
+        return torch.stack([rho, rho * u, rho * v, p / (GAMMA - 1) + rho * (u * u + v * v) / 2]).contiguous()
+
+    QA, QB = state(seed[0]), state(seed[1])
+    dt = F.get_max_dt(QA, disc, eq, 0.15)
+    w1 = torch.tensor(std.basis.ω, dtype=torch.float64, device="cuda")
+    wnode = ((w1[a % n] * w1[a // n]) * (dx / 2) ** 2).repeat(disc.nelements)
+    uB, _ = A.forward(disc, F.SSPRK33, QB, dt, nsteps)
+    disc.check()
+    rhoB = uB[0].clone()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    Q = QA.clone()
+    F.timeintegrate(Q.clone(), disc, eq, F.SSPRK33, 5 * dt, dt=dt)  # warm-up
+    torch.cuda.synchronize()
+    ev[0].record(stream)
+    F.timeintegrate(Q, disc, eq, F.SSPRK33, nsteps * dt, dt=dt)
+    ev[1].record(stream)
+    uT, cps = A.forward(disc, F.SSPRK33, QA, dt, nsteps, checkpoint_every)
+    ubar = torch.zeros_like(uT)
+    ubar[0] = 2.0 * wnode * (uT[0] - rhoB)
+    torch.cuda.synchronize()
+    ev[2].record(stream)
+    A.reverse(disc, F.SSPRK33, cps, dt, nsteps, ubar, checkpoint_every)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    disc.check()
+    fwd_ms, rev_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    gnorm = float(torch.linalg.norm(ubar))
+    return {"value": rev_ms / fwd_ms, "forward_ms": fwd_ms, "reverse_ms": rev_ms,
+            "gradient_norm": gnorm,
+            "workload": "2-D Euler periodic 128^2 quads on [0,2]^2, N=5, FP64, split Chandrasekhar + "
+                        "LxF(StdAverage), SSPRK33 x 200 fixed-dt steps (CFL 0.15), L2 density misfit, "
+                        "checkpoints every 20 steps with recompute; value = reverse sweep "
+                        "(recompute + VJP) / forward"}
+
+
 def cpu_baseline(nel=32, steps=6):
     """Oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on a 32^3 sample of the same
     workload (about 10 s of CPU work), CK2N54 steps, DOF-updates/s per stage."""
@@ -168,6 +240,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
                     help="time one form only (profiling runs)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain copies of 1 GiB (FP32), the reference kernel for "
                          "scaling FETCH_SIZE / WRITE_SIZE counters")
@@ -253,6 +326,10 @@ def main():
         out["density_rel_l2_err_fp32_vs_fp64"] = float(_t.linalg.norm(d) / _t.linalg.norm(r64["rho"]))
         out["density_err_after_steps"] = args.steps + args.warmup
     out["adjoint_forward_ratio"] = None
+    if not args.no_adjoint:
+        adj = adjoint_ratio()
+        out["adjoint_forward_ratio"] = adj["value"]
+        out["adjoint"] = adj
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -81,30 +81,37 @@ def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtyp
     return stages[0], cps
 
 
+def reverse(disc, solver, checkpoints, dt, nsteps, ubar, checkpoint_every):
+    """Reverse sweep: ubar (cotangent of u_T, modified in place) -> cotangent of u_0, from
+    the checkpoints of ``forward``.  Per segment one forward recomputation that keeps the
+    stage inputs, then the reverse steps."""
+    code = _solver_code(solver)
+    S = _nstages(code)
+    K = max(1, int(checkpoint_every))
+    seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
+    nxt = disc.new_state()
+    for sgi in reversed(range(len(checkpoints))):
+        n0 = sgi * K
+        nloc = min(K, nsteps - n0)
+        seg_bufs[0][0].copy_(checkpoints[sgi])
+        for i in range(nloc):
+            dst = seg_bufs[i + 1][0] if i + 1 < nloc else nxt
+            rk_step_record(disc, solver, dt, seg_bufs[i], dst)
+        for i in reversed(range(nloc)):
+            rk_step_vjp(disc, solver, dt, seg_bufs[i], ubar)
+    return ubar
+
+
 def gradient(disc, solver, Q0, dt, nsteps, seed, checkpoint_every=20, checkpoint_dtype=None,
              return_state=False):
     """Cotangent of the initial state for a loss of the final state.
 
     seed(u_T) returns dL/du_T (a device state tensor).  Memory: nsteps/checkpoint_every
     checkpoints plus checkpoint_every x (stages) states for the segment being reversed."""
-    code = _solver_code(solver)
-    S = _nstages(code)
     K = max(1, int(checkpoint_every))
-    wdt = disc.torch_dtype()
     uT, cps = forward(disc, solver, Q0, dt, nsteps, K, checkpoint_dtype)
-    ubar = seed(uT).to(wdt).contiguous()
-    nseg = len(cps)
-    seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
-    nxt = disc.new_state()
-    for sgi in reversed(range(nseg)):
-        n0 = sgi * K
-        nloc = min(K, nsteps - n0)
-        seg_bufs[0][0].copy_(cps[sgi])
-        for i in range(nloc):
-            dst = seg_bufs[i + 1][0] if i + 1 < nloc else nxt
-            rk_step_record(disc, solver, dt, seg_bufs[i], dst)
-        for i in reversed(range(nloc)):
-            rk_step_vjp(disc, solver, dt, seg_bufs[i], ubar)
+    ubar = seed(uT).to(disc.torch_dtype()).contiguous().clone()
+    reverse(disc, solver, cps, dt, nsteps, ubar, K)
     return (ubar, uT) if return_state else ubar
 
 
@@ -141,18 +148,7 @@ class TimeIntegrate(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, guT):
-        disc, solver, dt, nsteps, K = ctx.disc, ctx.solver, ctx.dt, ctx.nsteps, ctx.K
-        code = _solver_code(solver)
-        S = _nstages(code)
+        disc = ctx.disc
         ubar = guT.to(disc.torch_dtype()).contiguous().clone()
-        bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
-        nxt = disc.new_state()
-        for sgi in reversed(range(len(ctx.cps))):
-            n0 = sgi * K
-            nloc = min(K, nsteps - n0)
-            bufs[0][0].copy_(ctx.cps[sgi])
-            for i in range(nloc):
-                rk_step_record(disc, solver, dt, bufs[i], bufs[i + 1][0] if i + 1 < nloc else nxt)
-            for i in reversed(range(nloc)):
-                rk_step_vjp(disc, solver, dt, bufs[i], ubar)
+        reverse(disc, ctx.solver, ctx.cps, ctx.dt, ctx.nsteps, ubar, ctx.K)
         return ubar, None, None, None, None, None
