@@ -26,7 +26,10 @@ struct flou_hip_handle {
     double gamma = 1.4, intensity = 1.0, a[3] = {0, 0, 0}, anorm = 0.0;
     int gather = 0;
     int egeom_stride = 0;
+    int curved = 0;   // per-node geometry (GEOM_CURVED)
+    int int_npe = 1;  // node stride of the integrate weights (1: per-node Jω)
     // device tables
+    void* d_ngeom = nullptr;
     void* d_ops = nullptr;
     int* d_nb = nullptr;
     int* d_finfo = nullptr;
@@ -99,7 +102,7 @@ void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
                     h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
-                    h->d_fegeom, h->d_fdsharp};
+                    h->d_fegeom, h->d_fdsharp, h->d_ngeom};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -145,6 +148,7 @@ RhsArgs<T> make_args(flou_hip_handle* h) {
     a.egeom = (const T*)h->d_egeom;
     a.egeom_stride = h->egeom_stride;
     a.fgeom = (const T*)h->d_fgeom;
+    a.ngeom = (const T*)h->d_ngeom;
     a.ops = (const T*)h->d_ops;
     a.bcstate = (const T*)h->d_bcstate;
     a.bcconst = (const T*)h->d_bcconst;
@@ -303,9 +307,10 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     if (d->nelements < 1 || d->nelements > (int64_t)1 << 30) return bad(FLOU_HIP_EINVAL, "bad nelements");
     if (d->nfaces < 1 || d->nfaces > (int64_t)1 << 31) return bad(FLOU_HIP_EINVAL, "bad nfaces");
     if (d->nboundaries < 0) return bad(FLOU_HIP_EINVAL, "bad nboundaries");
-    if (d->geometry == FLOU_HIP_GEOM_CURVED)
-        return bad(FLOU_HIP_EUNSUPPORTED, "per-node (curved) geometry is not implemented on the GPU path");
-    if (d->geometry != FLOU_HIP_GEOM_AFFINE) return bad(FLOU_HIP_EINVAL, "unknown geometry kind");
+    if (d->geometry != FLOU_HIP_GEOM_AFFINE && d->geometry != FLOU_HIP_GEOM_CURVED)
+        return bad(FLOU_HIP_EINVAL, "unknown geometry kind");
+    if (d->geometry == FLOU_HIP_GEOM_CURVED && d->nd == 1)
+        return bad(FLOU_HIP_EINVAL, "curved geometry needs nd >= 2");
     if (!d->D || !d->Ds || !d->Dsharp || !d->l_minus || !d->l_plus || !d->dg_minus || !d->dg_plus || !d->omega)
         return bad(FLOU_HIP_EINVAL, "missing operator arrays");
     if (!d->elem_faceinds || !d->elem_facepos || !d->face_eleminds || !d->face_elempos)
@@ -379,70 +384,94 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         if (d->l_plus[j] != (j == n - 1 ? 1.0 : 0.0)) h->gather = 0;
     }
 
-    // geometry: detect the uniform (all elements alike) case
+    // geometry
     const int NG = nd * nd + 1;
-    bool uniform = true;
-    for (int64_t e = 1; e < ne && uniform; ++e) {
-        if (d->elem_jac[e] != d->elem_jac[0]) uniform = false;
-        for (int c = 0; c < nd * nd; ++c)
-            if (d->elem_metric[e * nd * nd + c] != d->elem_metric[c]) uniform = false;
-    }
-    std::vector<double> eg;
-    if (uniform) {
-        eg.resize(NG);
-        for (int c = 0; c < nd * nd; ++c) eg[c] = d->elem_metric[c];
-        eg[nd * nd] = d->elem_jac[0];
-        h->egeom_stride = 0;
-    } else {
-        eg.resize(ne * NG);
-        for (int64_t e = 0; e < ne; ++e) {
-            for (int c = 0; c < nd * nd; ++c) eg[e * NG + c] = d->elem_metric[e * nd * nd + c];
-            eg[e * NG + nd * nd] = d->elem_jac[e];
-        }
-        h->egeom_stride = NG;
-    }
-    for (int64_t e = 0; e < ne; ++e)
-        if (!(d->elem_jac[e] > 0.0)) return bad(FLOU_HIP_EINVAL, "element Jacobians must be positive");
     const int FG = 3 * nd + 1;
-    // faces: frames + jac per master position when every face with that position agrees
-    std::vector<int> rep(NF, -1);
-    bool funiform = true;
-    for (int64_t f = 0; f < nf && funiform; ++f) {
-        const int pos = d->face_elempos[f * 2];
-        if (pos < 0 || pos >= NF) return bad(FLOU_HIP_EINVAL, "bad master face position");
-        if (rep[pos] < 0) { rep[pos] = (int)f; continue; }
-        const int64_t r = rep[pos];
-        if (d->face_jac[f] != d->face_jac[r]) funiform = false;
-        for (int c = 0; c < 3 * nd; ++c)
-            if (d->face_frames[f * 3 * nd + c] != d->face_frames[r * 3 * nd + c]) funiform = false;
-    }
-    std::vector<double> fg;
-    if (funiform) {
-        fg.assign(NF * FG, 0.0);
-        for (int p = 0; p < NF; ++p) {
-            if (rep[p] < 0) continue;
-            for (int c = 0; c < 3 * nd; ++c) fg[p * FG + c] = d->face_frames[(int64_t)rep[p] * 3 * nd + c];
-            fg[p * FG + 3 * nd] = d->face_jac[rep[p]];
+    std::vector<double> eg, fg, ng;
+    bool uniform = false;  // affine: every element has the same metric and J
+    h->curved = d->geometry == FLOU_HIP_GEOM_CURVED;
+    if (h->curved) {
+        // per node: metric (column-major nd x nd) and J, stored component-major
+        // [NG][ndofs] so that a thread per node reads coalesced; per face node: frame + jac.
+        const int64_t N = h->ndofs;
+        ng.resize((size_t)NG * N);
+        for (int64_t i = 0; i < N; ++i) {
+            if (!(d->elem_jac[i] > 0.0)) return bad(FLOU_HIP_EINVAL, "node Jacobians must be positive");
+            for (int c = 0; c < nd * nd; ++c) ng[c * N + i] = d->elem_metric[i * nd * nd + c];
+            ng[(size_t)nd * nd * N + i] = d->elem_jac[i];
         }
+        eg.resize(ne * NG);  // per element: node 0 (only read as a placeholder)
+        for (int64_t e = 0; e < ne; ++e)
+            for (int c = 0; c < NG; ++c) eg[e * NG + c] = ng[(size_t)c * N + e * h->npe];
+        h->egeom_stride = NG;
+        fg.resize((size_t)nf * h->npf * FG);
+        for (int64_t q = 0; q < nf * h->npf; ++q) {
+            for (int c = 0; c < 3 * nd; ++c) fg[q * FG + c] = d->face_frames[q * 3 * nd + c];
+            fg[q * FG + 3 * nd] = d->face_jac[q];
+        }
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < NF; ++k) fgi[e * NF + k] = (int)fid[e * NF + k];
     } else {
-        fg.resize(nf * FG);
-        for (int64_t f = 0; f < nf; ++f) {
-            for (int c = 0; c < 3 * nd; ++c) fg[f * FG + c] = d->face_frames[f * 3 * nd + c];
-            fg[f * FG + 3 * nd] = d->face_jac[f];
+        uniform = true;
+        for (int64_t e = 1; e < ne && uniform; ++e) {
+            if (d->elem_jac[e] != d->elem_jac[0]) uniform = false;
+            for (int c = 0; c < nd * nd; ++c)
+                if (d->elem_metric[e * nd * nd + c] != d->elem_metric[c]) uniform = false;
         }
+        if (uniform) {
+            eg.resize(NG);
+            for (int c = 0; c < nd * nd; ++c) eg[c] = d->elem_metric[c];
+            eg[nd * nd] = d->elem_jac[0];
+            h->egeom_stride = 0;
+        } else {
+            eg.resize(ne * NG);
+            for (int64_t e = 0; e < ne; ++e) {
+                for (int c = 0; c < nd * nd; ++c) eg[e * NG + c] = d->elem_metric[e * nd * nd + c];
+                eg[e * NG + nd * nd] = d->elem_jac[e];
+            }
+            h->egeom_stride = NG;
+        }
+        for (int64_t e = 0; e < ne; ++e)
+            if (!(d->elem_jac[e] > 0.0)) return bad(FLOU_HIP_EINVAL, "element Jacobians must be positive");
+        // faces: frames + jac per master position when every face with that position agrees
+        std::vector<int> rep(NF, -1);
+        bool funiform = true;
+        for (int64_t f = 0; f < nf && funiform; ++f) {
+            const int pos = d->face_elempos[f * 2];
+            if (pos < 0 || pos >= NF) return bad(FLOU_HIP_EINVAL, "bad master face position");
+            if (rep[pos] < 0) { rep[pos] = (int)f; continue; }
+            const int64_t r = rep[pos];
+            if (d->face_jac[f] != d->face_jac[r]) funiform = false;
+            for (int c = 0; c < 3 * nd; ++c)
+                if (d->face_frames[f * 3 * nd + c] != d->face_frames[r * 3 * nd + c]) funiform = false;
+        }
+        if (funiform) {
+            fg.assign(NF * FG, 0.0);
+            for (int p = 0; p < NF; ++p) {
+                if (rep[p] < 0) continue;
+                for (int c = 0; c < 3 * nd; ++c) fg[p * FG + c] = d->face_frames[(int64_t)rep[p] * 3 * nd + c];
+                fg[p * FG + 3 * nd] = d->face_jac[rep[p]];
+            }
+        } else {
+            fg.resize(nf * FG);
+            for (int64_t f = 0; f < nf; ++f) {
+                for (int c = 0; c < 3 * nd; ++c) fg[f * FG + c] = d->face_frames[f * 3 * nd + c];
+                fg[f * FG + 3 * nd] = d->face_jac[f];
+            }
+        }
+        for (int64_t e = 0; e < ne; ++e)
+            for (int k = 0; k < NF; ++k) {
+                const int64_t f = fid[e * NF + k];
+                fgi[e * NF + k] = funiform ? d->face_elempos[f * 2] : (int)f;
+            }
     }
-    for (int64_t e = 0; e < ne; ++e)
-        for (int k = 0; k < NF; ++k) {
-            const int64_t f = fid[e * NF + k];
-            fgi[e * NF + k] = funiform ? d->face_elempos[f * 2] : (int)f;
-        }
 
     // reductions: Δx per element and quadrature weights (FP64)
-    std::vector<double> dxe(ne), jac(ne), w(h->npe);
+    std::vector<double> dxe(ne), jac(h->curved ? 0 : ne), w(h->npe);
     for (int64_t e = 0; e < ne; ++e) {
         double dx = d->elem_volume[e] / h->npe;
         dxe[e] = nd == 1 ? dx : (nd == 2 ? std::sqrt(dx) : std::cbrt(dx));
-        jac[e] = d->elem_jac[e];
+        if (!h->curved) jac[e] = d->elem_jac[e];
     }
     for (int i = 0; i < h->npe; ++i) {
         int idx = i;
@@ -452,15 +481,24 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         for (int k = 1; k < nd; ++k) x = x * wi[k];
         w[i] = x;
     }
+    h->int_npe = h->npe;
+    if (h->curved) {  // integrate with per-node weights J_i ω_i (one rounding, as Σ (Jω)_i Q_i)
+        jac.resize(h->ndofs);
+        for (int64_t i = 0; i < h->ndofs; ++i) jac[i] = d->elem_jac[i] * w[i % h->npe];
+        w.assign(1, 1.0);
+        h->int_npe = 1;
+    }
 
     // kernels
     if (h->prec == FLOU_HIP_FP64) {
-        h->l64 = nd == 1 ? plain_f64_1d(n, h->eq, h->vol, h->tp)
-                         : (nd == 2 ? plain_f64_2d(n, h->eq, h->vol, h->tp) : plain_f64_3d(n, h->eq, h->vol, h->tp));
+        h->l64 = nd == 1 ? plain_f64_1d(n, h->eq, h->vol, h->tp, h->curved)
+                         : (nd == 2 ? plain_f64_2d(n, h->eq, h->vol, h->tp, h->curved)
+                                    : plain_f64_3d(n, h->eq, h->vol, h->tp, h->curved));
         if (!h->l64) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     } else {
-        h->l32 = nd == 1 ? plain_f32_1d(n, h->eq, h->vol, h->tp)
-                         : (nd == 2 ? plain_f32_2d(n, h->eq, h->vol, h->tp) : plain_f32_3d(n, h->eq, h->vol, h->tp));
+        h->l32 = nd == 1 ? plain_f32_1d(n, h->eq, h->vol, h->tp, h->curved)
+                         : (nd == 2 ? plain_f32_2d(n, h->eq, h->vol, h->tp, h->curved)
+                                    : plain_f32_3d(n, h->eq, h->vol, h->tp, h->curved));
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
     // adjoint kernels: StdAverage surface average with or without LxF dissipation
@@ -469,11 +507,13 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     } else if (h->sdiss == FLOU_HIP_DISS_MATRIX) {
         h->vjp_why = "adjoint: MatrixDissipation is not differentiated";
     } else if (h->prec == FLOU_HIP_FP64) {
-        h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp)
-                         : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp) : vjp_f64_3d(n, h->eq, h->vol, h->tp));
+        h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp, h->curved)
+                         : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp, h->curved)
+                                    : vjp_f64_3d(n, h->eq, h->vol, h->tp, h->curved));
     } else {
-        h->v32 = nd == 1 ? vjp_f32_1d(n, h->eq, h->vol, h->tp)
-                         : (nd == 2 ? vjp_f32_2d(n, h->eq, h->vol, h->tp) : vjp_f32_3d(n, h->eq, h->vol, h->tp));
+        h->v32 = nd == 1 ? vjp_f32_1d(n, h->eq, h->vol, h->tp, h->curved)
+                         : (nd == 2 ? vjp_f32_2d(n, h->eq, h->vol, h->tp, h->curved)
+                                    : vjp_f32_3d(n, h->eq, h->vol, h->tp, h->curved));
     }
     if (!h->v64 && !h->v32 && h->vjp_why.empty()) h->vjp_why = "adjoint: no kernel for this combination";
 
@@ -483,7 +523,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable).  Anything
     // else runs the plain kernel in FP32.
     std::vector<float> fe, fds;
-    if (h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
+    if (!h->curved && h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
         h->savg == FLOU_HIP_AVG_STD && h->sdiss != FLOU_HIP_DISS_MATRIX && n >= 4 &&
         h->ndofs < (int64_t(1) << 30)) {
         bool ok = true;
@@ -544,6 +584,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     rc |= upload(h, (void**)&h->d_fid, fid);
     rc |= upload_real(h, &h->d_egeom, eg);
     rc |= upload_real(h, &h->d_fgeom, fg);
+    rc |= upload_real(h, &h->d_ngeom, ng);
     rc |= upload(h, (void**)&h->d_dxe, dxe);
     rc |= upload(h, (void**)&h->d_jac, jac);
     rc |= upload(h, (void**)&h->d_w, w);
@@ -703,10 +744,10 @@ int flou_hip_integrate(flou_hip_handle* h, const void* Q, int32_t ivar, double* 
     if (ivar < 0 || ivar >= h->nv) return fail(h, FLOU_HIP_EINVAL, "ivar out of range");
     hipError_t e;
     if (h->prec == FLOU_HIP_FP64)
-        e = launch_integrate<double>((const double*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->npe,
+        e = launch_integrate<double>((const double*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->int_npe,
                                      h->d_jac, h->d_w, h->d_partial, h->d_out, h->stream);
     else
-        e = launch_integrate<float>((const float*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->npe,
+        e = launch_integrate<float>((const float*)Q + (int64_t)ivar * h->ndofs, h->ndofs, h->int_npe,
                                     h->d_jac, h->d_w, h->d_partial, h->d_out, h->stream);
     if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, hipGetErrorString(e));
     HIPCHECK(h, hipMemcpyAsync(h->h_out, h->d_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));

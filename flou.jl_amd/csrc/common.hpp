@@ -61,7 +61,9 @@ struct RhsArgs {
     const int* fid;    // ne*2nd mesh face index
     const T* egeom;    // per element: metric (nd*nd, column-major) then J
     int egeom_stride;  // 0 when every element has the same geometry
-    const T* fgeom;    // per entry: frame (n, t, b: 3*nd) then surface J
+    const T* ngeom;    // curved: per node, component-major [nd*nd + 1][ndofs] (metric, J)
+    const T* fgeom;    // per entry: frame (n, t, b: 3*nd) then surface J; curved: per face
+                       // node, entry fgi*npf + f
     const T* ops;      // packed 1-D operators
     const T* bcstate;  // Dirichlet exterior states, nfaces * npf * nv
     const T* bcconst;  // inflow states, nboundaries * nv
@@ -138,20 +140,20 @@ typedef hipError_t (*RhsLauncherF64)(const RhsArgs<double>&, hipStream_t);
 typedef hipError_t (*RhsLauncherF32)(const RhsArgs<float>&, hipStream_t);
 
 // Generic ("plain") element kernels, one translation unit per (precision, dimension).
-RhsLauncherF64 plain_f64_1d(int n, int eq, int vol, int tp);
-RhsLauncherF64 plain_f64_2d(int n, int eq, int vol, int tp);
-RhsLauncherF64 plain_f64_3d(int n, int eq, int vol, int tp);
-RhsLauncherF32 plain_f32_1d(int n, int eq, int vol, int tp);
-RhsLauncherF32 plain_f32_2d(int n, int eq, int vol, int tp);
-RhsLauncherF32 plain_f32_3d(int n, int eq, int vol, int tp);
+RhsLauncherF64 plain_f64_1d(int n, int eq, int vol, int tp, int curved);
+RhsLauncherF64 plain_f64_2d(int n, int eq, int vol, int tp, int curved);
+RhsLauncherF64 plain_f64_3d(int n, int eq, int vol, int tp, int curved);
+RhsLauncherF32 plain_f32_1d(int n, int eq, int vol, int tp, int curved);
+RhsLauncherF32 plain_f32_2d(int n, int eq, int vol, int tp, int curved);
+RhsLauncherF32 plain_f32_3d(int n, int eq, int vol, int tp, int curved);
 
 typedef hipError_t (*VjpLauncherF64)(const VjpArgs<double>&, hipStream_t);
 typedef hipError_t (*VjpLauncherF32)(const VjpArgs<float>&, hipStream_t);
-VjpLauncherF64 vjp_f64_1d(int n, int eq, int vol, int tp);
-VjpLauncherF64 vjp_f64_2d(int n, int eq, int vol, int tp);
-VjpLauncherF64 vjp_f64_3d(int n, int eq, int vol, int tp);
-VjpLauncherF32 vjp_f32_1d(int n, int eq, int vol, int tp);
-VjpLauncherF32 vjp_f32_2d(int n, int eq, int vol, int tp);
-VjpLauncherF32 vjp_f32_3d(int n, int eq, int vol, int tp);
+VjpLauncherF64 vjp_f64_1d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF64 vjp_f64_2d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF64 vjp_f64_3d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF32 vjp_f32_1d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF32 vjp_f32_2d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF32 vjp_f32_3d(int n, int eq, int vol, int tp, int curved);
 
 }  // namespace flouhip

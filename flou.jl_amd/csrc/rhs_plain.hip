@@ -22,7 +22,7 @@
 
 namespace flouhip {
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP>
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a) {
     using C = Cfg<ND, N>;
     constexpr int NPE = C::NPE, NPF = C::NPF, NF = C::NF, EPB = C::EPB, TPB = C::TPB;
@@ -71,10 +71,17 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
             sP[le][NV][node] = P[0] / (T(2) * P[ND + 1]);
         }
         if constexpr (VOL == VOL_STRONG) {
+            const T* Mp = &sG[le][0];
+            T Mn[ND * ND];
+            if constexpr (CURV) {
+#pragma unroll
+                for (int c = 0; c < ND * ND; ++c) Mn[c] = a.ngeom[c * Nd + (int64_t)e * NPE + node];
+                Mp = Mn;
+            }
 #pragma unroll
             for (int dir = 0; dir < ND; ++dir) {
                 T Ft[NV];
-                plain::contravariant_flux<T, ND, EQ>(q, &sG[le][0], dir, a.gamma, avel, Ft);
+                plain::contravariant_flux<T, ND, EQ>(q, Mp, dir, a.gamma, avel, Ft);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) sFt[le][dir * NV + v][node] = Ft[v];
             }
@@ -104,7 +111,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         const int slot = e2 * NF + k;
         const int info = a.finfo[slot];
         const int nbe = a.nb[slot];
-        const T* g = a.fgeom + (int64_t)a.fgi[slot] * (3 * ND + 1);
+        const T* g = a.fgeom + ((int64_t)a.fgi[slot] * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
         if (nbe >= 0) {
             const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
             const T* Qn = a.Q + (int64_t)nbe * NPE;
@@ -178,8 +185,16 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     T acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = T(0);
+    const int64_t gnode = (int64_t)e * NPE + node;
     const T* M = &sG[le][0];
-    const T J = sG[le][ND * ND];
+    T J = sG[le][ND * ND];
+    T Mnode[ND * ND];
+    if constexpr (CURV) {
+#pragma unroll
+        for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + gnode];
+        J = a.ngeom[ND * ND * Nd + gnode];
+        M = Mnode;
+    }
 #pragma unroll
     for (int dir = 0; dir < ND; ++dir) {
         int f, ii;
@@ -221,12 +236,19 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
                         for (int v = 0; v < NV; ++v) Pj[v] = sP[le][v][nj];
                         bj = sP[le][NV][nj];
                     }
+                    const T* Mj = M + ND * dir;
+                    T Mjc[ND];
+                    if constexpr (CURV) {
+#pragma unroll
+                        for (int k = 0; k < ND; ++k) Mjc[k] = a.ngeom[(k + ND * dir) * Nd + (int64_t)e * NPE + nj];
+                        Mj = Mjc;
+                    }
                     // F♯ entries are twopointflux(Q_lower position, Q_higher position)
                     if (j > ii)
                         plain::twopointflux<T, ND, EQ, TP>(qi, qj, Pi, Pj, bi, bj, M + ND * dir,
-                                                           M + ND * dir, a.gamma, avel, a.errflag, F);
+                                                           Mj, a.gamma, avel, a.errflag, F);
                     else
-                        plain::twopointflux<T, ND, EQ, TP>(qj, qi, Pj, Pi, bj, bi, M + ND * dir,
+                        plain::twopointflux<T, ND, EQ, TP>(qj, qi, Pj, Pi, bj, bi, Mj,
                                                            M + ND * dir, a.gamma, avel, a.errflag, F);
                 }
 #pragma unroll
@@ -248,7 +270,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     }
 
     // Mass matrix (ldiv! by Diagonal(J)) and the stage epilogue
-    const int64_t gi = (int64_t)e * NPE + node;
+    const int64_t gi = gnode;
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         const T kv = acc[v] / J;
@@ -277,38 +299,38 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     }
 }
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP>
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 static hipError_t launch_plain(const RhsArgs<T>& a, hipStream_t s) {
     using C = Cfg<ND, N>;
     const int nblocks = (a.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_plain<T, ND, N, EQ, VOL, TP>), dim3(nblocks), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_plain<T, ND, N, EQ, VOL, TP, CURV>), dim3(nblocks), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int ND, int N>
+template <typename T, int ND, int N, bool CURV>
 static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
     if (eq == EQ_ADVECTION)
-        return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0>
-                                 : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0>;
-    if (vol == VOL_STRONG) return &launch_plain<T, ND, N, EQ_EULER, VOL_STRONG, 0>;
+        return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
+                                 : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
+    if (vol == VOL_STRONG) return &launch_plain<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV>;
     switch (tp) {
-        case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG>;
-        case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR>;
-        case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER>;
+        case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG, CURV>;
+        case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR, CURV>;
+        case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER, CURV>;
         default: return nullptr;
     }
 }
 
-template <typename T, int ND>
+template <typename T, int ND, bool CURV>
 static hipError_t (*pick_plain_n(int n, int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
     switch (n) {
-        case 2: return pick_plain<T, ND, 2>(eq, vol, tp);
-        case 3: return pick_plain<T, ND, 3>(eq, vol, tp);
-        case 4: return pick_plain<T, ND, 4>(eq, vol, tp);
-        case 5: return pick_plain<T, ND, 5>(eq, vol, tp);
-        case 6: return pick_plain<T, ND, 6>(eq, vol, tp);
-        case 7: return pick_plain<T, ND, 7>(eq, vol, tp);
-        case 8: return pick_plain<T, ND, 8>(eq, vol, tp);
+        case 2: return pick_plain<T, ND, 2, CURV>(eq, vol, tp);
+        case 3: return pick_plain<T, ND, 3, CURV>(eq, vol, tp);
+        case 4: return pick_plain<T, ND, 4, CURV>(eq, vol, tp);
+        case 5: return pick_plain<T, ND, 5, CURV>(eq, vol, tp);
+        case 6: return pick_plain<T, ND, 6, CURV>(eq, vol, tp);
+        case 7: return pick_plain<T, ND, 7, CURV>(eq, vol, tp);
+        case 8: return pick_plain<T, ND, 8, CURV>(eq, vol, tp);
         default: return nullptr;
     }
 }
@@ -336,9 +358,14 @@ static hipError_t (*pick_plain_n(int n, int eq, int vol, int tp))(const RhsArgs<
 #error "define FLOUHIP_ND"
 #endif
 
-hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp))(
+hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp, int curved))(
     const RhsArgs<FLOUHIP_SCALAR>&, hipStream_t) {
-    return pick_plain_n<FLOUHIP_SCALAR, FLOUHIP_ND>(n, eq, vol, tp);
+#if FLOUHIP_ND > 1
+    if (curved) return pick_plain_n<FLOUHIP_SCALAR, FLOUHIP_ND, true>(n, eq, vol, tp);
+#else
+    if (curved) return nullptr;
+#endif
+    return pick_plain_n<FLOUHIP_SCALAR, FLOUHIP_ND, false>(n, eq, vol, tp);
 }
 
 }  // namespace flouhip

@@ -23,7 +23,7 @@
 
 namespace flouhip {
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP>
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     using C = Cfg<ND, N>;
     constexpr int NPE = C::NPE, NPF = C::NPF, NF = C::NF, EPB = C::EPB, TPB = C::TPB;
@@ -55,7 +55,8 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         if (e0 + le < a.ne) {
             const int64_t g = v * Nd + (int64_t)e0 * NPE + j;
             sQ[le][v][nd] = a.Q[g];
-            sW[le][v][nd] = A.c * A.Y[g] / sG[le][ND * ND];
+            const T J = CURV ? a.ngeom[ND * ND * Nd + (int64_t)e0 * NPE + j] : sG[le][ND * ND];
+            sW[le][v][nd] = A.c * A.Y[g] / J;
         }
     }
     __syncthreads();
@@ -84,7 +85,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         const int info = a.finfo[slot];
         const int nbe = a.nb[slot];
         const bool slave = finfo_side(info);
-        const T* g = a.fgeom + (int64_t)a.fgi[slot] * (3 * ND + 1);
+        const T* g = a.fgeom + ((int64_t)a.fgi[slot] * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
         const T jac = g[3 * ND];
         T Fb[NV];
 #pragma unroll
@@ -100,12 +101,13 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             for (int v = 0; v < NV; ++v) {
                 T s = T(0), t = T(0);
                 for (int p = 0; p < N; ++p) {
-                    const int64_t gi = v * Nd + base + line_node<ND, N>(ndir, f, p);
+                    const int64_t nn = base + line_node<ND, N>(ndir, f, p);
+                    const int64_t gi = v * Nd + nn;
                     s = s + lq[p] * a.Q[gi];
-                    t = t + gq[p] * A.Y[gi];
+                    t = t + gq[p] * (CURV ? A.Y[gi] / a.ngeom[ND * ND * Nd + nn] : A.Y[gi]);
                 }
                 ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, f, nlr ? N - 1 : 0)] : s;
-                const T Ln = A.c * t / Jn;
+                const T Ln = CURV ? A.c * t : A.c * t / Jn;
                 Fb[v] = Fb[v] + (slave ? -Ln : Ln);  // -σ_nb L_nb, σ_nb = -σ_own
             }
         } else {
@@ -188,6 +190,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         V[v] = T(0);
     }
     const T* M = &sG[le][0];
+    T Mnode[ND * ND];
+    if constexpr (CURV) {
+#pragma unroll
+        for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + (int64_t)e * NPE + node];
+        M = Mnode;
+    }
 #pragma unroll
     for (int dir = 0; dir < ND; ++dir) {
         int f, ii;
@@ -239,13 +247,17 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
                         qj[v] = sQ[le][v][nj];
                         Gb[v] = -dij * wi[v] - dji * sW[le][v][nj];
                     }
+                    T nb[ND];  // averaged metric (Ja_i + Ja_j)/2 of the pair
+#pragma unroll
+                    for (int k = 0; k < ND; ++k)
+                        nb[k] = CURV ? (n[k] + a.ngeom[(k + ND * dir) * Nd + (int64_t)e * NPE + nj]) / T(2) : n[k];
                     if constexpr (EQ == EQ_ADVECTION) {
                         T an = T(0);
 #pragma unroll
-                        for (int k = 0; k < ND; ++k) an = an + avel[k] * n[k];
+                        for (int k = 0; k < ND; ++k) an = an + avel[k] * nb[k];
                         V[0] = V[0] + an * Gb[0] / T(2);
                     } else {
-                        adj::twopoint_vjp1<T, ND, TP>(qi, qj, n, a.gamma, Gb, V);
+                        adj::twopoint_vjp1<T, ND, TP>(qi, qj, nb, a.gamma, Gb, V);
                     }
                 }
             }
@@ -275,38 +287,38 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     }
 }
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP>
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 static hipError_t launch_vjp(const VjpArgs<T>& a, hipStream_t s) {
     using C = Cfg<ND, N>;
     const int nblocks = (a.r.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_vjp<T, ND, N, EQ, VOL, TP>), dim3(nblocks), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_vjp<T, ND, N, EQ, VOL, TP, CURV>), dim3(nblocks), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
-template <typename T, int ND, int N>
+template <typename T, int ND, int N, bool CURV>
 static hipError_t (*pick_vjp(int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
     if (eq == EQ_ADVECTION)
-        return vol == VOL_STRONG ? &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0>
-                                 : &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0>;
-    if (vol == VOL_STRONG) return &launch_vjp<T, ND, N, EQ_EULER, VOL_STRONG, 0>;
+        return vol == VOL_STRONG ? &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
+                                 : &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
+    if (vol == VOL_STRONG) return &launch_vjp<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV>;
     switch (tp) {
-        case TP_STDAVG: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG>;
-        case TP_CHANDRASEKHAR: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR>;
-        case TP_KENNEDY_GRUBER: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER>;
+        case TP_STDAVG: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG, CURV>;
+        case TP_CHANDRASEKHAR: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR, CURV>;
+        case TP_KENNEDY_GRUBER: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER, CURV>;
         default: return nullptr;
     }
 }
 
-template <typename T, int ND>
+template <typename T, int ND, bool CURV>
 static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
     switch (n) {
-        case 2: return pick_vjp<T, ND, 2>(eq, vol, tp);
-        case 3: return pick_vjp<T, ND, 3>(eq, vol, tp);
-        case 4: return pick_vjp<T, ND, 4>(eq, vol, tp);
-        case 5: return pick_vjp<T, ND, 5>(eq, vol, tp);
-        case 6: return pick_vjp<T, ND, 6>(eq, vol, tp);
-        case 7: return pick_vjp<T, ND, 7>(eq, vol, tp);
-        case 8: return pick_vjp<T, ND, 8>(eq, vol, tp);
+        case 2: return pick_vjp<T, ND, 2, CURV>(eq, vol, tp);
+        case 3: return pick_vjp<T, ND, 3, CURV>(eq, vol, tp);
+        case 4: return pick_vjp<T, ND, 4, CURV>(eq, vol, tp);
+        case 5: return pick_vjp<T, ND, 5, CURV>(eq, vol, tp);
+        case 6: return pick_vjp<T, ND, 6, CURV>(eq, vol, tp);
+        case 7: return pick_vjp<T, ND, 7, CURV>(eq, vol, tp);
+        case 8: return pick_vjp<T, ND, 8, CURV>(eq, vol, tp);
         default: return nullptr;
     }
 }
@@ -334,9 +346,14 @@ static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp))(const VjpArgs<T>
 #error "define FLOUHIP_ND"
 #endif
 
-hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp))(
+hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp, int curved))(
     const VjpArgs<FLOUHIP_SCALAR>&, hipStream_t) {
-    return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND>(n, eq, vol, tp);
+#if FLOUHIP_ND > 1
+    if (curved) return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, true>(n, eq, vol, tp);
+#else
+    if (curved) return nullptr;
+#endif
+    return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, false>(n, eq, vol, tp);
 }
 
 }  // namespace flouhip

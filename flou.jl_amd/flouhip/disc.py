@@ -17,6 +17,7 @@ import numpy as np
 
 from . import _capi as C
 from .equations import EulerEquation, LinearAdvection, _surface_codes
+from .mesh import curved_geometry
 from .stdregions import StdHex, StdQuad, StdSegment
 
 try:
@@ -96,9 +97,13 @@ class MultielementDisc:
         d.face_elempos = _ptr(arr(mesh.face_pos, np.int8), ctypes.c_int8)
         d.face_orientation = None
         d.face_boundary = _ptr(arr(mesh.face_boundary, np.int32), ctypes.c_int32)
-        ejac, emet, evol = mesh.element_geometry(std)
-        ffr, fjac = mesh.face_geometry()
-        d.geometry = C.GEOM_AFFINE
+        if getattr(mesh, "mapping", None) is None:
+            ejac, emet, evol = mesh.element_geometry(std)
+            ffr, fjac = mesh.face_geometry()
+            d.geometry = C.GEOM_AFFINE
+        else:
+            ejac, emet, evol, ffr, fjac = curved_geometry(mesh, std)
+            d.geometry = C.GEOM_CURVED
         d.elem_jac = _ptr(arr(ejac, np.float64), ctypes.c_double)
         d.elem_metric = _ptr(arr(emet, np.float64), ctypes.c_double)
         d.elem_volume = _ptr(arr(evol, np.float64), ctypes.c_double)

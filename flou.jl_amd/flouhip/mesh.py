@@ -216,6 +216,14 @@ class CartesianMesh:
         return self
 
     # ------------------------------------------------------------------ geometry
+    def apply_mapping(self, fn):
+        """Curve the mesh: every node x of the Cartesian mesh moves to fn(x) (fn maps an
+        (m, nd) array of points to an (m, nd) array).  Geometry then becomes per node,
+        computed from the mapped nodes of each element (curved_geometry).  A periodic
+        mesh stays periodic when fn(x + L e_k) = fn(x) + L e_k."""
+        self.mapping = fn
+        return self
+
     def element_geometry(self, std):
         """Per-element J, metric (column-major nd x nd) and volume (PhysicalRegions.jl:370-408)."""
         nd, dx = self.nd, self.Δx
@@ -264,7 +272,13 @@ class CartesianMesh:
         return frames[mp], fjac[mp]
 
     def node_coords(self, std):
-        """Coordinates of every solution node, (ndofs, nd) (Mesh.jl:380-446)."""
+        """Coordinates of every solution node, (ndofs, nd) (Mesh.jl:380-446), mapped when
+        the mesh carries a mapping."""
+        X = self._cartesian_node_coords(std)
+        fn = getattr(self, "mapping", None)
+        return X if fn is None else np.asarray(fn(X), dtype=np.float64)
+
+    def _cartesian_node_coords(self, std):
         nd, n = self.nd, std.n
         r = (std.basis.ξ + 1.0) / 2.0
         ne = self.nelements
@@ -341,3 +355,94 @@ def _line_node(nd, n, d, f, p):
 
 def apply_periodicBCs(mesh, *pairs):
     return mesh.apply_periodicBCs(*pairs)
+
+
+def _dref(u, D, nd, n, k):
+    """Derivative along reference direction k of nodal fields u (ne, n^nd), x fastest."""
+    ne = u.shape[0]
+    if nd == 1:
+        return u @ D.T
+    if nd == 2:
+        v = u.reshape(ne, n, n)  # [e, iy, ix]
+        r = np.einsum("ij,eyj->eyi", D, v) if k == 0 else np.einsum("ij,ejx->eix", D, v)
+        return r.reshape(ne, n * n)
+    v = u.reshape(ne, n, n, n)  # [e, iz, iy, ix]
+    if k == 0:
+        r = np.einsum("ij,ezyj->ezyi", D, v)
+    elif k == 1:
+        r = np.einsum("ij,ezjx->ezix", D, v)
+    else:
+        r = np.einsum("ij,ejyx->eiyx", D, v)
+    return r.reshape(ne, n ** 3)
+
+
+def curved_geometry(mesh, std):
+    """Per-node geometry of a mapped mesh from its mapped GLL nodes.
+
+    Metric terms Ja^i (columns of `metric`, PhysicalRegions.jl:1001-1033 convention) in the
+    conservative curl form  Ja^i_n = -x̂_i · ∇_ξ × I^N(X_l ∇_ξ X_m), (n, m, l) cyclic
+    (Kopriva, J. Sci. Comput. 26 (2006)), which satisfies the discrete metric identities
+    Σ_i D_i Ja^i = 0 and so preserves free streams; in 2-D Ja^1 = (y_η, -x_η),
+    Ja^2 = (-y_ξ, x_ξ).  J = a_1·(a_2 × a_3) from the interpolant's covariant basis.  Faces:
+    per face node of the master element, n = ±Ja^d/|Ja^d| (outward), surface J = |Ja^d|,
+    t, b completing an orthonormal frame.
+
+    Returns (node_jac (ndofs,), node_metric (ndofs, nd*nd) column-major per node,
+    elem_volume (ne,), face_frames (nfaces*npf, 3, nd), face_jac (nfaces*npf,))."""
+    nd, n = mesh.nd, std.n
+    ne, npe = mesh.nelements, n ** nd
+    D = np.asarray(std.D, dtype=np.float64)
+    X = mesh.node_coords(std).reshape(ne, npe, nd)
+    # The curl form is invariant under a constant shift of X; shifting each element to its
+    # first node keeps the products X_l ∂X_m small and their round-off at the metric's scale.
+    x = [X[:, :, c] - X[:, :1, c] for c in range(nd)]
+    dX = [[_dref(x[c], D, nd, n, k) for k in range(nd)] for c in range(nd)]  # dX[c][k] = ∂_k X_c
+    Ja = np.zeros((ne, npe, nd, nd))  # Ja[..., r, c] = (Ja^c)_r = metric[r, c]
+    if nd == 1:
+        Ja[:, :, 0, 0] = 1.0
+        J = dX[0][0]
+    elif nd == 2:
+        Ja[:, :, 0, 0], Ja[:, :, 1, 0] = dX[1][1], -dX[0][1]
+        Ja[:, :, 0, 1], Ja[:, :, 1, 1] = -dX[1][0], dX[0][0]
+        J = dX[0][0] * dX[1][1] - dX[0][1] * dX[1][0]
+    else:
+        for nn in range(3):
+            m, l = (nn + 1) % 3, (nn + 2) % 3
+            V = [x[l] * dX[m][k] for k in range(3)]
+            Ja[:, :, nn, 0] = -(_dref(V[2], D, 3, n, 1) - _dref(V[1], D, 3, n, 2))
+            Ja[:, :, nn, 1] = -(_dref(V[0], D, 3, n, 2) - _dref(V[2], D, 3, n, 0))
+            Ja[:, :, nn, 2] = -(_dref(V[1], D, 3, n, 0) - _dref(V[0], D, 3, n, 1))
+        a = [np.stack([dX[c][k] for c in range(3)], -1) for k in range(3)]
+        J = np.einsum("eni,eni->en", a[0], np.cross(a[1], a[2]))
+    if np.any(J <= 0):
+        raise ValueError("mapped mesh has a non-positive Jacobian")
+    w = np.asarray(std.ω, dtype=np.float64)
+    evol = (J * w[None, :]).sum(axis=1)
+    metric = np.transpose(Ja, (0, 1, 3, 2)).reshape(ne * npe, nd * nd)  # column-major blocks
+    # faces, from the master element
+    npf = n ** (nd - 1)
+    nf = mesh.nfaces
+    frames = np.zeros((nf, npf, 3, nd))
+    fjac = np.zeros((nf, npf))
+    e_m, pos = mesh.face_elems[:, 0], mesh.face_pos[:, 0]
+    for f_ in range(npf):
+        for p in range(2 * nd):
+            sel = np.nonzero(pos == p)[0]
+            if sel.size == 0:
+                continue
+            d, lr = p // 2, p % 2
+            node = _line_node(nd, n, d, f_, n - 1 if lr else 0)
+            v = Ja[e_m[sel], node, :, d]  # Ja^d at the face node
+            nrm = np.linalg.norm(v, axis=1)
+            nv = (1.0 if lr else -1.0) * v / nrm[:, None]
+            fjac[sel, f_] = nrm
+            frames[sel, f_, 0] = nv
+            if nd == 2:
+                frames[sel, f_, 1] = np.stack([-nv[:, 1], nv[:, 0]], 1)
+            elif nd == 3:
+                ref = np.where(np.abs(nv[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
+                t = ref - np.sum(ref * nv, 1)[:, None] * nv
+                t /= np.linalg.norm(t, axis=1)[:, None]
+                frames[sel, f_, 1] = t
+                frames[sel, f_, 2] = np.cross(nv, t)
+    return (J.reshape(-1), metric, evol, frames.reshape(nf * npf, 3, nd), fjac.reshape(-1))

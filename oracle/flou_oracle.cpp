@@ -554,6 +554,21 @@ oracle_handle* oracle_create(int nd, int n, const int* nel, const double* start,
 
 void oracle_destroy(oracle_handle* h) { delete h; }
 
+// Per-node geometry of a mapped (curved) mesh, computed by the caller: node_metric
+// (ndofs*nd*nd, column-major per node), node_jac (ndofs), face_frames (nf*npf*3*nd, per
+// face node of the master side), face_jac (nf*npf), elem_volume (ne).
+void oracle_set_curved(oracle_handle* h, const double* node_metric, const double* node_jac,
+                       const double* face_frames, const double* face_jac, const double* elem_volume) {
+    Disc& d = h->d;
+    const size_t N = (size_t)d.ndofs();
+    d.curved = true;
+    d.n_metric.assign(node_metric, node_metric + N * d.nd * d.nd);
+    d.n_jac.assign(node_jac, node_jac + N);
+    d.fn_frame.assign(face_frames, face_frames + (size_t)d.nf * d.npf * 3 * d.nd);
+    d.fn_jac.assign(face_jac, face_jac + (size_t)d.nf * d.npf);
+    d.e_vol.assign(elem_volume, elem_volume + d.ne);
+}
+
 void oracle_set_option(oracle_handle* h, int which, int value) {
     if (which == 0) h->d.chandrasekhar_surface_typo = value;
 }

@@ -134,6 +134,20 @@ struct Disc {
     std::vector<double> e_vol;   // ne
     std::vector<double> f_frame; // nf * 3 * nd : (n, t, b)
     std::vector<double> f_jac;   // nf
+    // curved (per-node) geometry, used instead of the per-element/per-face arrays above
+    bool curved = false;
+    std::vector<double> n_jac;     // ndofs
+    std::vector<double> n_metric;  // ndofs * nd * nd, column-major per node
+    std::vector<double> fn_frame;  // nf * npf * 3 * nd, per face node of the master side
+    std::vector<double> fn_jac;    // nf * npf
+    const double* frame(long face, int fnode) const {
+        return curved ? &fn_frame[((size_t)face * npf + fnode) * 3 * nd] : &f_frame[(size_t)face * 3 * nd];
+    }
+    double facejac(long face, int fnode) const { return curved ? fn_jac[(size_t)face * npf + fnode] : f_jac[face]; }
+    const double* metric(long e, int i) const {
+        return curved ? &n_metric[((size_t)e * npe + i) * nd * nd] : &e_metric[(size_t)e * nd * nd];
+    }
+    double jac(long e, int i) const { return curved ? n_jac[(size_t)e * npe + i] : e_jac[e]; }
     std::vector<double> coords;  // ndofs * nd node coordinates
     std::vector<double> f_coords;// nf * npf * nd face node coordinates
     // equation and operators
@@ -490,7 +504,7 @@ void apply_bc(const Disc& d, const T* Qin, long face, int fnode, T* Qext) {
     const int nv = d.nv;
     const int ib = d.f_bd[face];
     const int type = d.bctype.at(ib);
-    const double* fr = &d.f_frame[(size_t)face * 3 * d.nd];
+    const double* fr = d.frame(face, fnode);
     if (type == BC_DIRICHLET) {
         // exterior state fixed per face node (GenericBC with a time-independent Qext)
         const double* s = &d.f_bcstate[((size_t)face * d.npf + fnode) * nv];
@@ -553,7 +567,6 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
 #pragma omp parallel for schedule(static)
     for (long e = 0; e < ne; ++e) try {
         const long base = e * npe;
-        const double* M = &d.e_metric[(size_t)e * nd * nd];
         std::vector<T> Qe((size_t)npe * nv), Ft((size_t)npe * nd * nv), acc((size_t)npe * nv);
         for (int i = 0; i < npe; ++i)
             for (int v = 0; v < nv; ++v) Qe[i * nv + v] = Q[(size_t)v * N + base + i];
@@ -561,7 +574,7 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
             for (int v = 0; v < nv; ++v) acc[i * nv + v] = dQ[(size_t)v * N + base + i];
         for (int i = 0; i < npe; ++i)
             for (int dir = 0; dir < nd; ++dir)
-                contravariant_flux(d, &Qe[i * nv], M, dir, &Ft[((size_t)i * nd + dir) * nv]);
+                contravariant_flux(d, &Qe[i * nv], d.metric(e, i), dir, &Ft[((size_t)i * nd + dir) * nv]);
         const int nlines = npf;
         for (int dir = 0; dir < nd; ++dir) {
             for (int f = 0; f < nlines; ++f) {
@@ -587,7 +600,8 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
                         for (int b = a + 1; b < n; ++b) {
                             const int nb = line_node(nd, n, dir, f, b);
                             T F[5];
-                            twopointflux(d, &Qe[na * nv], &Qe[nb * nv], M + nd * dir, M + nd * dir, F);
+                            twopointflux(d, &Qe[na * nv], &Qe[nb * nv], d.metric(e, na) + nd * dir,
+                                         d.metric(e, nb) + nd * dir, F);
                             for (int v = 0; v < nv; ++v) {
                                 Fs[((size_t)a * n + b) * nv + v] = F[v];
                                 Fs[((size_t)b * n + a) * nv + v] = F[v];
@@ -621,9 +635,9 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
     // interface_fluxes! (Interfaces.jl:111-136); Cartesian orientation is 0 (j == i)
 #pragma omp parallel for schedule(static)
     for (long face = 0; face < nf; ++face) try {
-        const double* fr = &d.f_frame[(size_t)face * 3 * nd];
-        const double jac = d.f_jac[face];
         for (int f = 0; f < npf; ++f) {
+            const double* fr = d.frame(face, f);
+            const double jac = d.facejac(face, f);
             const T* Ql = QF(face, 0, f);
             const T* Qr = QF(face, 1, f);
             T F[5];
@@ -679,12 +693,13 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
     // apply_massmatrix! (MultielementDiscontinuous.jl:132-137): ldiv!(Diagonal(J), dQ)
 #pragma omp parallel for schedule(static)
     for (long e = 0; e < ne; ++e) {
-        const double J = d.e_jac[e];
-        for (int i = 0; i < npe; ++i)
+        for (int i = 0; i < npe; ++i) {
+            const double J = d.jac(e, i);
             for (int v = 0; v < nv; ++v) {
                 T& x = dQ[(size_t)v * N + e * npe + i];
                 x = x / J;
             }
+        }
     }
 }
 
@@ -741,7 +756,7 @@ double integrate(const Disc& d, const T* Q, int ivar) {
             for (int k = 0; k < d.nd; ++k) { wi[k] = d.ops.w[idx % d.n]; idx /= d.n; }
             w = wi[0];
             for (int k = 1; k < d.nd; ++k) w = w * wi[k];
-            s = s + (d.e_jac[e] * w) * val(Q[(size_t)ivar * N + e * d.npe + i]);
+            s = s + (d.jac(e, i) * w) * val(Q[(size_t)ivar * N + e * d.npe + i]);
         }
         total = total + s;
     }

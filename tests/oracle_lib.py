@@ -39,6 +39,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.oracle_set_curved.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oracle_sizes.argtypes = [ctypes.c_void_p, _lp]
         L.oracle_connectivity.argtypes = [ctypes.c_void_p, _lp, _ip, _lp, _ip, _ip]
         L.oracle_geometry.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
@@ -113,6 +114,13 @@ class OracleDisc:
         if getattr(self, "h", None):
             lib().oracle_destroy(self.h)
             self.h = None
+
+    def set_curved(self, node_metric, node_jac, face_frames, face_jac, elem_volume):
+        """Per-node geometry of a mapped mesh (arrays as from flouhip.mesh.curved_geometry)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64)
+             for x in (node_metric, node_jac, face_frames, face_jac, elem_volume)]
+        self._keep_curved = a
+        lib().oracle_set_curved(self.h, *[_p(x) for x in a])
 
     def set_option(self, which, value):
         lib().oracle_set_option(self.h, which, value)

@@ -82,7 +82,7 @@ def _valid_desc(keep):
     ("volume_op", 5, C.EINVAL),
     ("tp_flux", 9, C.EINVAL),
     ("surface_diss", 4, C.EINVAL),
-    ("geometry", C.GEOM_CURVED, C.EUNSUPPORTED),
+    ("geometry", 7, C.EINVAL),
     ("gamma", 0.5, C.EINVAL),
 ])
 def test_create_rejects_invalid_descriptions(field, value, code):

@@ -1,0 +1,147 @@
+"""Curved (mapped) meshes on the GPU: per-node metric and J, per-face-node frames.
+
+FP64 plain kernels against the oracle on the same per-node geometry (≤ 1e-13), FP32 within
+2e-5, free-stream preservation, the integral with per-node Jω, the adjoint's dot-product
+identity against the oracle's dual-number JVP, and the mixed-precision adjoint of the
+curved 3-D configuration (FP32 forward and adjoint, FP64 checkpoints) against FP64.
+"""
+import numpy as np
+import pytest
+
+import flouhip as F
+import oracle_lib as O
+from cases import GAMMA, periodic_euler_state
+from flouhip import adjoint as A
+from flouhip import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+AMP = 0.05
+
+
+def warp(X):
+    s = np.ones((X.shape[0], 1))
+    for k in range(X.shape[1]):
+        s = s * np.sin(2 * np.pi * X[:, [k]])
+    return X + AMP * s
+
+
+TP = {"ch": (F.ChandrasekharAverage, O.TP_CHANDRASEKHAR), "kg": (F.KennedyGruber, O.TP_KENNEDY_GRUBER),
+      "std": (F.StdAverage, O.TP_STDAVG)}
+
+
+def build_curved(nd, n, nel, vol="split", tp="ch", precision="fp64"):
+    b = F.LagrangeBasis("GLL", n)
+    std = [None, None, F.StdQuad, F.StdHex][nd](b, F.DGSEMrec(b), nd + 2)
+    mesh = F.CartesianMesh(nd, (0.0,) * nd, (1.0,) * nd, (nel,) * nd)
+    mesh.apply_periodicBCs(*[(str(2 * k + 1), str(2 * k + 2)) for k in range(nd)])
+    mesh.apply_mapping(warp)
+    eq = F.EulerEquation(nd, GAMMA)
+    if vol == "strong":
+        op = F.StrongDivOperator(F.LxF(F.StdAverage(), 1.0))
+        volc, tpc = O.VOL_STRONG, 0
+    else:
+        op = F.SplitDivOperator(TP[tp][0](), F.LxF(F.StdAverage(), 1.0))
+        volc, tpc = O.VOL_SPLIT, TP[tp][1]
+    disc = F.MultielementDisc(mesh, std, eq, op, precision=precision)
+    J, met, ev, fr, fj = M.curved_geometry(mesh, std)
+    orc = O.OracleDisc(nd, n, [nel] * nd, [0.0] * nd, [1.0] * nd, [1] * nd, O.build_ops(n, "GLL"),
+                       O.EQ_EULER, [GAMMA], volc, tpc, O.AVG_STD, O.DISS_LXF, 1.0)
+    orc.set_curved(met, J, fr.reshape(-1), fj, ev)
+    X = mesh.node_coords(std)
+    return disc, orc, eq, X, J, std
+
+
+def rel_err(a, b):
+    scale = np.max(np.abs(b), axis=0)
+    return np.max(np.abs(a - b) / scale)
+
+
+CASES = [(2, 6, 4, "split", "ch"), (2, 5, 4, "split", "kg"), (2, 4, 4, "strong", "std"),
+         (3, 5, 3, "split", "ch"), (3, 4, 3, "strong", "std"), (3, 7, 2, "split", "kg")]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(map(str, c)))
+def test_curved_rhs_fp64_matches_oracle(case):
+    nd, n, nel, vol, tp = case
+    disc, orc, eq, X, J, std = build_curved(nd, n, nel, vol, tp)
+    Q = periodic_euler_state(X, nd, 1.0, seed=7, amp=0.2)
+    dQ = disc.new_state()
+    F.rhs_(dQ, disc.to_device(Q), F.EquationConfig(disc, eq))
+    disc.check()
+    got, ref = disc.to_host(dQ), orc.rhs(Q)
+    err = rel_err(got, ref)
+    print(f"{case}: max rel err {err:.2e}, bit-identical {np.mean(got == ref):.4f}")
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[3]], ids=["2d", "3d"])
+def test_curved_rhs_fp32_against_fp64_oracle(case):
+    nd, n, nel, vol, tp = case
+    disc, orc, eq, X, J, std = build_curved(nd, n, nel, vol, tp, precision="fp32")
+    assert not disc.uses_fast_kernel()
+    Q = periodic_euler_state(X, nd, 1.0, seed=7, amp=0.2)
+    dQ = disc.new_state()
+    F.rhs_(dQ, disc.to_device(Q), F.EquationConfig(disc, eq))
+    err = rel_err(disc.to_host(dQ).astype(np.float64), orc.rhs(Q))
+    print(f"FP32 curved {case}: max rel err {err:.2e}")
+    assert err <= 2e-5
+
+
+def test_curved_free_stream_and_integral_on_gpu():
+    disc, orc, eq, X, J, std = build_curved(3, 6, 3)
+    Q = periodic_euler_state(X, 3, 1.0, seed=7, amp=0.2)
+    Q0 = np.zeros_like(Q)
+    Q0[:, 0], Q0[:, 1:4], Q0[:, 4] = 1.2, [0.3, -0.2, 0.1], 2.5
+    dQ, dQ0 = disc.new_state(), disc.new_state()
+    cfg = F.EquationConfig(disc, eq)
+    F.rhs_(dQ, disc.to_device(Q), cfg)
+    F.rhs_(dQ0, disc.to_device(Q0), cfg)
+    fs = float(dQ0.abs().max() / dQ.abs().max())
+    print(f"curved free stream on GPU: {fs:.2e}")
+    assert fs <= 1e-12
+    Qd = disc.to_device(Q)
+    for v in range(5):
+        got = F.integrate(Qd, disc, v)
+        ref = orc.integrate(Q, v)
+        assert abs(got - ref) <= 1e-14 * abs(ref)
+
+
+@pytest.mark.parametrize("case", [CASES[0], CASES[3], CASES[4], CASES[5]], ids=["2d-ch", "3d-ch", "3d-strong", "3d-kg"])
+def test_curved_vjp_dot_product_with_oracle_jvp(case):
+    nd, n, nel, vol, tp = case
+    disc, orc, eq, X, J, std = build_curved(nd, n, nel, vol, tp)
+    Q = periodic_euler_state(X, nd, 1.0, seed=9, amp=0.2)
+    rng = np.random.default_rng(4)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    _, JV = orc.rhs_jvp(Q, V)
+    Qbar = torch.zeros((nd + 2, Q.shape[0]), dtype=torch.float64, device="cuda")
+    A.rhs_vjp(Qbar, disc.to_device(W), disc.to_device(Q), disc)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(disc.to_host(Qbar) * V))
+    err = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"curved VJP {case}: rel {err:.2e}")
+    assert err <= 1e-12
+
+
+def test_curved_mixed_precision_adjoint():
+    """3-D curved mesh, N=6: FP32 forward and adjoint with FP64 checkpoints every 10 of 30
+    SSPRK33 steps, against the FP64 adjoint of the same loss (mean-square density)."""
+    res = {}
+    for prec in ("fp64", "fp32"):
+        disc, orc, eq, X, J, std = build_curved(3, 7, 3, precision=prec)
+        Q = periodic_euler_state(X, 3, 1.0, seed=11, amp=0.1)
+        dt = 0.1 * orc.max_dt(Q, 1.0)
+        wq = torch.tensor(J * np.tile(std.ω, 27), device="cuda", dtype=disc.torch_dtype())
+
+        def seed(u):
+            g = torch.zeros_like(u)
+            g[0] = 2.0 * wq * (u[0] - 1.0)
+            return g
+
+        res[prec] = disc.to_host(A.gradient(disc, F.SSPRK33, disc.to_device(Q), dt, 30, seed,
+                                            checkpoint_every=10, checkpoint_dtype=torch.float64)).astype(np.float64)
+    err = np.linalg.norm(res["fp32"] - res["fp64"]) / np.linalg.norm(res["fp64"])
+    print(f"curved FP32 adjoint vs FP64 adjoint: relative L2 error {err:.2e}")
+    assert err <= 1e-4
