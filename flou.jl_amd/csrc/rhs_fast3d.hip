@@ -45,13 +45,16 @@ __device__ __forceinline__ float lg(float x) { return __logf(x); }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 constexpr int r4(int x) { return (x + 3) & ~3; }
+#ifndef FAST_EPB6
+#define FAST_EPB6 2
+#endif
 
 template <int N>
 struct FCfg {
     static constexpr int NPE = N * N * N;
     static constexpr int NPF = N * N;
     static constexpr int NL = 3 * N * N;                             // line tasks per element
-    static constexpr int EPB = N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? 2 : 1));
+    static constexpr int EPB = N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? FAST_EPB6 : 1));
     static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;          // threads per block
     static constexpr int NN = EPB * NPE;                             // nodes per block
     static constexpr int NFN = EPB * 6 * NPF;                        // face nodes per block
