@@ -296,12 +296,12 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     if (d->volume_op != FLOU_HIP_STRONG && d->volume_op != FLOU_HIP_SPLIT)
         return bad(FLOU_HIP_EINVAL, "unknown volume operator");
     if (d->tp_flux < 0 || d->tp_flux > 2) return bad(FLOU_HIP_EINVAL, "unknown two-point flux");
-    if (d->surface_avg < 0 || d->surface_avg > 1 || d->surface_diss < 0 || d->surface_diss > 2)
+    if (d->surface_avg < 0 || d->surface_avg > 1 || d->surface_diss < 0 || d->surface_diss > 3)
         return bad(FLOU_HIP_EINVAL, "unknown surface flux");
     if (d->equation == FLOU_HIP_ADVECTION) {
         if (d->volume_op == FLOU_HIP_SPLIT && d->tp_flux != FLOU_HIP_TP_STDAVG)
             return bad(FLOU_HIP_EINVAL, "advection has only the StdAverage two-point flux");
-        if (d->surface_avg != FLOU_HIP_AVG_STD || d->surface_diss == FLOU_HIP_DISS_MATRIX)
+        if (d->surface_avg != FLOU_HIP_AVG_STD || d->surface_diss >= FLOU_HIP_DISS_MATRIX)
             return bad(FLOU_HIP_EINVAL, "advection has only StdAverage and LxF surface fluxes");
     }
     if (d->nelements < 1 || d->nelements > (int64_t)1 << 30) return bad(FLOU_HIP_EINVAL, "bad nelements");
@@ -504,8 +504,8 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // adjoint kernels: StdAverage surface average with or without LxF dissipation
     if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
         h->vjp_why = "adjoint: only the StdAverage surface average is differentiated";
-    } else if (h->sdiss == FLOU_HIP_DISS_MATRIX) {
-        h->vjp_why = "adjoint: MatrixDissipation is not differentiated";
+    } else if (h->sdiss == FLOU_HIP_DISS_MATRIX || h->sdiss == FLOU_HIP_DISS_SCALAR) {
+        h->vjp_why = "adjoint: MatrixDissipation and ScalarDissipation are not differentiated";
     } else if (h->prec == FLOU_HIP_FP64) {
         h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp, h->curved)
                          : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp, h->curved)
@@ -524,7 +524,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // else runs the plain kernel in FP32.
     std::vector<float> fe, fds;
     if (!h->curved && h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
-        h->savg == FLOU_HIP_AVG_STD && h->sdiss != FLOU_HIP_DISS_MATRIX && n >= 4 &&
+        h->savg == FLOU_HIP_AVG_STD && (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
         h->ndofs < (int64_t(1) << 30)) {
         bool ok = true;
         double dmax = 0.0;

@@ -11,7 +11,7 @@ enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
 enum { VOL_STRONG = 0, VOL_SPLIT = 1 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
-enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2 };
+enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };
 enum { BC_DIRICHLET = 0, BC_INFLOW = 1, BC_OUTFLOW = 2, BC_SLIP = 3 };
 
 // What the fused element kernel does with L(u_in) once it has it.

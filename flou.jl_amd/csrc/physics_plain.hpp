@@ -284,6 +284,34 @@ __device__ inline void euler_numericalflux(const T* Ql, const T* Qr, int savg, i
         const T lam = (la < lb) ? lb : la;
 #pragma unroll
         for (int v = 0; v < NV; ++v) Fn[v] = Fn[v] + ((lam * (Ql[v] - Qr[v])) / T(2)) * intensity;
+    } else if (sdiss == DISS_SCALAR) {
+        // ScalarDissipation (Euler.jl:228-301); `typo` reproduces the 3-D left momenta
+        // taken from Qr (Euler.jl:257)
+        const T rl = Pl[0], rr = Pr[0];
+        T um[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) um[k] = (Pl[1 + k] + Pr[1 + k]) / T(2);
+        const T rho = (rl + rr) / T(2);
+        const T bl = rl / (T(2) * pl), br = rr / (T(2) * pr);
+        const T beta = logarithmic_mean(bl, br, err);
+        const T al = dsqrt(g * pl / rl, err);
+        const T ar = dsqrt(g * pr / rr, err);
+        const T la = fabs(ul) + al, lb = fabs(ur) + ar;
+        const T lam = (la < lb) ? lb : la;
+        T D[NV];
+        D[0] = rr - rl;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) D[1 + k] = Qr[1 + k] - ((ND == 3 && typo) ? Qr[1 + k] : Ql[1 + k]);
+        T s = (T(1) / beta) / (g - T(1)) + Pl[1] * Pr[1];
+#pragma unroll
+        for (int k = 1; k < ND; ++k) s = s + Pl[1 + k] * Pr[1 + k];
+        T t = um[0] * (Pr[1] - Pl[1]);
+#pragma unroll
+        for (int k = 1; k < ND; ++k) t = t + um[k] * (Pr[1 + k] - Pl[1 + k]);
+        t = t + (T(1) / br - T(1) / bl) / (T(2) * (g - T(1)));
+        D[ND + 1] = (s * (rr - rl)) / T(2) + rho * t;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) Fn[v] = Fn[v] - ((lam / T(2)) * D[v]) * intensity;
     } else if (sdiss == DISS_MATRIX) {
         const T rl = Pl[0], rr = Pr[0];
         T um[ND];

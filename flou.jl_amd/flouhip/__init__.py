@@ -10,7 +10,7 @@ from .disc import (CarpenterKennedy2N54, EquationConfig, MultielementDisc, ORK25
                    get_max_dt, integrate, rhs_, rk_stage, timeintegrate)
 from .equations import (ChandrasekharAverage, EulerEquation, EulerInflowBC, EulerOutflowBC,
                         EulerSlipBC, GenericBC, KennedyGruber, LinearAdvection, LxF,
-                        MatrixDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
+                        MatrixDissipation, ScalarDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
                         nvariables, pressure, soundvelocity, vars_cons2prim, vars_prim2cons)
 from .mesh import CartesianMesh, apply_periodicBCs
 from .stdregions import DGSEMrec, LagrangeBasis, StdHex, StdQuad, StdSegment
@@ -26,7 +26,7 @@ __all__ = [
     "CarpenterKennedy2N54", "CartesianMesh", "ChandrasekharAverage", "DGSEMrec", "DomainError",
     "EquationConfig", "EulerEquation", "EulerInflowBC", "EulerOutflowBC", "EulerSlipBC",
     "GenericBC", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
-    "MultielementDisc", "ORK256", "SSPRK33", "SplitDivOperator", "StdAverage", "StdHex",
+    "MultielementDisc", "ORK256", "SSPRK33", "ScalarDissipation", "SplitDivOperator", "StdAverage", "StdHex",
     "StdQuad", "StdSegment", "StrongDivOperator", "apply_periodicBCs", "get_max_dt",
     "integrate", "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
     "timeintegrate", "vars_cons2prim", "vars_prim2cons", "version",

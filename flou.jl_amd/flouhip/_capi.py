@@ -15,7 +15,7 @@ ADVECTION, EULER = 0, 1
 STRONG, SPLIT = 0, 1
 TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
 AVG_STD, AVG_CHANDRASEKHAR = 0, 1
-DISS_NONE, DISS_LXF, DISS_MATRIX = 0, 1, 2
+DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
 BC_DIRICHLET, BC_INFLOW, BC_OUTFLOW, BC_SLIP = 0, 1, 2, 3
 SSPRK33, CK2N54, ORK256 = 0, 1, 2
 GEOM_AFFINE, GEOM_CURVED = 0, 1

@@ -73,10 +73,19 @@ class MatrixDissipation:
         self.diss = C.DISS_MATRIX
 
 
+class ScalarDissipation:
+    """ScalarDissipation(avg, intensity) (Equations/Euler.jl:228-301): avg - λ/2 D intensity,
+    D the entropy-variable jump mapped to conservative variables with scalar λ."""
+
+    def __init__(self, avg, intensity=1.0):
+        self.avg, self.intensity = avg, float(intensity)
+        self.diss = C.DISS_SCALAR
+
+
 def _surface_codes(numflux):
     if isinstance(numflux, (StdAverage, ChandrasekharAverage)):
         return numflux.code, C.DISS_NONE, 1.0
-    if isinstance(numflux, (LxF, MatrixDissipation)):
+    if isinstance(numflux, (LxF, MatrixDissipation, ScalarDissipation)):
         if numflux.avg.code is None:
             raise ValueError("this average cannot be used as a surface flux")
         return numflux.avg.code, numflux.diss, numflux.intensity

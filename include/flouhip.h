@@ -54,9 +54,10 @@ enum { FLOU_HIP_TP_STDAVG = 0, FLOU_HIP_TP_CHANDRASEKHAR = 1, FLOU_HIP_TP_KENNED
 /* Surface numerical flux = average + dissipation:
  *   StdAverage()                      -> (AVG_STD, DISS_NONE)
  *   LxF(StdAverage(), c)              -> (AVG_STD, DISS_LXF), intensity c
- *   MatrixDissipation(Chandrasekhar(), c) -> (AVG_CHANDRASEKHAR, DISS_MATRIX), intensity c */
+ *   MatrixDissipation(Chandrasekhar(), c) -> (AVG_CHANDRASEKHAR, DISS_MATRIX), intensity c
+ *   ScalarDissipation(avg, c)             -> (avg, DISS_SCALAR), intensity c (Euler.jl:228-301) */
 enum { FLOU_HIP_AVG_STD = 0, FLOU_HIP_AVG_CHANDRASEKHAR = 1 };
-enum { FLOU_HIP_DISS_NONE = 0, FLOU_HIP_DISS_LXF = 1, FLOU_HIP_DISS_MATRIX = 2 };
+enum { FLOU_HIP_DISS_NONE = 0, FLOU_HIP_DISS_LXF = 1, FLOU_HIP_DISS_MATRIX = 2, FLOU_HIP_DISS_SCALAR = 3 };
 
 /* Boundary conditions (applyBC!, src/FlouSpatial/Interfaces.jl:25-49;
  * src/FlouSpatial/Equations/Euler.jl:69-94) */
@@ -144,9 +145,9 @@ const char* flou_hip_last_error(const flou_hip_handle* h);
 const char* flou_hip_create_error(void);
 
 /* Options.
- *  0: reproduce the 3-D ChandrasekharAverage surface-flux term of
- *     src/FlouSpatial/Equations/Euler.jl:216, which squares wl where wr is meant (default 0:
- *     the corrected flux).
+ *  0: reproduce two 3-D surface-flux typos of src/FlouSpatial/Equations/Euler.jl: the
+ *     ChandrasekharAverage term of line 216, which squares wl where wr is meant, and the
+ *     ScalarDissipation left momenta of line 257, taken from Qr (default 0: corrected).
  *  1: FP32 handles: 1 = always run the plain (reference-order) kernel, 0 = use the fast
  *     FP32 kernel where it applies (default).
  *  2: query, returns 1 if this FP32 handle has the fast kernel, else 0 (value ignored). */

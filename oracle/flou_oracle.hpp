@@ -90,7 +90,7 @@ enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
 enum { VOL_STRONG = 0, VOL_SPLIT = 1 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
-enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2 };
+enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };
 enum { BC_DIRICHLET = 0, BC_INFLOW = 1, BC_OUTFLOW = 2, BC_SLIP = 3 };
 enum { NODES_GLL = 0, NODES_GL = 1 };
 enum { RK_SSPRK33 = 0, RK_CK2N54 = 1, RK_ORK256 = 2 };
@@ -425,6 +425,30 @@ inline void euler_numericalflux(const Disc& d, const T* Ql, const T* Qr, T* Fn) 
         T lam = omax(oabs(ul) + al, oabs(ur) + ar);
         for (int v = 0; v < nv; ++v)
             Fn[v] = Fn[v] + ((lam * (Ql[v] - Qr[v])) / 2.0) * d.intensity;
+    } else if (d.sdiss == DISS_SCALAR) {
+        // ScalarDissipation (Euler.jl:228-301).  In 3-D the reference takes the left
+        // momenta from Qr (Euler.jl:257), which zeroes the momentum dissipation; reproduced
+        // when chandrasekhar_surface_typo (the "reference typos" switch) is set.
+        const T rl = Pl[0], rr = Pr[0];
+        const bool typo = nd == 3 && d.chandrasekhar_surface_typo;
+        T um[3];
+        for (int k = 0; k < nd; ++k) um[k] = (Pl[1 + k] + Pr[1 + k]) / 2.0;
+        const T rho = (rl + rr) / 2.0;
+        const T bl = rl / (2.0 * pl), br = rr / (2.0 * pr);
+        const T beta = logarithmic_mean(bl, br);
+        const T al = soundvelocity(rl, pl, g);
+        const T ar = soundvelocity(rr, pr, g);
+        const T lam = omax(oabs(ul) + al, oabs(ur) + ar);
+        T D[5];
+        D[0] = rr - rl;
+        for (int k = 0; k < nd; ++k) D[1 + k] = Qr[1 + k] - (typo ? Qr[1 + k] : Ql[1 + k]);
+        T s = (1.0 / beta) / (g - 1.0) + Pl[1] * Pr[1];
+        for (int k = 1; k < nd; ++k) s = s + Pl[1 + k] * Pr[1 + k];
+        T t = um[0] * (Pr[1] - Pl[1]);
+        for (int k = 1; k < nd; ++k) t = t + um[k] * (Pr[1 + k] - Pl[1 + k]);
+        t = t + (1.0 / br - 1.0 / bl) / (2.0 * (g - 1.0));
+        D[nd + 1] = (s * (rr - rl)) / 2.0 + rho * t;
+        for (int v = 0; v < nv; ++v) Fn[v] = Fn[v] - ((lam / 2.0) * D[v]) * d.intensity;
     } else if (d.sdiss == DISS_MATRIX) {
         // Euler.jl:308-380
         const T rl = Pl[0], rr = Pr[0];

@@ -15,7 +15,7 @@ EQ_ADVECTION, EQ_EULER = 0, 1
 VOL_STRONG, VOL_SPLIT = 0, 1
 TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
 AVG_STD, AVG_CHANDRASEKHAR = 0, 1
-DISS_NONE, DISS_LXF, DISS_MATRIX = 0, 1, 2
+DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
 BC_DIRICHLET, BC_INFLOW, BC_OUTFLOW, BC_SLIP = 0, 1, 2, 3
 NODES_GLL, NODES_GL = 0, 1
 RK_SSPRK33, RK_CK2N54, RK_ORK256 = 0, 1, 2

@@ -19,6 +19,8 @@ SURF = {
     "lxf": (lambda: F.LxF(F.StdAverage(), 1.0), O.AVG_STD, O.DISS_LXF),
     "avg": (lambda: F.StdAverage(), O.AVG_STD, O.DISS_NONE),
     "matrix": (lambda: F.MatrixDissipation(F.ChandrasekharAverage(), 1.0), O.AVG_CHANDRASEKHAR, O.DISS_MATRIX),
+    "scalar": (lambda: F.ScalarDissipation(F.ChandrasekharAverage(), 1.0), O.AVG_CHANDRASEKHAR, O.DISS_SCALAR),
+    "scalar-std": (lambda: F.ScalarDissipation(F.StdAverage(), 0.5), O.AVG_STD, O.DISS_SCALAR),
 }
 TP = {"std": (F.StdAverage, O.TP_STDAVG), "ch": (F.ChandrasekharAverage, O.TP_CHANDRASEKHAR),
       "kg": (F.KennedyGruber, O.TP_KENNEDY_GRUBER)}
@@ -48,8 +50,9 @@ def build(nd, n, nel, vol, tp="std", surf="lxf", periodic=None, nodes="GLL", eq=
     else:
         op = F.SplitDivOperator(TP[tp][0](), sf())
         volc, tpc = O.VOL_SPLIT, TP[tp][1]
+    inten = getattr(sf(), "intensity", 1.0)
     return mesh, std, equation, op, oracle_for(nd, n, nel, start, finish, periodic, eqc, eqp, volc,
-                                               tpc, savg, sdiss, 1.0, nodes=nodes)
+                                               tpc, savg, sdiss, inten, nodes=nodes)
 
 
 def rel_err(a, b):
@@ -74,6 +77,10 @@ RHS_CASES = [
     (3, 7, [2, 2, 1], "split", "kg", "lxf", "euler", "GLL"),
     (3, 8, [1, 2, 2], "split", "std", "avg", "euler", "GLL"),
     (3, 4, [2, 2, 2], "split", "ch", "matrix", "euler", "GLL"),
+    (1, 5, [6], "split", "ch", "scalar", "euler", "GLL"),
+    (2, 4, [3, 3], "split", "ch", "scalar", "euler", "GLL"),
+    (3, 4, [2, 2, 2], "split", "ch", "scalar", "euler", "GLL"),
+    (3, 5, [2, 2, 2], "strong", "std", "scalar-std", "euler", "GLL"),
     (2, 5, [4, 4], "strong", "std", "lxf", "adv", "GLL"),
     (2, 5, [4, 4], "split", "std", "lxf", "adv", "GLL"),
     (2, 5, [4, 3], "strong", "std", "lxf", "adv", "GL"),
@@ -96,7 +103,7 @@ def test_rhs_fp64_matches_oracle(case):
     mesh, std, equation, op, orc = build(nd, n, nel, vol, tp, surf, nodes=nodes, eq=eq)
     disc = F.MultielementDisc(mesh, std, equation, op, precision="fp64")
     Q = _state(disc, orc, nd, eq, seed=17, amp=0.3)
-    for typo in ((0, 1) if surf == "matrix" and nd == 3 else (0,)):
+    for typo in ((0, 1) if surf in ("matrix", "scalar", "scalar-std") and nd == 3 else (0,)):
         # 3-D ChandrasekharAverage surface flux with and without the Euler.jl:216 typo
         orc.set_option(0, typo)
         disc.set_option(0, typo)
@@ -110,7 +117,7 @@ def test_rhs_fp64_matches_oracle(case):
         print(f"{case} typo={typo}: bit-identical fraction {same:.4f}, max rel err {rel_err(got, ref):.2e}")
         assert rel_err(got, ref) <= 1e-13
         # only the log branch of logarithmic_mean (libm vs device log, 1 ulp) may differ
-        assert same >= (0.9 if (tp == "ch" or surf == "matrix") else 1.0)
+        assert same >= (0.9 if (tp == "ch" or surf in ("matrix", "scalar", "scalar-std")) else 1.0)
 
 
 def test_rhs_strong_form_is_bit_identical():

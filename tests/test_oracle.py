@@ -391,3 +391,88 @@ def test_negative_pressure_is_a_domain_error():
     Q[0, 3] = -1.0  # ρe < 0 at a face node => p < 0 => sqrt of a negative in LxF
     with pytest.raises(ArithmeticError):
         d.rhs(Q)
+
+
+# ---------------------------------------------------------------- ScalarDissipation
+def _scalar_disc(nd, typo):
+    d = O.OracleDisc(nd, 3, [2] * nd, [0.0] * nd, [1.0] * nd, [1] * nd, O.build_ops(3, "GLL"),
+                     O.EQ_EULER, [GAMMA], O.VOL_SPLIT, O.TP_CHANDRASEKHAR, O.AVG_CHANDRASEKHAR,
+                     O.DISS_SCALAR, 1.0)
+    d.set_option(0, typo)
+    avg = O.OracleDisc(nd, 3, [2] * nd, [0.0] * nd, [1.0] * nd, [1] * nd, O.build_ops(3, "GLL"),
+                       O.EQ_EULER, [GAMMA], O.VOL_SPLIT, O.TP_CHANDRASEKHAR, O.AVG_CHANDRASEKHAR,
+                       O.DISS_NONE, 1.0)
+    avg.set_option(0, typo)
+    return d, avg
+
+
+def _frame(nd):
+    fr = np.zeros((3, nd))
+    fr[0, 0] = 1.0
+    if nd >= 2:
+        fr[1, 1] = 1.0
+    if nd == 3:
+        fr[2, 2] = 1.0
+    return fr
+
+
+def _lam(Ql, Qr, nd):
+    def side(Q):
+        rho = Q[0]
+        u = Q[1:nd + 1] / rho
+        p = (GAMMA - 1) * (Q[nd + 1] - 0.5 * rho * np.dot(u, u))
+        return abs(u[0]) + np.sqrt(GAMMA * p / rho)
+    return max(side(Ql), side(Qr))
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_scalar_dissipation_is_the_linearised_jump(nd):
+    """ScalarDissipation (Euler.jl:228-301): Fn = avg - λ/2·D; for Qr = Ql + εδ the
+    dissipation vector D/ε tends to δ (D is ∂Q/∂W·[W], energy row included), and vanishes
+    for equal states (consistency)."""
+    d, avg = _scalar_disc(nd, 0)
+    rng = np.random.default_rng(nd)
+    fr = _frame(nd).ravel()
+    for _ in range(10):
+        X = rng.uniform(0, 1, (1, nd))
+        Ql = random_euler_state(X, nd, seed=int(rng.integers(1e6)), amp=0.3, smooth=False)[0]
+        assert np.allclose(d.numflux(Ql, Ql, fr), avg.numflux(Ql, Ql, fr), rtol=0, atol=1e-14)
+        delta = rng.standard_normal(nd + 2)
+        eps = 1e-6
+        Qr = Ql + eps * delta
+        D = (avg.numflux(Ql, Qr, fr) - d.numflux(Ql, Qr, fr)) / (_lam(Ql, Qr, nd) / 2)
+        assert np.allclose(D / eps, delta, rtol=0, atol=1e-4 * np.max(np.abs(delta)))
+
+
+@pytest.mark.parametrize("nd", [1, 2, 3])
+def test_scalar_dissipation_dissipates_entropy(nd):
+    """(W_r - W_l)·D ≥ 0 for strongly different random states: the dissipation term never
+    produces entropy."""
+    d, avg = _scalar_disc(nd, 0)
+    rng = np.random.default_rng(10 + nd)
+    fr = _frame(nd).ravel()
+    X = rng.uniform(0, 1, (200, nd))
+    QL = random_euler_state(X, nd, seed=1, amp=0.5, smooth=False)
+    QR = random_euler_state(X, nd, seed=2, amp=0.5, smooth=False)
+    for Ql, Qr in zip(QL, QR):
+        D = avg.numflux(Ql, Qr, fr) - d.numflux(Ql, Qr, fr)
+        dW = O.cons2entropy(nd, GAMMA, Qr) - O.cons2entropy(nd, GAMMA, Ql)
+        assert np.dot(dW, D) >= -1e-14 * np.sum(np.abs(dW * D))
+
+
+def test_scalar_dissipation_3d_typo_drops_momentum_jumps():
+    """Euler.jl:257 takes the 3-D left momenta from Qr: with the typo switch on, the
+    momentum rows of D vanish; the density and energy rows are unchanged."""
+    good, avg = _scalar_disc(3, 0)
+    bad, avg_bad = _scalar_disc(3, 1)  # same typo switch for the average part
+    rng = np.random.default_rng(4)
+    fr = _frame(3).ravel()
+    X = rng.uniform(0, 1, (5, 3))
+    QL = random_euler_state(X, 3, seed=3, amp=0.3, smooth=False)
+    QR = random_euler_state(X, 3, seed=4, amp=0.3, smooth=False)
+    for Ql, Qr in zip(QL, QR):
+        Dg = avg.numflux(Ql, Qr, fr) - good.numflux(Ql, Qr, fr)
+        Db = avg_bad.numflux(Ql, Qr, fr) - bad.numflux(Ql, Qr, fr)
+        assert np.allclose(Db[1:4], 0.0, atol=1e-15)
+        assert np.allclose(Db[[0, 4]], Dg[[0, 4]], rtol=1e-12, atol=1e-14)
+        assert np.max(np.abs(Dg[1:4])) > 1e-3
