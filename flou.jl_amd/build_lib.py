@@ -26,7 +26,7 @@ EXACT = ["-ffp-contract=off"]
 
 
 def units():
-    u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT),
+    u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT), ("limiter.o", "limiter.hip", EXACT),
          ("rhs_fast3d.o", "rhs_fast3d.hip", ["-ffp-contract=fast"])]
     for prec in ("F64", "F32"):
         for nd in (1, 2, 3):

@@ -28,6 +28,9 @@ struct flou_hip_handle {
     int egeom_stride = 0;
     int curved = 0;   // per-node geometry (GEOM_CURVED)
     int int_npe = 1;  // node stride of the integrate weights (1: per-node Jω)
+    int lim_kind = FLOU_HIP_LIMITER_NONE;  // stage limiter applied by flou_hip_rk_stage
+    double lim_min = 0.0;
+    double* d_evol = nullptr;  // element volumes (limiter means)
     // device tables
     void* d_ngeom = nullptr;
     void* d_ops = nullptr;
@@ -102,7 +105,7 @@ void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
                     h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
-                    h->d_fegeom, h->d_fdsharp, h->d_ngeom};
+                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -586,6 +589,8 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     rc |= upload_real(h, &h->d_fgeom, fg);
     rc |= upload_real(h, &h->d_ngeom, ng);
     rc |= upload(h, (void**)&h->d_dxe, dxe);
+    if (h->eq == FLOU_HIP_EULER)
+        rc |= upload(h, (void**)&h->d_evol, std::vector<double>(d->elem_volume, d->elem_volume + ne));
     rc |= upload(h, (void**)&h->d_jac, jac);
     rc |= upload(h, (void**)&h->d_w, w);
     h->bctype.assign(std::max(h->nbd, 1), FLOU_HIP_BC_OUTFLOW);
@@ -702,6 +707,63 @@ int flou_hip_rhs(flou_hip_handle* h, void* dQ, const void* Q, double t) {
     return launch_stage(h, MODE_RHS, 0, 0.0, 0.0, 0.0, Q, nullptr, nullptr, dQ);
 }
 
+namespace {
+// Zhang & Shu limiter in place on Q (Euler.jl:619-653), one thread per element.
+int apply_limiter(flou_hip_handle* h, void* Q, double minval) {
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64)
+        e = launch_zhang_shu<double>((double*)Q, h->ndofs, h->ne, h->npe, h->nd, h->int_npe, h->d_jac, h->d_w,
+                                     h->d_evol, minval, h->gamma, h->stream);
+    else
+        e = launch_zhang_shu<float>((float*)Q, h->ndofs, h->ne, h->npe, h->nd, h->int_npe, h->d_jac, h->d_w,
+                                    h->d_evol, minval, h->gamma, h->stream);
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, std::string("limiter launch: ") + hipGetErrorString(e));
+    return FLOU_HIP_OK;
+}
+}  // namespace
+
+int flou_hip_set_limiter(flou_hip_handle* h, int32_t kind, double minval) {
+    if (!h) return FLOU_HIP_EINVAL;
+    if (kind == FLOU_HIP_LIMITER_NONE) {
+        h->lim_kind = kind;
+        return FLOU_HIP_OK;
+    }
+    if (kind != FLOU_HIP_LIMITER_ZHANG_SHU) return fail(h, FLOU_HIP_EINVAL, "unknown limiter");
+    if (h->eq != FLOU_HIP_EULER) return fail(h, FLOU_HIP_EUNSUPPORTED, "the Zhang & Shu limiter needs the Euler equations");
+    if (!std::isfinite(minval))
+        return fail(h, FLOU_HIP_EINVAL, "The minimum value must be specified when using the limiter of Zhang & Shu.");
+    h->lim_kind = kind;
+    h->lim_min = minval;
+    return FLOU_HIP_OK;
+}
+
+int flou_hip_limit(flou_hip_handle* h, void* Q, double minval) {
+    if (!h || !Q) return FLOU_HIP_EINVAL;
+    if (h->eq != FLOU_HIP_EULER) return fail(h, FLOU_HIP_EUNSUPPORTED, "the Zhang & Shu limiter needs the Euler equations");
+    if (!std::isfinite(minval))
+        return fail(h, FLOU_HIP_EINVAL, "The minimum value must be specified when using the limiter of Zhang & Shu.");
+    return apply_limiter(h, Q, minval);
+}
+
+int flou_hip_monitor(flou_hip_handle* h, const void* Q, int32_t kind, double* out) {
+    if (!h || !Q || !out) return FLOU_HIP_EINVAL;
+    if (h->eq != FLOU_HIP_EULER) return fail(h, FLOU_HIP_EUNSUPPORTED, "the monitors need the Euler equations");
+    if (kind != FLOU_HIP_MONITOR_KINETIC_ENERGY && kind != FLOU_HIP_MONITOR_ENTROPY)
+        return fail(h, FLOU_HIP_EINVAL, "Unknown monitor");
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64)
+        e = launch_monitor<double>((const double*)Q, h->ndofs, h->int_npe, h->nd, h->gamma, kind, h->d_jac,
+                                   h->d_w, h->d_partial, reduce_max_blocks(), h->d_out, h->d_err, h->stream);
+    else
+        e = launch_monitor<float>((const float*)Q, h->ndofs, h->int_npe, h->nd, h->gamma, kind, h->d_jac,
+                                  h->d_w, h->d_partial, reduce_max_blocks(), h->d_out, h->d_err, h->stream);
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, hipGetErrorString(e));
+    HIPCHECK(h, hipMemcpyAsync(h->h_out, h->d_out, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHECK(h, hipStreamSynchronize(h->stream));
+    *out = h->h_out[0];
+    return flou_hip_check(h);
+}
+
 int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double dt, double t,
                       const void* u_in, const void* u_base, void* reg, void* u_out) {
     (void)t;
@@ -713,14 +775,18 @@ int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double 
         if (stage > 0 && !u_base) return fail(h, FLOU_HIP_EINVAL, "u_base missing");
         if (stage == 1 && overlaps(u_base, u_out, bytes)) return fail(h, FLOU_HIP_EINVAL, "u_out overlaps u_base");
         const int mode = stage == 0 ? MODE_SSP1 : (stage == 1 ? MODE_SSP2 : MODE_SSP3);
-        return launch_stage(h, mode, 0, dt, 0.0, 0.0, u_in, u_base, nullptr, u_out);
+        int rc = launch_stage(h, mode, 0, dt, 0.0, 0.0, u_in, u_base, nullptr, u_out);
+        if (!rc && h->lim_kind != FLOU_HIP_LIMITER_NONE) rc = apply_limiter(h, u_out, h->lim_min);
+        return rc;
     }
     double A, B;
     if (!tableau(scheme, stage, A, B)) return fail(h, FLOU_HIP_EINVAL, "unknown scheme or stage");
     if (!reg) return fail(h, FLOU_HIP_EINVAL, "2N schemes need the register buffer");
     if (overlaps(reg, u_in, bytes) || overlaps(reg, u_out, bytes))
         return fail(h, FLOU_HIP_EINVAL, "register overlaps a state buffer");
-    return launch_stage(h, MODE_2N, stage == 0, dt, A, B, u_in, nullptr, reg, u_out);
+    int rc = launch_stage(h, MODE_2N, stage == 0, dt, A, B, u_in, nullptr, reg, u_out);
+    if (!rc && h->lim_kind != FLOU_HIP_LIMITER_NONE) rc = apply_limiter(h, u_out, h->lim_min);
+    return rc;
 }
 
 int flou_hip_max_dt(flou_hip_handle* h, const void* Q, double cfl, double* out) {
@@ -822,6 +888,8 @@ int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void*
 int flou_hip_rk_step_record(flou_hip_handle* h, int32_t scheme, double dt, double t,
                             void* const* stages, void* u_next) {
     if (!h || !stages || !u_next) return FLOU_HIP_EINVAL;
+    if (h->lim_kind != FLOU_HIP_LIMITER_NONE)
+        return fail(h, FLOU_HIP_EUNSUPPORTED, "adjoint: the stage limiter is not differentiated");
     const int S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
     if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
     for (int s = 0; s < S; ++s)
@@ -844,6 +912,8 @@ int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t
                          const void* const* stages, void* ubar) {
     (void)t;
     if (!h || !stages || !ubar) return FLOU_HIP_EINVAL;
+    if (h->lim_kind != FLOU_HIP_LIMITER_NONE)
+        return fail(h, FLOU_HIP_EUNSUPPORTED, "adjoint: the stage limiter is not differentiated");
     if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
     const int S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
     for (int s = 0; s < S; ++s)

@@ -6,8 +6,10 @@ this package fails with ImportError when the library is missing: there is no CPU
 """
 from . import _capi
 from ._capi import DomainError
-from .disc import (CarpenterKennedy2N54, EquationConfig, MultielementDisc, ORK256, SSPRK33,
-                   get_max_dt, integrate, rhs_, rk_stage, timeintegrate)
+from .disc import (CarpenterKennedy2N54, EquationConfig, MonitorOutput, MultielementDisc, ORK256,
+                   SSPRK33, ZhangShuLimiter, get_limiter, get_limiter_callback, get_max_dt,
+                   get_monitor, get_monitor_callback, integrate, list_limiters, list_monitors,
+                   make_callback_list, rhs_, rk_stage, timeintegrate)
 from .equations import (ChandrasekharAverage, EulerEquation, EulerInflowBC, EulerOutflowBC,
                         EulerSlipBC, GenericBC, KennedyGruber, LinearAdvection, LxF,
                         MatrixDissipation, ScalarDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
@@ -26,8 +28,9 @@ __all__ = [
     "CarpenterKennedy2N54", "CartesianMesh", "ChandrasekharAverage", "DGSEMrec", "DomainError",
     "EquationConfig", "EulerEquation", "EulerInflowBC", "EulerOutflowBC", "EulerSlipBC",
     "GenericBC", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
-    "MultielementDisc", "ORK256", "SSPRK33", "ScalarDissipation", "SplitDivOperator", "StdAverage", "StdHex",
-    "StdQuad", "StdSegment", "StrongDivOperator", "apply_periodicBCs", "get_max_dt",
-    "integrate", "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
+    "MonitorOutput", "MultielementDisc", "ORK256", "SSPRK33", "ScalarDissipation", "SplitDivOperator", "StdAverage", "StdHex",
+    "StdQuad", "StdSegment", "StrongDivOperator", "ZhangShuLimiter", "apply_periodicBCs",
+    "get_limiter", "get_limiter_callback", "get_max_dt", "get_monitor", "get_monitor_callback",
+    "integrate", "list_limiters", "list_monitors", "make_callback_list", "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
     "timeintegrate", "vars_cons2prim", "vars_prim2cons", "version",
 ]

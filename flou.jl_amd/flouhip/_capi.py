@@ -16,6 +16,8 @@ STRONG, SPLIT = 0, 1
 TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
 AVG_STD, AVG_CHANDRASEKHAR = 0, 1
 DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
+LIMITER_NONE, LIMITER_ZHANG_SHU = 0, 1
+MONITOR_KINETIC_ENERGY, MONITOR_ENTROPY = 0, 1
 BC_DIRICHLET, BC_INFLOW, BC_OUTFLOW, BC_SLIP = 0, 1, 2, 3
 SSPRK33, CK2N54, ORK256 = 0, 1, 2
 GEOM_AFFINE, GEOM_CURVED = 0, 1
@@ -74,6 +76,9 @@ SYMBOLS = {
     "flou_hip_max_dt": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _d]),
     "flou_hip_integrate": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _d]),
     "flou_hip_check": (ctypes.c_int, [_vp]),
+    "flou_hip_set_limiter": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
+    "flou_hip_limit": (ctypes.c_int, [_vp, _vp, ctypes.c_double]),
+    "flou_hip_monitor": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _d]),
     "flou_hip_rhs_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "flou_hip_rk_step_record": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                ctypes.POINTER(_vp), _vp]),

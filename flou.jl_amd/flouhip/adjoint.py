@@ -19,7 +19,14 @@ import ctypes
 import torch
 
 from . import _capi as C
-from .disc import _solver_code
+from .disc import _solver
+
+
+def _solver_code(solver):
+    solver = _solver(solver)
+    if getattr(solver, "stage_limiter", None) is not None:
+        raise NotImplementedError("adjoint: the stage limiter is not differentiated")
+    return solver.code
 
 
 def _nstages(code):

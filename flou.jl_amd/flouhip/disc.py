@@ -5,6 +5,9 @@
     timeintegrate(Q0, disc, eq, solver, tf; dt, cfl)                   -> flou_hip_timeintegrate
     get_max_dt(Q, disc, eq, cfl)                                       -> flou_hip_max_dt
     integrate(Q, disc, ivar)                                           -> flou_hip_integrate
+    get_limiter(disc, eq, "zhang_shu", minval) / solver stage_limiter  -> flou_hip_limit,
+                                                                          flou_hip_set_limiter
+    get_monitor(disc, eq, name) / get_monitor_callback                 -> flou_hip_monitor
 
 State buffers are torch tensors on the GPU holding Flou's (ndofs, nvars) column-major
 matrix, i.e. a contiguous (nvars, ndofs) tensor (or its transpose view).  torch is only the
@@ -212,15 +215,20 @@ def rk_stage(disc, scheme, stage, dt, u_in, u_out, u_base=None, reg=None, t=0.0)
 
 
 class SSPRK33:
+    """OrdinaryDiffEq's SSPRK33; `stage_limiter` is what get_limiter_callback returns."""
     code = C.SSPRK33
     nstages = 3
+
+    def __init__(self, stage_limiter=None):
+        self.stage_limiter = stage_limiter
 
 
 class CarpenterKennedy2N54:
     code = C.CK2N54
     nstages = 5
 
-    def __init__(self, williamson_condition=False):
+    def __init__(self, stage_limiter=None, williamson_condition=False):
+        self.stage_limiter = stage_limiter
         self.williamson_condition = williamson_condition
 
 
@@ -228,26 +236,158 @@ class ORK256:
     code = C.ORK256
     nstages = 5
 
-    def __init__(self, williamson_condition=False):
+    def __init__(self, stage_limiter=None, williamson_condition=False):
+        self.stage_limiter = stage_limiter
         self.williamson_condition = williamson_condition
 
 
+def _solver(solver):
+    return solver() if isinstance(solver, type) else solver
+
+
 def _solver_code(solver):
-    if isinstance(solver, type):
-        solver = solver()
-    return solver.code
+    return _solver(solver).code
 
 
-def timeintegrate(Q0, disc, equation, solver, tfinal, dt, cfl=None, t0=0.0):
+# -- limiters and monitors (Equations/Euler.jl:541-654, FlouTime.jl:107-148) ------------
+
+def list_limiters(disc, equation):
+    return ("zhang_shu",) if isinstance(equation, EulerEquation) else ()
+
+
+class ZhangShuLimiter:
+    """get_limiter(disc, eq, :zhang_shu, minval) (Euler.jl:612-654): called as
+    limiter(Q, disc, equation), it limits the device state Q in place."""
+
+    def __init__(self, minval):
+        if minval is None:
+            raise ValueError("The minimum value must be specified when using the limiter of Zhang & Shu.")
+        self.minval = float(minval)
+
+    def __call__(self, Q, disc, equation=None):
+        disc._sync_stream()
+        C.check(C.lib.flou_hip_limit(disc.h, disc._dptr(Q), self.minval), disc.h, "zhang_shu limiter")
+
+
+def get_limiter(disc, equation, name, p=None):
+    if name != "zhang_shu":
+        raise ValueError(f"Unknown limiter '{name}'.")
+    if not isinstance(equation, EulerEquation):
+        raise ValueError("the Zhang & Shu limiter is defined for the Euler equations")
+    return ZhangShuLimiter(p)
+
+
+def get_limiter_callback(disc, equation, name, p=None):
+    """get_limiter_callback (FlouTime.jl:136-148): the stage limiter handed to a solver,
+    e.g. ORK256(stage_limiter=get_limiter_callback(disc, eq, "zhang_shu", 1e-10)); the
+    library then applies it after every stage."""
+    return get_limiter(disc, equation, name, p)
+
+
+_MONITORS = {"kinetic_energy": C.MONITOR_KINETIC_ENERGY, "energy": C.MONITOR_KINETIC_ENERGY,
+             "entropy": C.MONITOR_ENTROPY}
+
+
+def list_monitors(disc, equation):
+    return ("kinetic_energy", "entropy") if isinstance(equation, EulerEquation) else ()
+
+
+class _Monitor:
+    def __init__(self, kind):
+        self.kind = kind
+
+    def __call__(self, Q, disc, equation=None):
+        disc._sync_stream()
+        out = ctypes.c_double()
+        C.check(C.lib.flou_hip_monitor(disc.h, disc._dptr(Q), self.kind, ctypes.byref(out)), disc.h, "monitor")
+        return out.value
+
+
+def get_monitor(disc, equation, name, p=None):
+    """get_monitor(disc, eq, name) (Euler.jl:545-590): a function (Q, disc, eq) -> Σ ∫ s(Q),
+    s the kinetic energy ("kinetic_energy", or the reference's own key "energy") or the
+    mathematical entropy ("entropy"), accumulated in FP64."""
+    if not isinstance(equation, EulerEquation) or name not in _MONITORS:
+        raise ValueError(f"Unknown monitor '{name}'.")
+    return _Monitor(_MONITORS[name])
+
+
+class MonitorOutput:
+    """MonitorOutput (FlouTime.jl:107-111): time, iteration and value after each selected step."""
+
+    def __init__(self):
+        self.time, self.iter, self.value = [], [], []
+
+
+class MonitorCallback:
+    def __init__(self, func, output, iter=True):
+        self.func, self.output, self.iter = func, output, iter
+
+    def selected(self, it):
+        return self.iter is True or it in self.iter
+
+    def __call__(self, Q, disc, equation, t, it):
+        if self.selected(it):
+            self.output.time.append(t)
+            self.output.iter.append(it)
+            self.output.value.append(self.func(Q, disc, equation))
+
+
+def get_monitor_callback(timetype, valuetype, disc, equation, name, p=None, iter=True):
+    """get_monitor_callback (FlouTime.jl:113-134) -> (callback, MonitorOutput); pass the
+    callback to timeintegrate(..., callback=...)."""
+    out = MonitorOutput()
+    return MonitorCallback(get_monitor(disc, equation, name, p), out, iter), out
+
+
+def make_callback_list(*callbacks):
+    if not callbacks:
+        return None
+    return callbacks[0] if len(callbacks) == 1 else list(callbacks)
+
+
+def timeintegrate(Q0, disc, equation, solver, tfinal, dt, cfl=None, t0=0.0, callback=None):
     """timeintegrate(Q0, disc, equation, solver, tf; dt, adaptive=false) -- FlouTime.jl:34-56.
 
     Integrates in place on the device tensor Q0 with fixed steps of `dt` (the last step is
     shortened to land on tfinal, as OrdinaryDiffEq does).  With `cfl`, every step uses
-    min(get_max_dt(Q, cfl), dt) (get_cfl_callback(cfl, dt), FlouTime.jl:92-104).
-    Returns (Q0, elapsed seconds).  Raises DomainError on a non-physical state
-    (the reference logs "Simulation crashed!")."""
-    code = _solver_code(solver)
+    min(get_max_dt(Q, cfl), dt) (get_cfl_callback(cfl, dt), FlouTime.jl:92-104).  A
+    solver's stage_limiter is applied after every stage; monitor callbacks run after every
+    step they select.  Returns (Q0, elapsed seconds).  Raises DomainError on a
+    non-physical state (the reference logs "Simulation crashed!")."""
+    solver = _solver(solver)
+    code = solver.code
+    limiter = getattr(solver, "stage_limiter", None)
+    cbs = [] if callback is None else (list(callback) if isinstance(callback, (list, tuple)) else [callback])
     disc._sync_stream()
+    if limiter is not None:
+        C.check(C.lib.flou_hip_set_limiter(disc.h, C.LIMITER_ZHANG_SHU, limiter.minval), disc.h, "stage limiter")
+    try:
+        if cbs:
+            return _timeintegrate_steps(Q0, disc, equation, code, tfinal, dt, cfl, t0, cbs)
+        return _timeintegrate(Q0, disc, code, tfinal, dt, cfl, t0)
+    finally:
+        if limiter is not None:
+            C.lib.flou_hip_set_limiter(disc.h, C.LIMITER_NONE, 0.0)
+
+
+def _timeintegrate_steps(Q0, disc, equation, code, tfinal, dt, cfl, t0, cbs):
+    ptr = disc._dptr(Q0)
+    t = ctypes.c_double(float(t0))
+    tic = time.perf_counter()
+    it = 0
+    while t.value < float(tfinal) * (1 - 1e-14):
+        step = min(float(dt), float(tfinal) - t.value)
+        C.check(C.lib.flou_hip_timeintegrate(disc.h, code, ptr, step, 1, float(cfl or 0.0),
+                                             ctypes.byref(t)), disc.h, "timeintegrate")
+        it += 1
+        for cb in cbs:
+            cb(Q0, disc, equation, t.value, it)
+    torch.cuda.synchronize()
+    return Q0, time.perf_counter() - tic
+
+
+def _timeintegrate(Q0, disc, code, tfinal, dt, cfl, t0):
     ptr = disc._dptr(Q0)
     t = ctypes.c_double(float(t0))
     tic = time.perf_counter()

@@ -202,6 +202,27 @@ int flou_hip_max_dt(flou_hip_handle* h, const void* Q, double cfl, double* out);
  * 0-based. */
 int flou_hip_integrate(flou_hip_handle* h, const void* Q, int32_t ivar, double* out);
 
+/* Stage limiter and monitors of the Euler equations (src/FlouSpatial/Equations/Euler.jl). */
+enum { FLOU_HIP_LIMITER_NONE = 0, FLOU_HIP_LIMITER_ZHANG_SHU = 1 };
+enum { FLOU_HIP_MONITOR_KINETIC_ENERGY = 0, FLOU_HIP_MONITOR_ENTROPY = 1 };
+
+/* get_limiter(disc, eq, :zhang_shu, minval) handed to the RK method as its stage limiter
+ * (get_limiter_callback, FlouTime.jl:136-148; zhang_shu_limiter, Euler.jl:612-654): once
+ * set, flou_hip_rk_stage (and so flou_hip_timeintegrate) applies the limiter to u_out
+ * after every stage.  FLOU_HIP_LIMITER_NONE switches it off.  minval must be finite (the
+ * reference raises ArgumentError without one); EUNSUPPORTED for linear advection.  The
+ * adjoint entry points refuse a handle with a limiter set. */
+int flou_hip_set_limiter(flou_hip_handle* h, int32_t kind, double minval);
+
+/* The Zhang & Shu limiter applied once, in place on Q (Euler.jl:619-653). */
+int flou_hip_limit(flou_hip_handle* h, void* Q, double minval);
+
+/* get_monitor(disc, eq, name) evaluated on Q (Euler.jl:545-590): Σ_e ∫ s(Q) with s the
+ * kinetic energy |ρu|²/2ρ (reference name :energy) or the mathematical entropy
+ * -ρ(log p - γ log ρ)/(γ-1); FP64 accumulation (FP64 result).  EDOMAIN (entropy) when
+ * p or ρ is negative somewhere (log of a negative number). */
+int flou_hip_monitor(flou_hip_handle* h, const void* Q, int32_t kind, double* out);
+
 /* Synchronises, returns FLOU_HIP_EDOMAIN if a kernel met a non-physical state since the
  * last call (sqrt or log of a negative number in the reference), and clears the flag. */
 int flou_hip_check(flou_hip_handle* h);

@@ -569,6 +569,27 @@ void oracle_set_curved(oracle_handle* h, const double* node_metric, const double
     d.e_vol.assign(elem_volume, elem_volume + d.ne);
 }
 
+void oracle_set_limiter(oracle_handle* h, int on, double minval) {
+    h->d.limiter_on = on != 0;
+    h->d.limiter_minval = minval;
+}
+
+// Zhang-Shu limiter applied once to Q (ndofs x nv, column-major), in place.
+int oracle_limit(oracle_handle* h, double* Q, double minval) {
+    zhang_shu(h->d, Q, minval);
+    return 0;
+}
+
+int oracle_monitor(oracle_handle* h, const double* Q, int kind, double* out) {
+    try {
+        *out = monitor(h->d, Q, kind);
+    } catch (std::domain_error& e) {
+        h->err = e.what();
+        return -5;
+    }
+    return 0;
+}
+
 void oracle_set_option(oracle_handle* h, int which, int value) {
     if (which == 0) h->d.chandrasekhar_surface_typo = value;
 }

@@ -148,6 +148,9 @@ struct Disc {
         return curved ? &n_metric[((size_t)e * npe + i) * nd * nd] : &e_metric[(size_t)e * nd * nd];
     }
     double jac(long e, int i) const { return curved ? n_jac[(size_t)e * npe + i] : e_jac[e]; }
+    // stage limiter (Zhang & Shu) applied by rk_run after every stage
+    bool limiter_on = false;
+    double limiter_minval = 0.0;
     std::vector<double> coords;  // ndofs * nd node coordinates
     std::vector<double> f_coords;// nf * npf * nd face node coordinates
     // equation and operators
@@ -765,6 +768,16 @@ double max_dt(const Disc& d, const T* Q, double cfl) {
     return dt;
 }
 
+// tensor-product quadrature weight of node i of the standard element
+inline double node_weight(const Disc& d, int i) {
+    double wi[3] = {1.0, 1.0, 1.0};
+    int idx = i;
+    for (int k = 0; k < d.nd; ++k) { wi[k] = d.ops.w[idx % d.n]; idx /= d.n; }
+    double w = wi[0];
+    for (int k = 1; k < d.nd; ++k) w = w * wi[k];
+    return w;
+}
+
 // integrate (MultielementDiscontinuous.jl:148-160; PhysicalRegions.jl:366-368)
 template <class T>
 double integrate(const Disc& d, const T* Q, int ivar) {
@@ -797,6 +810,87 @@ struct Tableau2N {
 };
 Tableau2N tableau_2n(int scheme);
 
+// Monitors (Equations/Euler.jl:541-590): Σ_e Jω_e' · s(Q_e) with s the kinetic energy
+// |ρu|²/(2ρ) (FlouCommon/Euler.jl:162-175) or the mathematical entropy -ρ s/(γ-1),
+// s = log p - γ log ρ (FlouCommon/Euler.jl:202-217).
+enum { MON_KINETIC_ENERGY = 0, MON_ENTROPY = 1 };
+inline double monitor(const Disc& d, const double* Q, int kind) {
+    const int nd = d.nd, npe = d.npe;
+    const long N = d.ndofs();
+    double total = 0.0;
+    for (long e = 0; e < d.ne; ++e) {
+        double s = 0.0;
+        for (int i = 0; i < npe; ++i) {
+            double q[5];
+            for (int v = 0; v < d.nv; ++v) q[v] = Q[(size_t)v * N + e * npe + i];
+            double m2 = q[1] * q[1];
+            for (int k = 1; k < nd; ++k) m2 = m2 + q[1 + k] * q[1 + k];
+            double val;
+            if (kind == MON_KINETIC_ENERGY) {
+                val = m2 / (2.0 * q[0]);
+            } else {
+                const double p = pressure(q, nd, d.gamma);
+                const double ent = olog(p) - d.gamma * olog(q[0]);
+                val = -(q[0] * ent) / (d.gamma - 1.0);
+            }
+            s = s + (d.jac(e, i) * node_weight(d, i)) * val;
+        }
+        total = total + s;
+    }
+    return total;
+}
+
+// Zhang & Shu positivity limiter as Flou states it (Equations/Euler.jl:596-654), per
+// element: means Q̄ = Jω'Q / vol; density pulled towards ρ̄ by θ = |(ρ̄-m)/(ρ̄-ρmin)|,
+// m = min(minval, ρ̄), when θ <= 1; then the pressure of the updated nodes: p̄ = Jω'p / vol,
+// θ = |(p̄-m)/(p̄-pmin)|, m = min(minval, p̄), all variables pulled towards Q̄ when θ <= 1.
+inline void zhang_shu(const Disc& d, double* Q, double minval) {
+    const int nd = d.nd, npe = d.npe, nv = d.nv;
+    const long N = d.ndofs();
+    for (long e = 0; e < d.ne; ++e) {
+        double qm[5] = {0, 0, 0, 0, 0};
+        for (int i = 0; i < npe; ++i) {
+            const double jw = d.jac(e, i) * node_weight(d, i);
+            for (int v = 0; v < nv; ++v) qm[v] = qm[v] + jw * Q[(size_t)v * N + e * npe + i];
+        }
+        for (int v = 0; v < nv; ++v) qm[v] = qm[v] / d.e_vol[e];
+        double* rho = &Q[e * npe];
+        double rmin = rho[0];
+        for (int i = 1; i < npe; ++i) rmin = std::min(rmin, rho[i]);
+        double m = std::min(minval, qm[0]);
+        double th = std::fabs((qm[0] - m) / (qm[0] - rmin));
+        if (th <= 1.0)
+            for (int i = 0; i < npe; ++i) rho[i] = th * (rho[i] - qm[0]) + qm[0];
+        std::vector<double> p(npe);
+        double pm = 0.0;
+        for (int i = 0; i < npe; ++i) {
+            double q[5];
+            for (int v = 0; v < nv; ++v) q[v] = Q[(size_t)v * N + e * npe + i];
+            p[i] = pressure(q, nd, d.gamma);
+            pm = pm + (d.jac(e, i) * node_weight(d, i)) * p[i];
+        }
+        pm = pm / d.e_vol[e];
+        double pmin = p[0];
+        for (int i = 1; i < npe; ++i) pmin = std::min(pmin, p[i]);
+        m = std::min(minval, pm);
+        th = std::fabs((pm - m) / (pm - pmin));
+        if (th <= 1.0)
+            for (int i = 0; i < npe; ++i)
+                for (int v = 0; v < nv; ++v) {
+                    double& x = Q[(size_t)v * N + e * npe + i];
+                    x = th * (x - qm[v]) + qm[v];
+                }
+    }
+}
+
+// Stage limiter hook of rk_run (the OrdinaryDiffEq stage_limiter! after every stage); only
+// for plain doubles.
+inline void stage_limit(const Disc& d, double* u) {
+    if (d.limiter_on) zhang_shu(d, u, d.limiter_minval);
+}
+template <int K>
+inline void stage_limit(const Disc&, Dual<K>*) {}
+
 template <class T>
 void rk_run(const Disc& d, int scheme, T* u, double dt, int nsteps, Work<T>& wk) {
     const long M = d.ndofs() * d.nv;
@@ -806,10 +900,13 @@ void rk_run(const Disc& d, int scheme, T* u, double dt, int nsteps, Work<T>& wk)
             for (long i = 0; i < M; ++i) u0[i] = u[i];
             rhs(d, u0.data(), k.data(), wk);
             for (long i = 0; i < M; ++i) u1[i] = u0[i] + dt * k[i];
+            stage_limit(d, u1.data());
             rhs(d, u1.data(), k.data(), wk);
             for (long i = 0; i < M; ++i) u1[i] = ((3.0 * u0[i] + u1[i]) + dt * k[i]) / 4.0;
+            stage_limit(d, u1.data());
             rhs(d, u1.data(), k.data(), wk);
             for (long i = 0; i < M; ++i) u[i] = ((u0[i] + 2.0 * u1[i]) + (2.0 * dt) * k[i]) / 3.0;
+            stage_limit(d, u);
         } else {
             Tableau2N tb = tableau_2n(scheme);
             for (int s = 0; s < tb.s; ++s) {
@@ -820,6 +917,7 @@ void rk_run(const Disc& d, int scheme, T* u, double dt, int nsteps, Work<T>& wk)
                     for (long i = 0; i < M; ++i) tmp[i] = tb.A[s] * tmp[i] + dt * k[i];
                 }
                 for (long i = 0; i < M; ++i) u[i] = u[i] + tb.B[s] * tmp[i];
+                stage_limit(d, u);
             }
         }
     }

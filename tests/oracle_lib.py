@@ -19,6 +19,7 @@ DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
 BC_DIRICHLET, BC_INFLOW, BC_OUTFLOW, BC_SLIP = 0, 1, 2, 3
 NODES_GLL, NODES_GL = 0, 1
 RK_SSPRK33, RK_CK2N54, RK_ORK256 = 0, 1, 2
+MON_KINETIC_ENERGY, MON_ENTROPY = 0, 1
 
 _lib = None
 
@@ -40,6 +41,9 @@ def lib():
         L.oracle_destroy.argtypes = [ctypes.c_void_p]
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.oracle_set_curved.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
+        L.oracle_set_limiter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.oracle_limit.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
+        L.oracle_monitor.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.oracle_sizes.argtypes = [ctypes.c_void_p, _lp]
         L.oracle_connectivity.argtypes = [ctypes.c_void_p, _lp, _ip, _lp, _ip, _ip]
         L.oracle_geometry.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
@@ -124,6 +128,21 @@ class OracleDisc:
 
     def set_option(self, which, value):
         lib().oracle_set_option(self.h, which, value)
+
+    def set_limiter(self, on, minval=0.0):
+        """Zhang & Shu stage limiter applied by rk_run after every stage."""
+        lib().oracle_set_limiter(self.h, int(bool(on)), float(minval))
+
+    def limit(self, Q, minval):
+        u = np.array(Q, dtype=np.float64, order="F", copy=True)
+        self._check(lib().oracle_limit(self.h, _p(u), float(minval)))
+        return u
+
+    def monitor(self, Q, kind):
+        Q = np.asfortranarray(Q, dtype=np.float64)
+        out = np.zeros(1)
+        self._check(lib().oracle_monitor(self.h, _p(Q), int(kind), _p(out)))
+        return float(out[0])
 
     def connectivity(self):
         nd = self.nd
