@@ -27,6 +27,7 @@ struct flou_hip_handle {
     int gather = 0;
     int egeom_stride = 0;
     int curved = 0;   // per-node geometry (GEOM_CURVED)
+    double blend = 1.0, sjac[3] = {1.0, 1.0, 1.0};  // HybridDivOperator
     int int_npe = 1;  // node stride of the integrate weights (1: per-node Jω)
     int lim_kind = FLOU_HIP_LIMITER_NONE;  // stage limiter applied by flou_hip_rk_stage
     double lim_min = 0.0;
@@ -166,6 +167,8 @@ RhsArgs<T> make_args(flou_hip_handle* h) {
     a.sdiss = h->sdiss;
     a.typo = h->typo;
     a.errflag = h->d_err;
+    a.blend = (T)h->blend;
+    for (int k = 0; k < 3; ++k) a.sjac[k] = (T)h->sjac[k];
     return a;
 }
 
@@ -296,8 +299,16 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     } else {
         return bad(FLOU_HIP_EINVAL, "unknown equation");
     }
-    if (d->volume_op != FLOU_HIP_STRONG && d->volume_op != FLOU_HIP_SPLIT)
+    if (d->volume_op != FLOU_HIP_STRONG && d->volume_op != FLOU_HIP_SPLIT && d->volume_op != FLOU_HIP_HYBRID)
         return bad(FLOU_HIP_EINVAL, "unknown volume operator");
+    if (d->volume_op == FLOU_HIP_HYBRID) {
+        if (d->equation != FLOU_HIP_EULER)
+            return bad(FLOU_HIP_EUNSUPPORTED, "HybridDivOperator: only the Euler equations are supported");
+        if (d->geometry != FLOU_HIP_GEOM_AFFINE)
+            return bad(FLOU_HIP_EUNSUPPORTED, "HybridDivOperator: subcell geometry exists for Cartesian elements only");
+        if (!std::isfinite(d->blend) || d->blend < 0.0)
+            return bad(FLOU_HIP_EINVAL, "HybridDivOperator: the blending constant must be finite and >= 0");
+    }
     if (d->tp_flux < 0 || d->tp_flux > 2) return bad(FLOU_HIP_EINVAL, "unknown two-point flux");
     if (d->surface_avg < 0 || d->surface_avg > 1 || d->surface_diss < 0 || d->surface_diss > 3)
         return bad(FLOU_HIP_EINVAL, "unknown surface flux");
@@ -331,6 +342,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     h->eq = d->equation; h->vol = d->volume_op; h->tp = d->tp_flux;
     h->savg = d->surface_avg; h->sdiss = d->surface_diss;
     h->gamma = d->gamma; h->intensity = d->intensity;
+    h->blend = d->volume_op == FLOU_HIP_HYBRID ? d->blend : 1.0;
     for (int k = 0; k < 3; ++k) h->a[k] = d->equation == FLOU_HIP_ADVECTION ? d->velocity[k] : 0.0;
     {
         double a2 = h->a[0] * h->a[0];
@@ -386,6 +398,9 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         if (d->l_minus[j] != (j == 0 ? 1.0 : 0.0)) h->gather = 0;
         if (d->l_plus[j] != (j == n - 1 ? 1.0 : 0.0)) h->gather = 0;
     }
+    if (h->vol == FLOU_HIP_HYBRID && !h->gather)
+        return bad(FLOU_HIP_EUNSUPPORTED,
+                   "HybridDivOperator: only nodes that include the element boundaries (GLL) are supported");
 
     // geometry
     const int NG = nd * nd + 1;
@@ -421,6 +436,10 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             for (int c = 0; c < nd * nd; ++c)
                 if (d->elem_metric[e * nd * nd + c] != d->elem_metric[c]) uniform = false;
         }
+        if (h->vol == FLOU_HIP_HYBRID && !uniform)
+            return bad(FLOU_HIP_EUNSUPPORTED, "HybridDivOperator: the elements must share one geometry (CartesianMesh)");
+        if (h->vol == FLOU_HIP_HYBRID)  // subcell Jacobians = diagonal metric (PhysicalRegions.jl:75-145)
+            for (int k = 0; k < nd; ++k) h->sjac[k] = d->elem_metric[k * nd + k];
         if (uniform) {
             eg.resize(NG);
             for (int c = 0; c < nd * nd; ++c) eg[c] = d->elem_metric[c];
@@ -505,7 +524,9 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
     // adjoint kernels: StdAverage surface average with or without LxF dissipation
-    if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
+    if (h->vol == FLOU_HIP_HYBRID) {
+        h->vjp_why = "adjoint: the HybridDivOperator is not differentiated";
+    } else if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
         h->vjp_why = "adjoint: only the StdAverage surface average is differentiated";
     } else if (h->sdiss == FLOU_HIP_DISS_MATRIX || h->sdiss == FLOU_HIP_DISS_SCALAR) {
         h->vjp_why = "adjoint: MatrixDissipation and ScalarDissipation are not differentiated";

@@ -8,7 +8,7 @@
 namespace flouhip {
 
 enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
-enum { VOL_STRONG = 0, VOL_SPLIT = 1 };
+enum { VOL_STRONG = 0, VOL_SPLIT = 1, VOL_HYBRID = 2 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
 enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };
@@ -73,6 +73,8 @@ struct RhsArgs {
     T a0, a1, a2;      // advection velocity
     int savg, sdiss, typo;
     int* errflag;
+    T blend;           // HybridDivOperator: blending constant c
+    T sjac[3];         // HybridDivOperator: subcell-interface surface Jacobian per direction
 };
 
 // Block index -> logical block, so that consecutive logical blocks (neighbouring

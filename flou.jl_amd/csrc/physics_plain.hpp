@@ -392,5 +392,34 @@ __device__ inline void euler_numericalflux(const T* Ql, const T* Qr, int savg, i
     }
 }
 
+
+// HybridDivOperator pieces (OpDivergence.jl:441-801).
+// Subcell-interface frame of direction dir on Cartesian elements (PhysicalSubgrid for
+// CartesianMesh, PhysicalRegions.jl:75-145): n along the axis, (t, b) in the reference's
+// order, then the surface Jacobian.
+template <typename T, int ND>
+__device__ inline void subgrid_frame(int dir, T sjac, T* g) {
+#pragma unroll
+    for (int i = 0; i < 3 * ND; ++i) g[i] = T(0);
+    if constexpr (ND == 1) {
+        g[0] = T(1);
+    } else if constexpr (ND == 2) {
+        if (dir == 0) { g[0] = T(1); g[2 + 1] = T(1); }
+        else { g[1] = T(1); g[2 + 0] = T(-1); }
+    } else {
+        const int t = (dir + 1) % 3, b = (dir + 2) % 3;
+        g[dir] = T(1); g[3 + t] = T(1); g[6 + b] = T(1);
+    }
+    g[3 * ND] = sjac;
+}
+
+// Blending factor (OpDivergence.jl:781-801): δ = max((√(b²+c) - b)/√(b²+c), 1/2).
+template <typename T>
+__device__ inline T hybrid_delta(T b, T c, int* err) {
+    T s = dsqrt(b * b + c, err);
+    s = (s - b) / s;
+    return (s < T(0.5)) ? T(0.5) : s;
+}
+
 }  // namespace plain
 }  // namespace flouhip

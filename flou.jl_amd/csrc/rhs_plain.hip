@@ -38,6 +38,9 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     __shared__ T sP[EPB][NP][NPE];
     __shared__ T sFt[EPB][NFT][NPE];
     __shared__ T sFn[EPB][NF][NV][NPF];
+    // HybridDivOperator: subcell-interface fluxes of one direction, [le][v][line][s]
+    constexpr int HB = VOL == VOL_HYBRID ? 1 : 0;
+    __shared__ T sFb[HB ? EPB : 1][HB ? NV : 1][HB ? NPF : 1][HB ? N : 1];
 
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -179,12 +182,85 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         }
     }
     __syncthreads();
-    if (!valid) return;
 
-    // Volume terms, one thread per node, in the reference's summation order
     T acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = T(0);
+    if constexpr (VOL == VOL_HYBRID) {
+        // Telescopic subcell form (OpDivergence.jl:500-620), direction by direction:
+        // F̄[s] on the interface between line positions s-1 and s (F̄[0] = F̄[N] = 0), then
+        // dQ[i] += (F̄[i] - F̄[i+1]) / ω[i].
+        const T* D = &sOps[ops_D(N)];
+        const T* w = &sOps[ops_w(N)];
+        for (int dir = 0; dir < ND; ++dir) {
+            T gs[3 * ND + 1];
+            plain::subgrid_frame<T, ND>(dir, a.sjac[dir], gs);
+            for (int it = tid; it < EPB * NPF * (N - 1); it += TPB) {
+                const int le2 = it / (NPF * (N - 1)), r = it % (NPF * (N - 1));
+                const int f = r / (N - 1), sp = 1 + r % (N - 1);
+                if (e0 + le2 >= a.ne) continue;
+                const T* M = &sG[le2][0] + ND * dir;
+                T Fb[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fb[v] = T(0);
+                for (int ik = sp; ik < N; ++ik) {
+                    const int nk = line_node<ND, N>(dir, f, ik);
+                    T qk[NV], Pk[NV];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) { qk[v] = sQ[le2][v][nk]; Pk[v] = sP[le2][v][nk]; }
+                    const T bk = sP[le2][NV][nk];
+                    for (int il = 0; il < sp; ++il) {
+                        const int nl = line_node<ND, N>(dir, f, il);
+                        T ql[NV], Pl[NV], Ft[NV];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) { ql[v] = sQ[le2][v][nl]; Pl[v] = sP[le2][v][nl]; }
+                        const T bl = sP[le2][NV][nl];
+                        plain::twopointflux<T, ND, EQ, TP>(ql, qk, Pl, Pk, bl, bk, M, M, a.gamma, avel,
+                                                           a.errflag, Ft);
+                        const T c = (T(2) * w[il]) * D[il * N + ik];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) Fb[v] = Fb[v] + c * Ft[v];
+                    }
+                }
+                // FV flux between positions sp-1 and sp on the subcell-interface frame
+                const int nl = line_node<ND, N>(dir, f, sp - 1), nr = line_node<ND, N>(dir, f, sp);
+                T Ql[NV], Qr[NV], Rl[NV], Rr[NV], Fr[NV], Fv[NV], Wl[NV], Wr[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) { Ql[v] = sQ[le2][v][nl]; Qr[v] = sQ[le2][v][nr]; }
+                plain::rotate2face<T, ND>(Ql, gs, Rl);
+                plain::rotate2face<T, ND>(Qr, gs, Rr);
+                plain::euler_numericalflux<T, ND>(Rl, Rr, a.savg, a.sdiss, a.typo, a.intensity, a.gamma,
+                                                  a.errflag, Fr);
+                plain::rotate2phys<T, ND>(Fr, gs, Fv);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Fv[v] = Fv[v] * gs[3 * ND];
+                // blending on the entropy production of the high-order interface flux
+                plain::cons2entropy<T, ND>(Ql, a.gamma, a.errflag, Wl);
+                plain::cons2entropy<T, ND>(Qr, a.gamma, a.errflag, Wr);
+                T b = (Wr[0] - Wl[0]) * (Fb[0] - Fv[0]);
+#pragma unroll
+                for (int v = 1; v < NV; ++v) b = b + (Wr[v] - Wl[v]) * (Fb[v] - Fv[v]);
+                const T dl = plain::hybrid_delta(b, a.blend, a.errflag);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sFb[le2][v][f][sp] = (T(1) - dl) * Fv[v] + dl * Fb[v];
+            }
+            __syncthreads();
+            if (valid) {
+                int f, ii;
+                node_line<ND, N>(node, dir, f, ii);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const T l = ii == 0 ? T(0) : sFb[le][v][f][ii];
+                    const T r = ii == N - 1 ? T(0) : sFb[le][v][f][ii + 1];
+                    acc[v] = acc[v] + (l - r) / w[ii];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    if (!valid) return;
+
+    // Volume terms, one thread per node, in the reference's summation order
     const int64_t gnode = (int64_t)e * NPE + node;
     const T* M = &sG[le][0];
     T J = sG[le][ND * ND];
@@ -196,7 +272,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         M = Mnode;
     }
 #pragma unroll
-    for (int dir = 0; dir < ND; ++dir) {
+    for (int dir = 0; dir < (VOL == VOL_HYBRID ? 0 : ND); ++dir) {
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         if constexpr (VOL == VOL_STRONG) {
@@ -309,10 +385,21 @@ static hipError_t launch_plain(const RhsArgs<T>& a, hipStream_t s) {
 
 template <typename T, int ND, int N, bool CURV>
 static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
-    if (eq == EQ_ADVECTION)
+    if (eq == EQ_ADVECTION) {
+        if (vol == VOL_HYBRID) return nullptr;
         return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
                                  : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
+    }
     if (vol == VOL_STRONG) return &launch_plain<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV>;
+    if (vol == VOL_HYBRID) {
+        if constexpr (CURV) return nullptr;  // subcell geometry is defined for Cartesian elements
+        switch (tp) {
+            case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID, TP_STDAVG, false>;
+            case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID, TP_CHANDRASEKHAR, false>;
+            case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID, TP_KENNEDY_GRUBER, false>;
+            default: return nullptr;
+        }
+    }
     switch (tp) {
         case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG, CURV>;
         case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR, CURV>;

@@ -11,9 +11,10 @@ from .disc import (CarpenterKennedy2N54, EquationConfig, MonitorOutput, Multiele
                    get_monitor, get_monitor_callback, integrate, list_limiters, list_monitors,
                    make_callback_list, rhs_, rk_stage, timeintegrate)
 from .equations import (ChandrasekharAverage, EulerEquation, EulerInflowBC, EulerOutflowBC,
-                        EulerSlipBC, GenericBC, KennedyGruber, LinearAdvection, LxF,
+                        EulerSlipBC, GenericBC, HybridDivOperator, KennedyGruber, LinearAdvection, LxF,
                         MatrixDissipation, ScalarDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
-                        nvariables, pressure, soundvelocity, vars_cons2prim, vars_prim2cons)
+                        normal_shockwave, nvariables, pressure, soundvelocity, vars_cons2prim,
+                        vars_prim2cons)
 from .mesh import CartesianMesh, apply_periodicBCs
 from .stdregions import DGSEMrec, LagrangeBasis, StdHex, StdQuad, StdSegment
 
@@ -27,10 +28,11 @@ def version():
 __all__ = [
     "CarpenterKennedy2N54", "CartesianMesh", "ChandrasekharAverage", "DGSEMrec", "DomainError",
     "EquationConfig", "EulerEquation", "EulerInflowBC", "EulerOutflowBC", "EulerSlipBC",
-    "GenericBC", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
+    "GenericBC", "HybridDivOperator", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
     "MonitorOutput", "MultielementDisc", "ORK256", "SSPRK33", "ScalarDissipation", "SplitDivOperator", "StdAverage", "StdHex",
     "StdQuad", "StdSegment", "StrongDivOperator", "ZhangShuLimiter", "apply_periodicBCs",
     "get_limiter", "get_limiter_callback", "get_max_dt", "get_monitor", "get_monitor_callback",
-    "integrate", "list_limiters", "list_monitors", "make_callback_list", "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
+    "integrate", "list_limiters", "list_monitors", "make_callback_list", "normal_shockwave",
+    "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
     "timeintegrate", "vars_cons2prim", "vars_prim2cons", "version",
 ]

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "libflouhip.so")
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, EDOMAIN = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1
 ADVECTION, EULER = 0, 1
-STRONG, SPLIT = 0, 1
+STRONG, SPLIT, HYBRID = 0, 1, 2
 TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
 AVG_STD, AVG_CHANDRASEKHAR = 0, 1
 DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
@@ -49,6 +49,7 @@ class Desc(ctypes.Structure):
         ("geometry", ctypes.c_int32),
         ("elem_jac", _d), ("elem_metric", _d), ("elem_volume", _d),
         ("face_frames", _d), ("face_jac", _d),
+        ("blend", ctypes.c_double),
     ]
 
 

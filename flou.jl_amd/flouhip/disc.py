@@ -49,9 +49,9 @@ class MultielementDisc:
             raise ValueError("mesh, standard element and equation dimensions differ")
         if len(bcs) != mesh.nboundaries():
             raise ValueError("The number of BCs does not match the number of boundaries.")
-        if operator.code == C.SPLIT and not std.basis.hasboundaries:
+        if operator.code in (C.SPLIT, C.HYBRID) and not std.basis.hasboundaries:
             raise NotImplementedError(
-                "the split form on Gauss nodes (subgrid entropy projection) is not on the GPU path")
+                "the split and hybrid forms on Gauss nodes (subgrid entropy projection) are not on the GPU path")
         self.mesh, self.std, self.equation, self.operator = mesh, std, equation, operator
         self.precision = precision
         self.dtype = {"fp64": np.float64, "fp32": np.float32}[precision]
@@ -75,8 +75,9 @@ class MultielementDisc:
         else:
             raise ValueError("unsupported equation")
         d.volume_op = operator.code
-        d.tp_flux = operator.tpflux.tp_code if operator.code == C.SPLIT else C.TP_STDAVG
+        d.tp_flux = operator.tpflux.tp_code if operator.code in (C.SPLIT, C.HYBRID) else C.TP_STDAVG
         d.surface_avg, d.surface_diss, d.intensity = _surface_codes(operator.numflux)
+        d.blend = getattr(operator, "blend", 1.0)
 
         keep = []  # arrays referenced by the descriptor must outlive create()
 

@@ -111,6 +111,32 @@ class SplitDivOperator:
         self.code = C.SPLIT
 
 
+class HybridDivOperator:
+    """HybridDivOperator([tpflux=numflux.avg, [fvflux=numflux]], numflux, blend)
+    (Equations/OpDivergence.jl:441-470): the split form in telescopic subcell form, each
+    subcell interface flux blended with a finite-volume flux by the entropy production of
+    the high-order one (δ = max((√(b²+c) - b)/√(b²+c), 1/2), c = blend).  The GPU path
+    takes fvflux = numflux, as the two- and three-argument constructors do."""
+
+    def __init__(self, *args):
+        if len(args) == 2:
+            numflux, blend = args
+            tpflux, fvflux = numflux.avg, numflux
+        elif len(args) == 3:
+            tpflux, numflux, blend = args
+            fvflux = numflux
+        elif len(args) == 4:
+            tpflux, fvflux, numflux, blend = args
+            if fvflux is not numflux and _surface_codes(fvflux) != _surface_codes(numflux):
+                raise NotImplementedError("HybridDivOperator: a finite-volume flux other than the "
+                                          "surface flux is not on the GPU path")
+        else:
+            raise TypeError("HybridDivOperator([tpflux, [fvflux]], numflux, blend)")
+        self.tpflux, self.fvflux, self.numflux = tpflux, fvflux, numflux
+        self.blend = float(blend)
+        self.code = C.HYBRID
+
+
 # ---------------------------------------------------------------- boundary conditions
 class GenericBC:
     """GenericBC(Qext) with Qext(Qin, x, frame, time, eq) -> state.  The GPU path evaluates
@@ -178,3 +204,13 @@ def vars_cons2prim(Q, eq):
 
 def soundvelocity(rho, p, eq):
     return np.sqrt(eq.γ * p / rho)
+
+
+def normal_shockwave(rho0, u0, p0, eq):
+    """(ρ1, u1, p1) downstream of a normal shock (FlouCommon/Euler.jl:346-359)."""
+    g = eq.γ
+    M0 = u0 / soundvelocity(rho0, p0, eq)
+    rho1 = rho0 * M0 ** 2 * (g + 1) / ((g - 1) * M0 ** 2 + 2)
+    p1 = p0 * (2 * g * M0 ** 2 - (g - 1)) / (g + 1)
+    M1 = np.sqrt(((g - 1) * M0 ** 2 + 2) / (2 * g * M0 ** 2 - (g - 1)))
+    return np.array([rho1, M1 * soundvelocity(rho1, p1, eq), p1])

@@ -45,8 +45,12 @@ enum { FLOU_HIP_FP64 = 0, FLOU_HIP_FP32 = 1 };
 /* Equations: LinearAdvection{ND} and EulerEquation{ND} */
 enum { FLOU_HIP_ADVECTION = 0, FLOU_HIP_EULER = 1 };
 
-/* Volume operators: StrongDivOperator, SplitDivOperator */
-enum { FLOU_HIP_STRONG = 0, FLOU_HIP_SPLIT = 1 };
+/* Volume operators: StrongDivOperator, SplitDivOperator, HybridDivOperator (the
+ * telescopic subcell form blending two-point and finite-volume fluxes,
+ * src/FlouSpatial/Equations/OpDivergence.jl:441-620, 781-801; nodes with boundaries (GLL),
+ * Cartesian elements, Euler; the FV flux is the surface flux, as in the two- and three-
+ * argument constructors) */
+enum { FLOU_HIP_STRONG = 0, FLOU_HIP_SPLIT = 1, FLOU_HIP_HYBRID = 2 };
 
 /* Two-point volume fluxes: StdAverage, ChandrasekharAverage, Kennedy-Gruber */
 enum { FLOU_HIP_TP_STDAVG = 0, FLOU_HIP_TP_CHANDRASEKHAR = 1, FLOU_HIP_TP_KENNEDY_GRUBER = 2 };
@@ -95,7 +99,7 @@ typedef struct flou_hip_desc {
     double gamma;       /* EulerEquation.γ */
     double velocity[3]; /* LinearAdvection.a */
 
-    int32_t volume_op;  /* FLOU_HIP_STRONG | FLOU_HIP_SPLIT */
+    int32_t volume_op;  /* FLOU_HIP_STRONG | FLOU_HIP_SPLIT | FLOU_HIP_HYBRID */
     int32_t tp_flux;    /* two-point flux of SplitDivOperator */
     int32_t surface_avg;
     int32_t surface_diss;
@@ -128,6 +132,8 @@ typedef struct flou_hip_desc {
     const double* elem_volume; /* nelements */
     const double* face_frames; /* AFFINE: nfaces*3*nd; CURVED: nfacedofs*3*nd; (n, t, b) */
     const double* face_jac;    /* AFFINE: nfaces; CURVED: nfacedofs */
+
+    double blend;              /* HybridDivOperator.blend (c >= 0); ignored otherwise */
 } flou_hip_desc;
 
 /* Library version string. */

@@ -569,6 +569,9 @@ void oracle_set_curved(oracle_handle* h, const double* node_metric, const double
     d.e_vol.assign(elem_volume, elem_volume + d.ne);
 }
 
+// HybridDivOperator blending constant (the operator itself is volop = VOL_HYBRID).
+void oracle_set_blend(oracle_handle* h, double blend) { h->d.blend = blend; }
+
 void oracle_set_limiter(oracle_handle* h, int on, double minval) {
     h->d.limiter_on = on != 0;
     h->d.limiter_minval = minval;

@@ -87,7 +87,7 @@ template <int K> inline Dual<K> omin(const Dual<K>& a, const Dual<K>& b) { retur
 // ------------------------------------------------------------------------------------
 // Enumerations (shared numbering with the public header's enums)
 enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
-enum { VOL_STRONG = 0, VOL_SPLIT = 1 };
+enum { VOL_STRONG = 0, VOL_SPLIT = 1, VOL_HYBRID = 2 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
 enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };
@@ -159,6 +159,7 @@ struct Disc {
     double a[3] = {0, 0, 0};
     int volop = VOL_STRONG, tpflux = TP_STDAVG, savg = AVG_STD, sdiss = DISS_LXF;
     double intensity = 1.0;
+    double blend = 1.0;                  // HybridDivOperator blending constant c
     int chandrasekhar_surface_typo = 1;  // reproduce Equations/Euler.jl:216 (3-D only)
     // boundary conditions per remaining boundary index
     std::vector<int> bctype;
@@ -519,6 +520,74 @@ inline void euler_numericalflux(const Disc& d, const T* Ql, const T* Qr, T* Fn) 
     }
 }
 
+// Subcell-interface frame and surface Jacobian of a Cartesian element, direction dir
+// (PhysicalSubgrid for CartesianMesh, PhysicalRegions.jl:75-145): n along the axis,
+// (t, b) as the reference orders them; jac = product of the other half-widths.
+inline void subgrid_frame(const Disc& d, int dir, double* fr, double& jac) {
+    const int nd = d.nd;
+    for (int i = 0; i < 3 * nd; ++i) fr[i] = 0.0;
+    if (nd == 1) {
+        fr[0] = 1.0;
+        jac = 1.0;
+    } else if (nd == 2) {
+        if (dir == 0) { fr[0] = 1.0; fr[2 + 1] = 1.0; jac = d.dx[1] / 2.0; }
+        else { fr[1] = 1.0; fr[2 + 0] = -1.0; jac = d.dx[0] / 2.0; }
+    } else {
+        const int t = (dir + 1) % 3, b = (dir + 2) % 3;
+        fr[dir] = 1.0; fr[3 + t] = 1.0; fr[6 + b] = 1.0;
+        jac = dir == 0 ? d.dx[1] * d.dx[2] / 4.0 : (dir == 1 ? d.dx[0] * d.dx[2] / 4.0 : d.dx[0] * d.dx[1] / 4.0);
+    }
+}
+
+// Blending factor of the hybrid operator (OpDivergence.jl:781-801, the active "Fisher"
+// form limited from below by 1/2): δ = max((√(b²+c) - b)/√(b²+c), 1/2).
+template <class T>
+inline T hybrid_delta(const T& b, double c) {
+    T s = osqrt(b * b + c);
+    s = (s - b) / s;
+    return omax(s, T(0.5));
+}
+
+// Subcell interface fluxes F̄[1..n-1] of one line (OpDivergence.jl:576-618; index s is the
+// interface between line positions s-1 and s; F̄[0] = F̄[n] = 0 stay outside).
+template <class T>
+void hybrid_line_fluxes(const Disc& d, const T* Qe, const T* We, long e, int dir, int f, T* Fb) {
+    const int nd = d.nd, n = d.n, nv = d.nv;
+    const Ops& o = d.ops;
+    double fr[9], sjac;
+    subgrid_frame(d, dir, fr, sjac);
+    for (int s = 1; s < n; ++s) {
+        T* F = Fb + (size_t)s * nv;
+        for (int v = 0; v < nv; ++v) F[v] = T(0.0);
+        for (int ik = s; ik < n; ++ik) {
+            const int nk = line_node(nd, n, dir, f, ik);
+            for (int il = 0; il < s; ++il) {
+                const int nl = line_node(nd, n, dir, f, il);
+                T Ft[5];
+                twopointflux(d, &Qe[nl * nv], &Qe[nk * nv], d.metric(e, nl) + nd * dir,
+                             d.metric(e, nk) + nd * dir, Ft);
+                const double c = (2.0 * o.w[il]) * o.D[il * n + ik];
+                for (int v = 0; v < nv; ++v) F[v] = F[v] + c * Ft[v];
+            }
+        }
+        // FV flux between positions s-1 and s on the subcell interface frame
+        const int nl = line_node(nd, n, dir, f, s - 1), nr = line_node(nd, n, dir, f, s);
+        T Rl[5], Rr[5], Fr[5], Fv[5];
+        rotate2face(nd, &Qe[nl * nv], fr, Rl);
+        rotate2face(nd, &Qe[nr * nv], fr, Rr);
+        euler_numericalflux(d, Rl, Rr, Fr);
+        rotate2phys(nd, Fr, fr, Fv);
+        for (int v = 0; v < nv; ++v) Fv[v] = Fv[v] * sjac;
+        // blending on the entropy production of the high-order flux
+        const T* Wl = &We[nl * nv];
+        const T* Wr = &We[nr * nv];
+        T b = (Wr[0] - Wl[0]) * (F[0] - Fv[0]);
+        for (int v = 1; v < nv; ++v) b = b + (Wr[v] - Wl[v]) * (F[v] - Fv[v]);
+        const T dl = hybrid_delta(b, d.blend);
+        for (int v = 0; v < nv; ++v) F[v] = (1.0 - dl) * Fv[v] + dl * F[v];
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // rhs! (Equations/Hyperbolic.jl:31-69).  Q, dQ: (ndofs, nv) column-major.
 template <class T>
@@ -602,10 +671,28 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
         for (int i = 0; i < npe; ++i)
             for (int dir = 0; dir < nd; ++dir)
                 contravariant_flux(d, &Qe[i * nv], d.metric(e, i), dir, &Ft[((size_t)i * nd + dir) * nv]);
+        std::vector<T> We, Fb;
+        if (d.volop == VOL_HYBRID) {
+            // entropy variables of the element, then the telescopic subcell form
+            // (OpDivergence.jl:500-620): dQ[i] += (F̄[i] - F̄[i+1]) / ω[i]
+            We.resize((size_t)npe * nv);
+            Fb.assign((size_t)(n + 1) * nv, T(0.0));
+            for (int i = 0; i < npe; ++i) cons2entropy(&Qe[i * nv], nd, d.gamma, &We[i * nv]);
+        }
         const int nlines = npf;
         for (int dir = 0; dir < nd; ++dir) {
             for (int f = 0; f < nlines; ++f) {
-                if (d.volop == VOL_STRONG) {
+                if (d.volop == VOL_HYBRID) {
+                    hybrid_line_fluxes(d, Qe.data(), We.data(), e, dir, f, Fb.data());
+                    for (int i = 0; i < n; ++i) {
+                        const int ni = line_node(nd, n, dir, f, i);
+                        for (int v = 0; v < nv; ++v) {
+                            const T l = i == 0 ? T(0.0) : Fb[(size_t)i * nv + v];
+                            const T r = i == n - 1 ? T(0.0) : Fb[(size_t)(i + 1) * nv + v];
+                            acc[ni * nv + v] = acc[ni * nv + v] + (l - r) / o.w[i];
+                        }
+                    }
+                } else if (d.volop == VOL_STRONG) {
                     // mul!(dQ[line], Ds, F̃[line, dir], -1, 1): k outer, i inner
                     for (int k = 0; k < n; ++k) {
                         const int nk = line_node(nd, n, dir, f, k);

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 EQ_ADVECTION, EQ_EULER = 0, 1
-VOL_STRONG, VOL_SPLIT = 0, 1
+VOL_STRONG, VOL_SPLIT, VOL_HYBRID = 0, 1, 2
 TP_STDAVG, TP_CHANDRASEKHAR, TP_KENNEDY_GRUBER = 0, 1, 2
 AVG_STD, AVG_CHANDRASEKHAR = 0, 1
 DISS_NONE, DISS_LXF, DISS_MATRIX, DISS_SCALAR = 0, 1, 2, 3
@@ -42,6 +42,7 @@ def lib():
         L.oracle_set_option.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
         L.oracle_set_curved.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oracle_set_limiter.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+        L.oracle_set_blend.argtypes = [ctypes.c_void_p, ctypes.c_double]
         L.oracle_limit.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double]
         L.oracle_monitor.argtypes = [ctypes.c_void_p, _dp, ctypes.c_int, _dp]
         L.oracle_sizes.argtypes = [ctypes.c_void_p, _lp]
@@ -128,6 +129,10 @@ class OracleDisc:
 
     def set_option(self, which, value):
         lib().oracle_set_option(self.h, which, value)
+
+    def set_blend(self, blend):
+        """Blending constant c of the HybridDivOperator (volop VOL_HYBRID)."""
+        lib().oracle_set_blend(self.h, float(blend))
 
     def set_limiter(self, on, minval=0.0):
         """Zhang & Shu stage limiter applied by rk_run after every stage."""
