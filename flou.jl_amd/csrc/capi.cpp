@@ -512,20 +512,27 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     }
 
     // kernels
+    // the split form on nodes without boundaries (Gauss) has entropy-projected surface terms
+    const int kvol = (h->vol == FLOU_HIP_SPLIT && !h->gather) ? (int)VOL_SPLIT_NB : h->vol;
+    if (kvol == VOL_SPLIT_NB && (h->eq != FLOU_HIP_EULER || h->curved))
+        return bad(FLOU_HIP_EUNSUPPORTED,
+                   "the split form on Gauss nodes is implemented for the Euler equations on Cartesian elements");
     if (h->prec == FLOU_HIP_FP64) {
-        h->l64 = nd == 1 ? plain_f64_1d(n, h->eq, h->vol, h->tp, h->curved)
-                         : (nd == 2 ? plain_f64_2d(n, h->eq, h->vol, h->tp, h->curved)
-                                    : plain_f64_3d(n, h->eq, h->vol, h->tp, h->curved));
+        h->l64 = nd == 1 ? plain_f64_1d(n, h->eq, kvol, h->tp, h->curved)
+                         : (nd == 2 ? plain_f64_2d(n, h->eq, kvol, h->tp, h->curved)
+                                    : plain_f64_3d(n, h->eq, kvol, h->tp, h->curved));
         if (!h->l64) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     } else {
-        h->l32 = nd == 1 ? plain_f32_1d(n, h->eq, h->vol, h->tp, h->curved)
-                         : (nd == 2 ? plain_f32_2d(n, h->eq, h->vol, h->tp, h->curved)
-                                    : plain_f32_3d(n, h->eq, h->vol, h->tp, h->curved));
+        h->l32 = nd == 1 ? plain_f32_1d(n, h->eq, kvol, h->tp, h->curved)
+                         : (nd == 2 ? plain_f32_2d(n, h->eq, kvol, h->tp, h->curved)
+                                    : plain_f32_3d(n, h->eq, kvol, h->tp, h->curved));
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
     // adjoint kernels: StdAverage surface average with or without LxF dissipation
     if (h->vol == FLOU_HIP_HYBRID) {
         h->vjp_why = "adjoint: the HybridDivOperator is not differentiated";
+    } else if (kvol == VOL_SPLIT_NB) {
+        h->vjp_why = "adjoint: the split form on Gauss nodes is not differentiated";
     } else if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
         h->vjp_why = "adjoint: only the StdAverage surface average is differentiated";
     } else if (h->sdiss == FLOU_HIP_DISS_MATRIX || h->sdiss == FLOU_HIP_DISS_SCALAR) {
@@ -852,8 +859,9 @@ int flou_hip_check(flou_hip_handle* h) {
         HIPCHECK(h, hipMemsetAsync(h->d_err, 0, sizeof(int), h->stream));
         HIPCHECK(h, hipStreamSynchronize(h->stream));
         return fail(h, FLOU_HIP_EDOMAIN,
-                    (flag & ERR_SQRT_NEG) ? "DomainError: sqrt of a negative number (non-physical state)"
-                                          : "DomainError: log of a negative number (non-physical state)");
+                    (flag & ERR_SQRT_NEG)  ? "DomainError: sqrt of a negative number (non-physical state)"
+                    : (flag & ERR_LOG_NEG) ? "DomainError: log of a negative number (non-physical state)"
+                                           : "DomainError: power of a negative number (non-physical state)");
     }
     return FLOU_HIP_OK;
 }

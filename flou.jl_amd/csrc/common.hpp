@@ -9,6 +9,9 @@ namespace flouhip {
 
 enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
 enum { VOL_STRONG = 0, VOL_SPLIT = 1, VOL_HYBRID = 2 };
+// Kernel-internal variant: the split form on nodes without boundaries (Gauss), whose
+// surface terms use entropy-projected end states (OpDivergence.jl:300-436).
+enum { VOL_SPLIT_NB = 3 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
 enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };
@@ -24,7 +27,7 @@ enum {
 };
 
 // Error flags raised by kernels (the reference throws DomainError at these points).
-enum { ERR_SQRT_NEG = 1, ERR_LOG_NEG = 2 };
+enum { ERR_SQRT_NEG = 1, ERR_LOG_NEG = 2, ERR_POW_NEG = 4 };
 
 // Packed 1-D operators, offsets in units of scalars.
 __host__ __device__ constexpr int ops_D(int) { return 0; }

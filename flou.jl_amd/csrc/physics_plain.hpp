@@ -232,6 +232,28 @@ __device__ inline void cons2entropy(const T* Q, T gamma, int* err, T* W) {
     W[ND + 1] = -Q[0] / p;
 }
 
+// vars_entropy2prim then vars_prim2cons (FlouCommon/Euler.jl:309-339, 255-271).
+template <typename T, int ND>
+__device__ inline void entropy2cons(const T* W, T gamma, int* err, T* Q) {
+    const T wl = W[ND + 1];
+    T u[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) u[k] = -W[1 + k] / wl;
+    T q2 = u[0] * u[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) q2 = q2 + u[k] * u[k];
+    const T s = gamma - (gamma - T(1)) * (W[0] - (wl * q2) / T(2));
+    if (-wl < T(0)) atomicOr(err, ERR_POW_NEG);
+    const T base = pow(-wl, gamma) * exp(s);
+    if (base < T(0)) atomicOr(err, ERR_POW_NEG);
+    const T p = pow(base, T(1) / (T(1) - gamma));
+    const T rho = -p * wl;
+    Q[0] = rho;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) Q[1 + k] = rho * u[k];
+    Q[ND + 1] = p / (gamma - T(1)) + (rho * q2) / T(2);
+}
+
 // Euler numerical flux in the face frame (Equations/Euler.jl:99-380).
 template <typename T, int ND>
 __device__ inline void euler_numericalflux(const T* Ql, const T* Qr, int savg, int sdiss,

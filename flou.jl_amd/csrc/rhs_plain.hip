@@ -41,6 +41,11 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     // HybridDivOperator: subcell-interface fluxes of one direction, [le][v][line][s]
     constexpr int HB = VOL == VOL_HYBRID ? 1 : 0;
     __shared__ T sFb[HB ? EPB : 1][HB ? NV : 1][HB ? NPF : 1][HB ? N : 1];
+    // Split form on Gauss nodes: projected end states (Q, primitives, β) of every line and
+    // the two-point fluxes of every node with them, one direction at a time
+    constexpr int NB = VOL == VOL_SPLIT_NB ? 1 : 0;
+    __shared__ T sNb[NB ? EPB : 1][NB ? NPF : 1][2][NB ? 2 * NV + 1 : 1];
+    __shared__ T sNf[NB ? EPB : 1][2][NB ? NV : 1][NB ? NPE : 1];
 
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -258,7 +263,11 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
             __syncthreads();
         }
     }
-    if (!valid) return;
+    // The Gauss split form has block-wide phases after the volume terms: its threads past
+    // the last element stay, skipping every per-node step.
+    if constexpr (!NB) {
+        if (!valid) return;
+    }
 
     // Volume terms, one thread per node, in the reference's summation order
     const int64_t gnode = (int64_t)e * NPE + node;
@@ -272,7 +281,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         M = Mnode;
     }
 #pragma unroll
-    for (int dir = 0; dir < (VOL == VOL_HYBRID ? 0 : ND); ++dir) {
+    for (int dir = 0; dir < (VOL == VOL_HYBRID ? 0 : ND) && valid; ++dir) {
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         if constexpr (VOL == VOL_STRONG) {
@@ -333,16 +342,94 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         }
     }
 
-    // FR lift: dQ[line[ii]] -= ∂gl[ii]*Fn_left + ∂gr[ii]*Fn_right, direction by direction
     const T* gm = &sOps[ops_gm(N)];
     const T* gp = &sOps[ops_gp(N)];
+    if constexpr (NB) {
+        // Entropy-projected surface terms (_splitdiv_nb_surface_contribution!,
+        // OpDivergence.jl:300-436), direction by direction: end states u(l·w), u(r·w) of each
+        // line; f♯(Q_i, end state) per node; then
+        // dQ[i] += ∂gl[i](Fl_i - (l·Fl + Fn_l)) - ∂gr[i](Fr_i - (r·Fr - Fn_r)).
+        const T* lm = &sOps[ops_lm(N)];
+        const T* lp = &sOps[ops_lp(N)];
+        for (int dir = 0; dir < ND; ++dir) {
+            for (int it = tid; it < EPB * NPF; it += TPB) {
+                const int le2 = it / NPF, f = it % NPF;
+                if (e0 + le2 >= a.ne) continue;
+                T Wl[NV], Wr[NV];
+                for (int k = 0; k < N; ++k) {
+                    const int nk = line_node<ND, N>(dir, f, k);
+                    T qk[NV], Wk[NV];
 #pragma unroll
-    for (int dir = 0; dir < ND; ++dir) {
-        int f, ii;
-        node_line<ND, N>(node, dir, f, ii);
+                    for (int v = 0; v < NV; ++v) qk[v] = sQ[le2][v][nk];
+                    plain::cons2entropy<T, ND>(qk, a.gamma, a.errflag, Wk);
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
-            acc[v] = acc[v] - (gm[ii] * sFn[le][2 * dir][v][f] + gp[ii] * sFn[le][2 * dir + 1][v][f]);
+                    for (int v = 0; v < NV; ++v) {
+                        Wl[v] = k == 0 ? lm[0] * Wk[v] : Wl[v] + lm[k] * Wk[v];
+                        Wr[v] = k == 0 ? lp[0] * Wk[v] : Wr[v] + lp[k] * Wk[v];
+                    }
+                }
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    T Qe[NV], Pe[NV];
+                    plain::entropy2cons<T, ND>(side ? Wr : Wl, a.gamma, a.errflag, Qe);
+                    plain::cons2prim<T, ND>(Qe, a.gamma, Pe);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) {
+                        sNb[le2][f][side][v] = Qe[v];
+                        sNb[le2][f][side][NV + v] = Pe[v];
+                    }
+                    sNb[le2][f][side][2 * NV] = Pe[0] / (T(2) * Pe[ND + 1]);
+                }
+            }
+            __syncthreads();
+            if (valid) {
+                int f, ii;
+                node_line<ND, N>(node, dir, f, ii);
+                T qi[NV], Pi[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) { qi[v] = sQ[le][v][node]; Pi[v] = sP[le][v][node]; }
+                const T bi = sP[le][NV][node];
+#pragma unroll
+                for (int side = 0; side < 2; ++side) {
+                    const T* E = &sNb[le][f][side][0];
+                    T Fs[NV];
+                    plain::twopointflux<T, ND, EQ, TP>(qi, E, Pi, E + NV, bi, E[2 * NV], M + ND * dir,
+                                                       M + ND * dir, a.gamma, avel, a.errflag, Fs);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sNf[le][side][v][node] = Fs[v];
+                }
+            }
+            __syncthreads();
+            if (valid) {
+                int f, ii;
+                node_line<ND, N>(node, dir, f, ii);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    const int n0 = line_node<ND, N>(dir, f, 0);
+                    T lF = lm[0] * sNf[le][0][v][n0], rF = lp[0] * sNf[le][1][v][n0];
+                    for (int k = 1; k < N; ++k) {
+                        const int nk = line_node<ND, N>(dir, f, k);
+                        lF = lF + lm[k] * sNf[le][0][v][nk];
+                        rF = rF + lp[k] * sNf[le][1][v][nk];
+                    }
+                    const T fl = sNf[le][0][v][node] - (lF + sFn[le][2 * dir][v][f]);
+                    const T fr = sNf[le][1][v][node] - (rF - sFn[le][2 * dir + 1][v][f]);
+                    acc[v] = acc[v] + (gm[ii] * fl - gp[ii] * fr);
+                }
+            }
+            __syncthreads();
+        }
+        if (!valid) return;
+    } else {
+        // FR lift: dQ[line[ii]] -= ∂gl[ii]*Fn_left + ∂gr[ii]*Fn_right, direction by direction
+#pragma unroll
+        for (int dir = 0; dir < ND; ++dir) {
+            int f, ii;
+            node_line<ND, N>(node, dir, f, ii);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                acc[v] = acc[v] - (gm[ii] * sFn[le][2 * dir][v][f] + gp[ii] * sFn[le][2 * dir + 1][v][f]);
+        }
     }
 
     // Mass matrix (ldiv! by Diagonal(J)) and the stage epilogue
@@ -386,11 +473,20 @@ static hipError_t launch_plain(const RhsArgs<T>& a, hipStream_t s) {
 template <typename T, int ND, int N, bool CURV>
 static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
     if (eq == EQ_ADVECTION) {
-        if (vol == VOL_HYBRID) return nullptr;
+        if (vol == VOL_HYBRID || vol == VOL_SPLIT_NB) return nullptr;
         return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
                                  : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
     }
     if (vol == VOL_STRONG) return &launch_plain<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV>;
+    if (vol == VOL_SPLIT_NB) {
+        if constexpr (CURV) return nullptr;  // subcell geometry is defined for Cartesian elements
+        switch (tp) {
+            case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_STDAVG, false>;
+            case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_CHANDRASEKHAR, false>;
+            case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_KENNEDY_GRUBER, false>;
+            default: return nullptr;
+        }
+    }
     if (vol == VOL_HYBRID) {
         if constexpr (CURV) return nullptr;  // subcell geometry is defined for Cartesian elements
         switch (tp) {

@@ -764,4 +764,13 @@ void oracle_cons2entropy(int nd, double gamma, const double* Q, double* W) {
     cons2entropy(Q, nd, gamma, W);
 }
 
+int oracle_entropy2cons(int nd, double gamma, const double* W, double* Q) {
+    try {
+        entropy2cons(W, nd, gamma, Q);
+    } catch (std::domain_error&) {
+        return -5;
+    }
+    return 0;
+}
+
 }  // extern "C"

@@ -38,6 +38,12 @@ inline double olog(double x) {
     return std::log(x);
 }
 inline double oabs(double x) { return std::fabs(x); }
+inline double oexp(double x) { return std::exp(x); }
+// Julia's x^y for a real, non-integer y throws DomainError when x < 0
+inline double opow(double x, double y) {
+    if (x < 0.0) throw std::domain_error("pow of negative number");
+    return std::pow(x, y);
+}
 inline double omax(double a, double b) { return (a < b) ? b : a; }
 inline double omin(double a, double b) { return (b < a) ? b : a; }
 
@@ -78,6 +84,11 @@ template <int K> inline bool operator<(const Dual<K>& a, const Dual<K>& b) { ret
 template <int K> inline bool operator<(const Dual<K>& a, double b) { return a.v < b; }
 template <int K> inline Dual<K> osqrt(const Dual<K>& a) {
     Dual<K> r; r.v = osqrt(a.v); for (int k = 0; k < K; ++k) r.d[k] = a.d[k] / (2.0 * r.v); return r; }
+template <int K> inline Dual<K> oexp(const Dual<K>& a) {
+    Dual<K> r; r.v = oexp(a.v); for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * r.v; return r; }
+template <int K> inline Dual<K> opow(const Dual<K>& a, double y) {
+    Dual<K> r; r.v = opow(a.v, y); const double g = y * opow(a.v, y - 1.0);
+    for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * g; return r; }
 template <int K> inline Dual<K> olog(const Dual<K>& a) {
     Dual<K> r; r.v = olog(a.v); for (int k = 0; k < K; ++k) r.d[k] = a.d[k] / a.v; return r; }
 template <int K> inline Dual<K> oabs(const Dual<K>& a) { return (a.v < 0.0) ? -a : a; }
@@ -381,6 +392,23 @@ inline void cons2entropy(const T* Q, int nd, double gamma, T* W) {
     W[0] = (gamma - s) / (gamma - 1.0) - (m2 / Q[0]) / (2.0 * p);
     for (int k = 0; k < nd; ++k) W[1 + k] = Q[1 + k] / p;
     W[nd + 1] = -Q[0] / p;
+}
+
+// Entropy variables back to conservative ones: vars_entropy2prim then vars_prim2cons
+// (FlouCommon/Euler.jl:309-339, 255-271).
+template <class T>
+inline void entropy2cons(const T* W, int nd, double gamma, T* Q) {
+    const T wl = W[nd + 1];
+    T u[3];
+    for (int k = 0; k < nd; ++k) u[k] = -W[1 + k] / wl;
+    T q2 = u[0] * u[0];
+    for (int k = 1; k < nd; ++k) q2 = q2 + u[k] * u[k];
+    const T s = gamma - (gamma - 1.0) * (W[0] - (wl * q2) / 2.0);
+    const T p = opow(opow(-wl, gamma) * oexp(s), 1.0 / (1.0 - gamma));
+    const T rho = -p * wl;
+    Q[0] = rho;
+    for (int k = 0; k < nd; ++k) Q[1 + k] = rho * u[k];
+    Q[nd + 1] = p / (gamma - 1.0) + (rho * q2) / 2.0;
 }
 
 // Euler surface flux in the rotated frame (Equations/Euler.jl:99-380)
@@ -781,6 +809,71 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
     }
     if (domain_failed) throw std::domain_error("DomainError in the interface fluxes");
 
+    // Split form on nodes without boundaries (Gauss): the entropy-projected surface terms
+    // (_splitdiv_nb_surface_contribution!, OpDivergence.jl:300-436) replace the FR lift.
+    bool gauss = false;
+    for (int a = 0; a < n; ++a)
+        if (o.lm[a] != (a == 0 ? 1.0 : 0.0) || o.lp[a] != (a == n - 1 ? 1.0 : 0.0)) gauss = true;
+    if (d.volop == VOL_SPLIT && gauss && d.eq == EQ_EULER) {
+#pragma omp parallel for schedule(static)
+        for (long e = 0; e < ne; ++e) try {
+            const long base = e * npe;
+            std::vector<T> Qe((size_t)npe * nv), We((size_t)npe * nv), Fl((size_t)n * nv), Fr((size_t)n * nv);
+            for (int i = 0; i < npe; ++i)
+                for (int v = 0; v < nv; ++v) Qe[i * nv + v] = Q[(size_t)v * N + base + i];
+            for (int i = 0; i < npe; ++i) cons2entropy(&Qe[i * nv], nd, d.gamma, &We[i * nv]);
+            for (int dir = 0; dir < nd; ++dir) {
+                double fr[9], sjac, nvec[3];
+                subgrid_frame(d, dir, fr, sjac);
+                for (int k = 0; k < nd; ++k) nvec[k] = fr[k] * sjac;
+                const long facel = d.e_face[e * 2 * nd + 2 * dir];
+                const int sidel = d.e_side[e * 2 * nd + 2 * dir];
+                const long facer = d.e_face[e * 2 * nd + 2 * dir + 1];
+                const int sider = d.e_side[e * 2 * nd + 2 * dir + 1];
+                for (int f = 0; f < npf; ++f) {
+                    // entropy-projected end states
+                    T Wl[5], Wr[5], Ql[5], Qr[5];
+                    for (int v = 0; v < nv; ++v) {
+                        const int n0 = line_node(nd, n, dir, f, 0);
+                        T sl = o.lm[0] * We[n0 * nv + v], sr = o.lp[0] * We[n0 * nv + v];
+                        for (int a = 1; a < n; ++a) {
+                            const int na = line_node(nd, n, dir, f, a);
+                            sl = sl + o.lm[a] * We[na * nv + v];
+                            sr = sr + o.lp[a] * We[na * nv + v];
+                        }
+                        Wl[v] = sl;
+                        Wr[v] = sr;
+                    }
+                    entropy2cons(Wl, nd, d.gamma, Ql);
+                    entropy2cons(Wr, nd, d.gamma, Qr);
+                    for (int a = 0; a < n; ++a) {
+                        const int na = line_node(nd, n, dir, f, a);
+                        twopointflux(d, &Qe[na * nv], Ql, d.metric(e, na) + nd * dir, nvec, &Fl[a * nv]);
+                        twopointflux(d, &Qe[na * nv], Qr, d.metric(e, na) + nd * dir, nvec, &Fr[a * nv]);
+                    }
+                    const T* fnl = FN(facel, sidel, f);
+                    const T* fnr = FN(facer, sider, f);
+                    for (int v = 0; v < nv; ++v) {
+                        T lF = o.lm[0] * Fl[v], rF = o.lp[0] * Fr[v];
+                        for (int a = 1; a < n; ++a) {
+                            lF = lF + o.lm[a] * Fl[a * nv + v];
+                            rF = rF + o.lp[a] * Fr[a * nv + v];
+                        }
+                        for (int a = 0; a < n; ++a) {
+                            const int na = line_node(nd, n, dir, f, a);
+                            const T fl = Fl[a * nv + v] - (lF + fnl[v]);
+                            const T frr = Fr[a * nv + v] - (rF - fnr[v]);
+                            T& x = dQ[(size_t)v * N + base + na];
+                            x = x + (o.gm[a] * fl - o.gp[a] * frr);
+                        }
+                    }
+                }
+            }
+        } catch (std::domain_error&) {
+            domain_failed = 1;
+        }
+        if (domain_failed) throw std::domain_error("DomainError in the entropy-projected surface terms");
+    } else {
     // surface_contribution! (OpDivergence.jl:42-100)
 #pragma omp parallel for schedule(static)
     for (long e = 0; e < ne; ++e) {
@@ -802,6 +895,7 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
                 }
             }
         }
+    }
     }
 
     // apply_massmatrix! (MultielementDiscontinuous.jl:132-137): ldiv!(Diagonal(J), dQ)

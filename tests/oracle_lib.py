@@ -67,6 +67,7 @@ def lib():
         L.oracle_twopoint.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oracle_numflux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_cons2entropy.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.oracle_entropy2cons.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_last_error.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -245,6 +246,14 @@ def cons2entropy(nd, gamma, Q):
     W = np.zeros(nd + 2)
     lib().oracle_cons2entropy(nd, float(gamma), _p(Q), _p(W))
     return W
+
+
+def entropy2cons(nd, gamma, W):
+    W = np.ascontiguousarray(W, dtype=np.float64)
+    Q = np.zeros(nd + 2)
+    if lib().oracle_entropy2cons(nd, float(gamma), _p(W), _p(Q)) != 0:
+        raise ArithmeticError("entropy2cons: pow of a negative number")
+    return Q
 
 
 def tableau_2n(scheme):

@@ -113,6 +113,16 @@ def test_create_rejects_unsupported_hybrid(setup, code, text):
     assert text in C.lib.flou_hip_create_error()
 
 
+def test_create_rejects_gauss_split_advection():
+    keep = []
+    d = _valid_desc(keep)
+    d.equation, d.nvars, d.tp_flux, d.surface_avg, d.surface_diss = C.ADVECTION, 1, 0, 0, 1
+    keep[3][:] = [0.5, 0.5, 0.0]  # traces that are not node values: Gauss-type nodes
+    h = ctypes.c_void_p()
+    assert C.lib.flou_hip_create(ctypes.byref(d), ctypes.byref(h)) == C.EUNSUPPORTED
+    assert b"Gauss" in C.lib.flou_hip_create_error()
+
+
 def test_create_rejects_inconsistent_connectivity():
     keep = []
     d = _valid_desc(keep)
