@@ -4,28 +4,27 @@
 //
 // Same mathematics as SplitDivOperator + interface_fluxes! + the FR lift
 // (OpDivergence.jl:200-299, Interfaces.jl:111-136), organised for gfx950:
-//   * EPB elements per block.  Staging: every thread loads all five variables of its
-//     nodes (coalesced, one uniform base per variable) and forms the two-point flux
-//     inputs in registers; the exterior traces of the block's 6*EPB faces (neighbour
-//     nodes or boundary states) are staged in LDS at the same time, so their latency
-//     overlaps the element loads;
-//   * one thread per (element, direction, line) -- every thread does the same work:
-//       - all N(N-1)/2 pairs of its line, each evaluated once (F♯ is symmetric), both rows
-//         accumulated in registers (F♯ never touches memory);
-//       - the numerical flux on the two face nodes where the line ends and the FR lift of
-//         those fluxes into the two end rows (GLL: ∂g lives on the ends);
-//     loads and stores of a line are specialised per direction, so node strides and the
-//     velocity permutation into the line frame are compile-time LDS offsets;
+//   * EPB elements per block, one thread per (element, direction, line) -- every thread
+//     does the same work.  Each thread first loads the face tables and then the two
+//     exterior traces of its line (neighbour nodes or boundary states, into registers),
+//     issued after the block's own nodes so that the staging waits for those only;
+//   * staging: every thread loads all five variables of its nodes (coalesced, one
+//     uniform base per variable) and writes the node states and the two-point flux
+//     inputs to LDS;
+//   * line phase: the numerical flux on the two face nodes where the line ends, lifted
+//     into the two end rows (GLL: ∂g lives on the ends), then all N(N-1)/2 pairs of the
+//     line, each evaluated once (F♯ is symmetric), both rows accumulated in registers
+//     (F♯ never touches memory).  Line-frame velocity slots are consecutive in LDS
+//     (u, v, w, u, v), so one per-lane offset serves every direction;
 //   * per node: Σ_d (Ja_dd/J)·(row sum of direction d) and the RK stage epilogue.  The row
-//     sums of the three directions go to three LDS regions, two of which reuse the flux
-//     inputs and the exterior traces once every line is done.
+//     sums go to LDS in line-frame order over the dead flux inputs and node states.
 // FP32 reformulations (the FP64 plain kernel keeps the reference's operation order):
 //   - node inputs are pre-halved (ρ/2, u/2) so a mean is one add;
 //   - log-mean ρ̂ = (a+b)/(2F), F = atanh(f)/f, f = (a-b)/(a+b), u = f²
 //     (Utilities.jl:34-44).  Each line picks a branch-free path from the spread of ρ and
-//     β along it: u < 1e-4: F = 1 + u/3, 1/F = 1 - u/3 (truncation < 2e-9), with
-//     1/(a+b) ≈ (1/a + 1/b)/4 inside the O(u) correction (relative error u);
-//     u < 1e-2: the reference's three-term series; otherwise per pair the series or
+//     β along it: u < 1e-4: F = 1 + u/3, 1/F = 1 - u/3 (truncation < 2e-9) with no
+//     reciprocal per pair (precomputed per-node inputs, see twopoint); u < 1e-2: the
+//     reference's three-term series; otherwise per pair the series or
 //     F = (ln a - ln b)/(2f);
 //   - 1/(2β̂) = F_β/(β1+β2) and p̂/ρ̂ = F_ρ/(β1+β2) (algebraically identical);
 //   - energy: |ū|² - ½·avg(|u|²) = ½ u1·u2, and the energy row is accumulated halved;
@@ -48,6 +47,9 @@ constexpr int r4(int x) { return (x + 3) & ~3; }
 #ifndef FAST_EPB6
 #define FAST_EPB6 2
 #endif
+#ifndef FAST_WAVES
+#define FAST_WAVES
+#endif
 
 template <int N>
 struct FCfg {
@@ -57,44 +59,63 @@ struct FCfg {
     static constexpr int EPB = N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? FAST_EPB6 : 1));
     static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;          // threads per block
     static constexpr int NN = EPB * NPE;                             // nodes per block
-    static constexpr int NFN = EPB * 6 * NPF;                        // face nodes per block
     static constexpr int NPT = (NN + TPB - 1) / TPB;                 // nodes per thread
-    static constexpr int FPT = (NFN + TPB - 1) / TPB;                // face nodes per thread
-    // LDS layout in floats.  Q: [5][NN]; P: [6][NN] two-point flux inputs;
-    // X: [EPB][6][5][NPF] exterior traces; C: [5][NN]; G: [EPB][4]; F: [EPB][6] float2.
-    // Row sums of direction 0 reuse X, direction 1 reuses P, direction 2 uses C.
-    // Arrays are indexed by block node, so lanes on consecutive lines hit distinct banks.
+    // LDS layout in floats.  Q: [5][NN] node states; P: [10][NN] two-point flux inputs
+    // (slots 1..5 hold u, v, w, u, v, so the three velocity slots of a line in any
+    // direction are consecutive; slot 8 is an extra Chandrasekhar input; slot 9 is spare
+    // room for the row sums); G: [EPB][4]; F: [EPB][6] float2.  Once every line is done
+    // the row sums (line-frame variable order) of directions 0 and 1 go to P, those of
+    // direction 2 to Q.  Arrays are indexed by block node.
     static constexpr int OFF_Q = 0;
     static constexpr int OFF_P = r4(5 * NN);
-    static constexpr int OFF_X = r4(OFF_P + 6 * NN);
-    static constexpr int SZ_X = 30 * EPB * NPF > 5 * NN ? 30 * EPB * NPF : 5 * NN;
-    static constexpr int OFF_C = r4(OFF_X + SZ_X);
-    static constexpr int OFF_G = r4(OFF_C + 5 * NN);
+    static constexpr int OFF_G = r4(OFF_P + 10 * NN);
     static constexpr int OFF_F = OFF_G + 4 * EPB;
     static constexpr int LDS = OFF_F + 12 * EPB;
 };
 
-// Node inputs (6 slots):
-//   Chandrasekhar:  ρ/2, u/2, v/2, w/2 | β = ρ/(2p), 2/ρ
+// Node inputs (6 values, stored in 8 LDS slots with u, v repeated after w; a 7th,
+// H = 1/(4β) = p/(2ρ), for the Chandrasekhar flux):
+//   Chandrasekhar:  ρ/2, u/2, v/2, w/2 | β = ρ/(2p), -(1/6)/ρ | H
 //   Kennedy-Gruber: ρ/2, u/2, v/2, w/2 | p/2, e/2
 //   StdAverage:     ρ,   u,   v,   w   | p,   ρe
-// In a line task slot 1 holds the velocity along the line, slots 2, 3 the other two
-// (cyclic order).  Chandrasekhar returns the flux factored as (m, F_1, u_t1, u_t2, h/2):
+// In a line task value 1 is the velocity along the line, values 2, 3 the other two
+// (cyclic order: LDS slots 1+d, 2+d, 3+d for direction d).  Chandrasekhar returns the flux factored as (m, F_1, u_t1, u_t2, h/2):
 // F = (m, F_1, m·u_t1, m·u_t2, m·h/2) with the energy component halved.
 template <int TP, int PATH>
 __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float inv_gm1, float* F) {
     if constexpr (TP == TP_CHANDRASEKHAR) {
         const float hs = Pi[0] + Pj[0], dA = Pi[0] - Pj[0];
-        const float sb = Pi[4] + Pj[4], db = Pi[4] - Pj[4];
-        const float rb = rcp(sb);
+        const float db = Pi[4] - Pj[4];
+        const float rb = PATH == 2 ? 0.0f : rcp(Pi[4] + Pj[4]);
         const float fb = db * rb;
         const float ub = fb * fb;
         float rho, X;  // X = (F_β/(γ-1) + F_ρ)/2
-        if constexpr (PATH == 1) {
-            const float fr4 = dA * (Pi[5] + Pj[5]);  // ≈ 4f
-            const float ur16 = fr4 * fr4;
-            X = fmaf(ub, inv_gm1 * (1.0f / 6.0f), fmaf(ur16, 1.0f / 96.0f, 0.5f * (inv_gm1 + 1.0f)));
-            rho = fmaf(dA * fr4, -1.0f / 12.0f, hs);
+        if constexpr (PATH == 2) {
+            // Both log-means to first order in u (u < 1e-4: F = 1 + u/3, truncation < 2e-9),
+            // without a reciprocal per pair:
+            //   1/(β1+β2) = (H1+H2)(1 - f_β²) exactly, and f_β = (β1-β2)/(β1+β2) ≈
+            //   (β1-β2)(H1+H2) inside the O(u) terms (relative error 2f⁴ < 2e-8);
+            //   slot 5 holds -(1/6)/ρ, so fr = dA·(slot5_1 + slot5_2) = -f_ρ/3·(1 + O(f²)),
+            //   ρ̂ = ρ̄(1 - f_ρ²/3) = hs + dA·fr and the F_ρ term f_ρ²/6 of X is 3/2·fr².
+            const float sh = Pi[6] + Pj[6];
+            const float fh = db * sh;
+            const float uh = fh * fh;
+            const float rbh = fmaf(-sh, uh, sh);
+            const float fr = dA * (Pi[5] + Pj[5]);
+            X = fmaf(uh, inv_gm1 * (1.0f / 6.0f), fmaf(fr * fr, 1.5f, 0.5f * (inv_gm1 + 1.0f)));
+            rho = fmaf(dA, fr, hs);
+            const float p = hs * rbh;
+            const float un = Pi[1] + Pj[1];
+            const float ut1 = Pi[2] + Pj[2];
+            const float ut2 = Pi[3] + Pj[3];
+            const float dot = fmaf(Pi[1], Pj[1], fmaf(Pi[2], Pj[2], Pi[3] * Pj[3]));
+            const float m = rho * un;
+            F[0] = m;
+            F[1] = fmaf(m, un, p);
+            F[2] = ut1;
+            F[3] = ut2;
+            F[4] = fmaf(rbh, X, dot);
+            return;
         } else {
             const float fr = dA * rcp(hs);
             const float ur = fr * fr;
@@ -146,10 +167,11 @@ __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float
     }
 }
 
-// Rows i and j of the line accumulate -D♯_ij F and -D♯_ji F.  The pairs (0, j) carry the
-// first contribution to every row, so they set the accumulators.
+// Rows i and j of the line accumulate -D♯_ij F and -D♯_ji F.  Rows 0 and N-1 already
+// hold the lifted face fluxes; the pairs (0, j) carry the first contribution to every
+// other row, so they set those accumulators.
 template <int N, int TP, int PATH>
-__device__ __forceinline__ void line_pairs(const float (&P)[N][6], float (&acc)[N][5],
+__device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[N][5],
                                            const float* __restrict__ Dsh, float inv_gm1) {
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -158,39 +180,23 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][6], float (&acc)[
             float F[5];
             twopoint<TP, PATH>(P[i], P[j], inv_gm1, F);
             const float cij = Dsh[i * N + j], cji = Dsh[j * N + i];
+            const bool setj = i == 0 && j < N - 1;  // first contribution to row j
             if constexpr (TP == TP_CHANDRASEKHAR) {
                 const float mi = cij * F[0], mj = cji * F[0];
-                if (i == 0) {
-                    acc[i][0] = (j == 1) ? -mi : acc[i][0] - mi;
-                    acc[j][0] = -mj;
-                    acc[i][1] = (j == 1) ? -cij * F[1] : fmaf(-cij, F[1], acc[i][1]);
-                    acc[j][1] = -cji * F[1];
+                acc[i][0] -= mi;
+                acc[i][1] = fmaf(-cij, F[1], acc[i][1]);
+                acc[j][0] = setj ? -mj : acc[j][0] - mj;
+                acc[j][1] = setj ? -cji * F[1] : fmaf(-cji, F[1], acc[j][1]);
 #pragma unroll
-                    for (int v = 2; v < 5; ++v) {
-                        acc[i][v] = (j == 1) ? -mi * F[v] : fmaf(-mi, F[v], acc[i][v]);
-                        acc[j][v] = -mj * F[v];
-                    }
-                } else {
-                    acc[i][0] -= mi;
-                    acc[j][0] -= mj;
-                    acc[i][1] = fmaf(-cij, F[1], acc[i][1]);
-                    acc[j][1] = fmaf(-cji, F[1], acc[j][1]);
-#pragma unroll
-                    for (int v = 2; v < 5; ++v) {
-                        acc[i][v] = fmaf(-mi, F[v], acc[i][v]);
-                        acc[j][v] = fmaf(-mj, F[v], acc[j][v]);
-                    }
+                for (int v = 2; v < 5; ++v) {
+                    acc[i][v] = fmaf(-mi, F[v], acc[i][v]);
+                    acc[j][v] = setj ? -mj * F[v] : fmaf(-mj, F[v], acc[j][v]);
                 }
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
-                    if (i == 0) {
-                        acc[i][v] = (j == 1) ? -cij * F[v] : fmaf(-cij, F[v], acc[i][v]);
-                        acc[j][v] = -cji * F[v];
-                    } else {
-                        acc[i][v] = fmaf(-cij, F[v], acc[i][v]);
-                        acc[j][v] = fmaf(-cji, F[v], acc[j][v]);
-                    }
+                    acc[i][v] = fmaf(-cij, F[v], acc[i][v]);
+                    acc[j][v] = setj ? -cji * F[v] : fmaf(-cji, F[v], acc[j][v]);
                 }
             }
         }
@@ -199,24 +205,27 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][6], float (&acc)[
 
 // Line addressing.  Node p of a line is block node base + p*stride; the momentum along the
 // line and the two others (line-frame slots 1, 2, 3) are physical variables m[0..2] --
-// per-lane offsets, so one instruction stream serves lines of every direction.
+// per-lane offsets, so one instruction stream serves lines of every direction.  vel is
+// the per-lane offset (1 + direction)*NN of the line's first velocity slot in P.
 struct LineAddr {
-    int base, stride;
+    int base, stride, vel;
     int m[3];
 };
 
 // flux inputs of the N nodes of a line, line-frame order (slot arrays of NN floats)
-template <int N, int NN>
-__device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr& L, float (&P)[N][6]) {
+template <int N, int NN, bool H>
+__device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr& L, float (&P)[N][7]) {
 #pragma unroll
     for (int p = 0; p < N; ++p) {
         const float* q = sP + L.base + p * L.stride;
+        const float* qv = q + L.vel;
         P[p][0] = q[0];
-        P[p][1] = q[L.m[0] * NN];
-        P[p][2] = q[L.m[1] * NN];
-        P[p][3] = q[L.m[2] * NN];
-        P[p][4] = q[4 * NN];
-        P[p][5] = q[5 * NN];
+        P[p][1] = qv[0];
+        P[p][2] = qv[NN];
+        P[p][3] = qv[2 * NN];
+        P[p][4] = q[6 * NN];
+        P[p][5] = q[7 * NN];
+        P[p][6] = H ? q[8 * NN] : 0.0f;
     }
 }
 
@@ -230,26 +239,27 @@ __device__ __forceinline__ void load_vars(const float* __restrict__ s, const Lin
     o[4] = s[4 * VS];
 }
 
+// row sums of a line in line-frame variable order (the node phase undoes the permutation)
 template <int N, int NN>
 __device__ __forceinline__ void store_line(float* __restrict__ R, const LineAddr& L, const float (&acc)[N][5]) {
 #pragma unroll
     for (int p = 0; p < N; ++p) {
         float* r = R + L.base + p * L.stride;
-        r[0] = acc[p][0];
-        r[L.m[0] * NN] = acc[p][1];
-        r[L.m[1] * NN] = acc[p][2];
-        r[L.m[2] * NN] = acc[p][3];
-        r[4 * NN] = acc[p][4];
+#pragma unroll
+        for (int c = 0; c < 5; ++c) r[c * NN] = acc[p][c];
     }
 }
 
-// Numerical flux on the face where a line ends (row R of the line) and its lift:
-// acc[R] -= coef·[s(f_own + f_ext) + λ(Q_own - Q_ext)], all in the line frame
-// (slot 1 = momentum along the line).
+// line-frame slot of physical variable v in a line of direction d (momenta cycle)
+__host__ __device__ constexpr int lslot(int d, int v) { return (v == 0 || v == 4) ? v : ((v - 1 - d + 3) % 3) + 1; }
+
+// Numerical flux on the face where a line ends (row R of the line) and its lift, which
+// starts row R: acc[R] = -coef·[s(f_own + f_ext) + λ(Q_own - Q_ext)], in the line frame
+// (slot 1 = momentum along the line); the energy entry is scaled by ehalf.
 template <int N, int R>
-__device__ __forceinline__ void face_lift(const float (&o)[5], const float (&x)[5], float s, float coef,
-                                          float gm1, float gamma, float diss, float (&acc)[N][5],
-                                          bool& bad) {
+__device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[5], float s, float coef,
+                                          float gm1, float gamma, float diss, float ehalf,
+                                          float (&acc)[N][5], bool& bad) {
     float fo[5], fx[5], ao, ax;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
@@ -269,10 +279,12 @@ __device__ __forceinline__ void face_lift(const float (&o)[5], const float (&x)[
         f[4] = (Q[4] + p) * un;
         if (side) ax = a; else ao = a;
     }
-    const float lam = fmaxf(ao, ax) * diss;
+    const float cs = coef * s, cl = coef * fmaxf(ao, ax) * diss;
 #pragma unroll
-    for (int v = 0; v < 5; ++v)
-        acc[R][v] = fmaf(-coef, fmaf(s, fo[v] + fx[v], lam * (o[v] - x[v])), acc[R][v]);
+    for (int v = 0; v < 5; ++v) {
+        const float t = fmaf(-cs, fo[v] + fx[v], -cl * (o[v] - x[v]));
+        acc[R][v] = v == 4 ? ehalf * t : t;
+    }
 }
 
 // Buffer resources: a uniform base in SGPRs, 32-bit lane byte offsets, and the
@@ -290,17 +302,15 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float x) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)off, 0, 0);
 }
 
-template <int N, int TP>
-__global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
+template <int N, int TP, int MODE>
+__global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const FastArgs a) {
     using C = FCfg<N>;
     constexpr int NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB, TPB = C::TPB;
-    constexpr int NN = C::NN, NFN = C::NFN, NPT = C::NPT, FPT = C::FPT;
+    constexpr int NN = C::NN, NPT = C::NPT;
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
     float* const sQ = sm + C::OFF_Q;
     float* const sP = sm + C::OFF_P;
-    float* const sX = sm + C::OFF_X;
-    float* const sC = sm + C::OFF_C;
     float* const sG = sm + C::OFF_G;
     float2* const sF = reinterpret_cast<float2*>(sm + C::OFF_F);
 
@@ -312,20 +322,26 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
     const uint32_t n0 = (uint32_t)e0 * NPE;  // first node of the block
     int err = 0;                              // domain-error bits, reported once at the end
 
-    // ---- staging ---------------------------------------------------------------------
-    // face tables of the face nodes this thread stages (first: they feed addresses)
-    int fnb[FPT], finf[FPT];
+    // ---- this thread's line task ----------------------------------------------------
+    // One thread per (element, direction, line); lanes past the block's line tasks repeat
+    // a valid line (their results are dropped), so the whole wave runs one stream.
+    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
+    const int lle = tl / NL, lt = tl - (tl / NL) * NL;
+    const int ldir = lt / NPF, lf = lt - (lt / NPF) * NPF;
+    const bool line_in = lle < nel;
+    const bool line_ok = tid < EPB * NL && line_in;
+
+    // face tables of both line ends (faces 2*ldir and 2*ldir+1), first: they feed the
+    // exterior addresses
+    int xnb[2], xinf[2];
 #pragma unroll
-    for (int r = 0; r < FPT; ++r) {
-        const int fn = tid + r * TPB;
-        const int le = fn / (6 * NPF), k = (fn - le * 6 * NPF) / NPF;
-        fnb[r] = -2;
-        finf[r] = 0;
-        if (fn < NFN && le < nel) {
-            fnb[r] = a.nb[(e0 + le) * 6 + k];
-            finf[r] = a.finfo[(e0 + le) * 6 + k];
-        }
+    for (int sd = 0; sd < 2; ++sd) {
+        const int slot = (e0 + lle) * 6 + 2 * ldir + sd;
+        xnb[sd] = line_in ? a.nb[slot] : -2;
+        xinf[sd] = line_in ? a.finfo[slot] : 0;
     }
+
+    // ---- staging ---------------------------------------------------------------------
     // element nodes, all five variables per node (lanes past the block's last element read 0)
     const uint32_t blk_bytes = (uint32_t)(nel * NPE) * 4u;
     float q[NPT][5];
@@ -335,46 +351,22 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
 #pragma unroll
         for (int r = 0; r < NPT; ++r) q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
     }
-    // exterior traces: neighbour node (interior faces, branch-free; other lanes read 0),
-    // then boundary states or the own node (outflow, slip) where the face is a boundary
-    float x[FPT][5];
-    bool slip[FPT];
-    uint32_t xoff[FPT];
+    // exterior traces at both line ends: the neighbour's node on interior faces
+    // (branch-free; other lanes read 0), issued after the element nodes so that the
+    // staging below waits only for those
+    uint32_t xo[2];
 #pragma unroll
-    for (int r = 0; r < FPT; ++r) {
-        const int f = ((tid + r * TPB) % (6 * NPF)) % NPF;
-        const int kp = finfo_nbpos(finf[r]);
-        xoff[r] = fnb[r] >= 0 ? ((uint32_t)fnb[r] * NPE + line_node<3, N>(kp >> 1, f, (kp & 1) ? N - 1 : 0)) * 4u
+    for (int sd = 0; sd < 2; ++sd) {
+        const int kp = finfo_nbpos(xinf[sd]);
+        xo[sd] = xnb[sd] >= 0 ? ((uint32_t)xnb[sd] * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
                               : 0xffffffffu;
     }
+    float xr[2][5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
 #pragma unroll
-        for (int r = 0; r < FPT; ++r) x[r][v] = bload(rv, xoff[r]);
-    }
-#pragma unroll
-    for (int r = 0; r < FPT; ++r) {
-        slip[r] = false;
-        if (fnb[r] == -1) {
-            const int fn = tid + r * TPB;
-            const int le = fn / (6 * NPF), rem = fn - le * 6 * NPF, k = rem / NPF, f = rem - k * NPF;
-            const int bd = finfo_bd(finf[r]);
-            const int type = a.bctype[bd];
-            const float* src;
-            int64_t vs = 1;
-            if (type == BC_DIRICHLET) {
-                src = a.bcstate + ((int64_t)a.fid[(e0 + le) * 6 + k] * NPF + f) * 5;
-            } else if (type == BC_INFLOW) {
-                src = a.bcconst + bd * 5;
-            } else {
-                src = a.Q + n0 + le * NPE + line_node<3, N>(k >> 1, f, (k & 1) ? N - 1 : 0);
-                vs = Nd;
-                slip[r] = type == BC_SLIP;
-            }
-#pragma unroll
-            for (int v = 0; v < 5; ++v) x[r][v] = src[v * vs];
-        }
+        for (int sd = 0; sd < 2; ++sd) xr[sd][v] = bload(rv, xo[sd]);
     }
     // per face: own outward normal sign along the axis and the lift coefficient
     if (tid < 6 * EPB) {
@@ -407,7 +399,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         float in[6];
         if constexpr (TP == TP_CHANDRASEKHAR) {
             in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
-            in[4] = 0.5f * rho * rcp(p); in[5] = 2.0f * ir;
+            in[4] = 0.5f * rho * rcp(p); in[5] = (-1.0f / 6.0f) * ir;
         } else if constexpr (TP == TP_KENNEDY_GRUBER) {
             in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
             in[4] = 0.5f * p; in[5] = 0.5f * q[r][4] * ir;
@@ -415,40 +407,71 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
             in[0] = rho; in[1] = u; in[2] = v; in[3] = w; in[4] = p; in[5] = q[r][4];
         }
 #pragma unroll
-        for (int c = 0; c < 6; ++c) sP[c * NN + j] = in[c];
+        for (int c = 0; c < 4; ++c) sP[c * NN + j] = in[c];
+        sP[4 * NN + j] = in[1];
+        sP[5 * NN + j] = in[2];
+        sP[6 * NN + j] = in[4];
+        sP[7 * NN + j] = in[5];
+        if constexpr (TP == TP_CHANDRASEKHAR) sP[8 * NN + j] = 0.5f * p * ir;
     }
+    // boundary faces: Dirichlet or inflow states, or the own node (outflow, slip)
 #pragma unroll
-    for (int r = 0; r < FPT; ++r) {
-        const int fn = tid + r * TPB;
-        if (fn >= NFN) continue;
-        const int le = fn / (6 * NPF), rem = fn - le * 6 * NPF, k = rem / NPF, f = rem - k * NPF;
-        const int d = k >> 1;
-        float* dst = sX + (le * 6 + k) * 5 * NPF + f;
+    for (int sd = 0; sd < 2; ++sd) {
+        if (xnb[sd] == -1) {
+            const int k = 2 * ldir + sd;
+            const int bd = finfo_bd(xinf[sd]);
+            const int type = a.bctype[bd];
+            const float* src;
+            int64_t vs = 1;
+            if (type == BC_DIRICHLET) {
+                src = a.bcstate + ((int64_t)a.fid[(e0 + lle) * 6 + k] * NPF + lf) * 5;
+            } else if (type == BC_INFLOW) {
+                src = a.bcconst + bd * 5;
+            } else {
+                src = a.Q + n0 + lle * NPE + line_node<3, N>(ldir, lf, sd ? N - 1 : 0);
+                vs = Nd;
+            }
 #pragma unroll
-        for (int c = 0; c < 5; ++c) {
-            const bool flip = slip[r] && c == 1 + d;  // slip wall: reflect the normal momentum
-            dst[c * NPF] = flip ? -x[r][c] : x[r][c];
+            for (int v = 0; v < 5; ++v) xr[sd][v] = src[v * vs];
         }
+    }
+    // exterior traces in the line frame; a slip wall reflects the normal momentum
+    float xl[2][5];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const bool slip = xnb[sd] == -1 && a.bctype[finfo_bd(xinf[sd])] == BC_SLIP;
+        const float un = ldir == 0 ? xr[sd][1] : (ldir == 1 ? xr[sd][2] : xr[sd][3]);
+        xl[sd][0] = xr[sd][0];
+        xl[sd][1] = slip ? -un : un;
+        xl[sd][2] = ldir == 0 ? xr[sd][2] : (ldir == 1 ? xr[sd][3] : xr[sd][1]);
+        xl[sd][3] = ldir == 0 ? xr[sd][3] : (ldir == 1 ? xr[sd][1] : xr[sd][2]);
+        xl[sd][4] = xr[sd][4];
     }
     __syncthreads();
 
     // ---- line tasks ------------------------------------------------------------------
-    // Lanes past the block's line tasks repeat a valid line (their results are dropped),
-    // so the whole wave runs one instruction stream.
-    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
-    const int lle = tl / NL, lt = tl - (tl / NL) * NL;
-    const int ldir = lt / NPF, lf = lt - (lt / NPF) * NPF;
-    const bool line_ok = tid < EPB * NL && lle < nel;
     LineAddr L;
     L.base = lle * NPE + (ldir == 0 ? N * lf : (ldir == 1 ? lf % N + NPF * (lf / N) : lf));
     L.stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    L.vel = (1 + ldir) * NN;
     L.m[0] = 1 + ldir;
     L.m[1] = ldir == 2 ? 1 : 2 + ldir;
     L.m[2] = ldir == 0 ? 3 : ldir;
-    float acc[N][5];  // set by the first pair of each row
+    float acc[N][5];
     {
-        float P[N][6];
-        load_line<N, NN>(sP, L, P);
+        // face fluxes at the two line ends start rows 0 and N-1
+        float o0[5], o1[5];
+        load_vars<NN>(sQ + L.base, L, o0);
+        load_vars<NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
+        const float2 g0 = sF[lle * 6 + 2 * ldir], g1 = sF[lle * 6 + 2 * ldir + 1];
+        const float ehalf = TP == TP_CHANDRASEKHAR ? 0.5f : 1.0f;  // energy row accumulated halved
+        bool bad = false;
+        face_init<N, 0>(o0, xl[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        face_init<N, N - 1>(o1, xl[1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        if (bad && line_ok) err |= ERR_SQRT_NEG;
+
+        float P[N][7];
+        load_line<N, NN, TP == TP_CHANDRASEKHAR>(sP, L, P);
         if constexpr (TP == TP_CHANDRASEKHAR) {
             float amax = P[0][0], amin = P[0][0], bmax = P[0][4], bmin = P[0][4];
 #pragma unroll
@@ -459,7 +482,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
             const float fa = (amax - amin) * rcp(amax + amin);
             const float fb = (bmax - bmin) * rcp(bmax + bmin);
             const float um = fmaxf(fa * fa, fb * fb) * 1.0001f;  // margin for rcp rounding
-            if (um < 1e-4f) line_pairs<N, TP, 1>(P, acc, a.dsharp, inv_gm1);
+            if (um < 1e-4f) line_pairs<N, TP, 2>(P, acc, a.dsharp, inv_gm1);
             else if (um < 1e-2f) line_pairs<N, TP, 3>(P, acc, a.dsharp, inv_gm1);
             else line_pairs<N, TP, 0>(P, acc, a.dsharp, inv_gm1);
 #pragma unroll
@@ -467,24 +490,12 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         } else {
             line_pairs<N, TP, 0>(P, acc, a.dsharp, inv_gm1);
         }
-        // face fluxes at the two line ends and their lift
-        float o0[5], x0[5], o1[5], x1[5];
-        load_vars<NN>(sQ + L.base, L, o0);
-        load_vars<NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
-        const float* xe = sX + (lle * 6 + 2 * ldir) * 5 * NPF + lf;
-        load_vars<NPF>(xe, L, x0);
-        load_vars<NPF>(xe + 5 * NPF, L, x1);
-        const float2 g0 = sF[lle * 6 + 2 * ldir], g1 = sF[lle * 6 + 2 * ldir + 1];
-        bool bad = false;
-        face_lift<N, 0>(o0, x0, g0.x, g0.y, gm1, gamma, a.diss, acc, bad);
-        face_lift<N, N - 1>(o1, x1, g1.x, g1.y, gm1, gamma, a.diss, acc, bad);
-        if (bad && line_ok) err |= ERR_SQRT_NEG;
     }
 
     // the stage epilogue's second operand (RK register or SSP base), loaded before the
     // barrier so its latency overlaps the other waves' line tasks
-    const float* aux = (a.mode == MODE_2N && !a.first) ? a.reg
-                     : ((a.mode == MODE_SSP2 || a.mode == MODE_SSP3) ? a.ubase : nullptr);
+    const float* aux = (MODE == MODE_2N && !a.first) ? a.reg
+                     : ((MODE == MODE_SSP2 || MODE == MODE_SSP3) ? a.ubase : nullptr);
     float xv[NPT][5];
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
@@ -492,8 +503,16 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
 #pragma unroll
         for (int r = 0; r < NPT; ++r) xv[r][v] = bload(rx, (uint32_t)(tid + r * TPB) * 4u);
     }
-    __syncthreads();  // flux inputs and exterior traces are dead from here
-    if (line_ok) store_line<N, NN>(ldir == 0 ? sX : (ldir == 1 ? sP : sC), L, acc);
+    // the stage input of this thread's nodes, before its LDS region takes row sums
+    float uq[NPT][5];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int j = tid + r * TPB;
+#pragma unroll
+        for (int v = 0; v < 5; ++v) uq[r][v] = sQ[v * NN + (j < NN ? j : 0)];
+    }
+    __syncthreads();  // flux inputs and node states are dead from here
+    if (line_ok) store_line<N, NN>(ldir == 0 ? sP : (ldir == 1 ? sP + 5 * NN : sQ), L, acc);
     __syncthreads();
 
     // ---- per node: Σ_d (Ja_dd/J) row sums and the stage epilogue -----------------------
@@ -507,28 +526,29 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
         const float s0 = sG[le * 4 + 0], s1 = sG[le * 4 + 1], s2 = sG[le * 4 + 2];
 #pragma unroll
         for (int v = 0; v < 5; ++v)
-            kv[r][v] = fmaf(s0, sX[v * NN + jj], fmaf(s1, sP[v * NN + jj], s2 * sC[v * NN + jj]));
+            kv[r][v] = fmaf(s0, sP[lslot(0, v) * NN + jj],
+                            fmaf(s1, sP[(5 + lslot(1, v)) * NN + jj], s2 * sQ[lslot(2, v) * NN + jj]));
     }
 #pragma unroll
     for (int v = 0; v < 5; ++v) {
         const rsrc_t ro = make_rsrc(a.out + v * Nd + n0, blk_bytes);
-        const rsrc_t rr = make_rsrc(a.reg + v * Nd + n0, a.mode == MODE_2N ? blk_bytes : 0u);
+        const rsrc_t rr = make_rsrc(a.reg + v * Nd + n0, MODE == MODE_2N ? blk_bytes : 0u);
 #pragma unroll
         for (int r = 0; r < NPT; ++r) {
             const int j = tid + r * TPB;
             const uint32_t off = (uint32_t)j * 4u;
-            const float uin = sQ[v * NN + (j < NN ? j : 0)];
+            const float uin = uq[r][v];
             const float k = kv[r][v], xx = xv[r][v];
             float out;
-            if (a.mode == MODE_RHS) {
+            if constexpr (MODE == MODE_RHS) {
                 out = k;
-            } else if (a.mode == MODE_2N) {
+            } else if constexpr (MODE == MODE_2N) {
                 const float t = a.first ? a.dt * k : fmaf(a.A, xx, a.dt * k);
                 bstore(rr, off, t);
                 out = fmaf(a.B, t, uin);
-            } else if (a.mode == MODE_SSP1) {
+            } else if constexpr (MODE == MODE_SSP1) {
                 out = fmaf(a.dt, k, uin);
-            } else if (a.mode == MODE_SSP2) {
+            } else if constexpr (MODE == MODE_SSP2) {
                 out = 0.25f * (3.0f * xx + uin + a.dt * k);
             } else {
                 out = (1.0f / 3.0f) * (xx + 2.0f * uin + 2.0f * a.dt * k);
@@ -542,7 +562,15 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) k_fast3d(const FastArgs a) {
 template <int N, int TP>
 static hipError_t launch(const FastArgs& a, hipStream_t s) {
     const int nblocks = (a.ne + FCfg<N>::EPB - 1) / FCfg<N>::EPB;
-    hipLaunchKernelGGL((k_fast3d<N, TP>), dim3(nblocks), dim3(FCfg<N>::TPB), 0, s, a);
+    const dim3 g(nblocks), b(FCfg<N>::TPB);
+    switch (a.mode) {
+        case MODE_RHS: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_RHS>), g, b, 0, s, a); break;
+        case MODE_2N: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N>), g, b, 0, s, a); break;
+        case MODE_SSP1: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP1>), g, b, 0, s, a); break;
+        case MODE_SSP2: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP2>), g, b, 0, s, a); break;
+        case MODE_SSP3: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP3>), g, b, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -326,3 +326,49 @@ def test_non_periodic_boundaries_match_oracle():
     F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
     got, ref = disc.to_host(dQd), orc.rhs(Q)
     assert rel_err(got, ref) <= 1e-13
+
+
+@pytest.mark.parametrize("kind", ["inflow-outflow", "dirichlet", "slip-outflow"])
+def test_fp32_fast_kernel_boundaries_match_oracle(kind):
+    """Every boundary kind on the FP32 fast path (3-D, N=5, x non-periodic), against the
+    FP64 oracle and against the FP32 plain kernel on the same handle."""
+    mesh, std, equation, op, orc = build(3, 6, [3, 2, 2], "split", "ch", periodic=[0, 1, 1],
+                                         finish=[2 * np.pi] * 3)
+    qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7])
+    X, FX = orc.coords()
+    if kind == "inflow-outflow":
+        bcs = {"1": F.EulerInflowBC(qin), "2": F.EulerOutflowBC()}
+        orc.set_bc(0, O.BC_INFLOW, qin)
+        orc.set_bc(1, O.BC_OUTFLOW)
+    elif kind == "slip-outflow":
+        bcs = {"1": F.EulerSlipBC(), "2": F.EulerOutflowBC()}
+        orc.set_bc(0, O.BC_SLIP)
+        orc.set_bc(1, O.BC_OUTFLOW)
+    else:
+        def qext(x):
+            r = 1.0 + 0.1 * np.sin(x[1]) * np.cos(x[2])
+            return np.array([r, 0.2 * r, 0.05 * r, -0.03 * r, 2.5 + 0.1 * np.cos(x[0] + x[2])])
+
+        bcs = {"1": F.GenericBC(lambda _q, x, _f, _t, _e: qext(x)),
+               "2": F.GenericBC(lambda _q, x, _f, _t, _e: qext(x))}
+        bc = np.zeros((orc.nf, orc.npf, 5))
+        for f in range(orc.nf):
+            for k in range(orc.npf):
+                bc[f, k] = qext(FX[f, k])
+        orc.set_bc(0, O.BC_DIRICHLET, bc)
+        orc.set_bc(1, O.BC_DIRICHLET, bc)
+    disc = F.MultielementDisc(mesh, std, equation, op, bcs, precision="fp32")
+    assert disc.uses_fast_kernel()
+    Q = _state(disc, orc, 3, "euler", seed=21, amp=0.1)
+    ref = orc.rhs(Q)
+    Qd = disc.to_device(Q)
+    out = {}
+    for plain in (0, 1):
+        disc.set_option(1, plain)
+        dQd = disc.new_state()
+        F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+        disc.check()
+        out[plain] = disc.to_host(dQd).astype(np.float64)
+    e_fast, e_plain = rel_err(out[0], ref), rel_err(out[1], ref)
+    print(f"FP32 fast rhs with {kind} boundaries: {e_fast:.2e} (plain FP32 {e_plain:.2e})")
+    assert e_fast < 2e-5 and e_plain < 2e-5
