@@ -291,7 +291,7 @@ def main():
     launch_ms = r["gpu_ms"] / stages
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
     if main_form == "fp32" and r["fast"]:
-        kname = "k_fast3d<6, CHANDRASEKHAR> (FP32 fused rhs + CK2N54 stage)"
+        kname = "k_fast3d<6, CHANDRASEKHAR, MODE_2N> (FP32 fused rhs + CK2N54 stage)"
         traffic = measured_traffic()
     else:
         kname = f"k_plain<{'float' if main_form == 'fp32' else 'double'},3,6,EULER,SPLIT,CHANDRASEKHAR> (fused rhs + CK2N54 stage)"

@@ -372,3 +372,34 @@ def test_fp32_fast_kernel_boundaries_match_oracle(kind):
     e_fast, e_plain = rel_err(out[0], ref), rel_err(out[1], ref)
     print(f"FP32 fast rhs with {kind} boundaries: {e_fast:.2e} (plain FP32 {e_plain:.2e})")
     assert e_fast < 2e-5 and e_plain < 2e-5
+
+
+@pytest.mark.parametrize("n,nel", [(6, [3, 3, 3]), (5, [2, 2, 2]), (4, [3, 1, 2]), (6, [1, 1, 1])],
+                         ids=["n6-27el", "n5-8el", "n4-6el", "n6-1el"])
+def test_fp32_fast_kernel_partial_blocks(n, nel):
+    """Element counts that are not multiples of the elements per block (2 at N=6, 3 at N=5,
+    4 at N=4), and a single element: the last block's missing elements must neither be
+    read nor written, and the stage forms must agree with the plain FP32 kernel."""
+    mesh, std, equation, op, orc = build(3, n, nel, "split", "ch", finish=[2 * np.pi] * 3)
+    disc = F.MultielementDisc(mesh, std, equation, op, precision="fp32")
+    assert disc.uses_fast_kernel()
+    Q = _state(disc, orc, 3, "euler", seed=31, amp=0.1)
+    ref = orc.rhs(Q)
+    Qd = disc.to_device(Q)
+    dQd = disc.new_state()
+    F.rhs_(dQd, Qd, F.EquationConfig(disc, equation))
+    disc.check()
+    err = rel_err(disc.to_host(dQd).astype(np.float64), ref)
+    print(f"FP32 fast rhs n={n} nel={nel}: {err:.2e}")
+    assert err < 2e-5
+    # every stage form: fast against plain FP32 on the same handle
+    dt = 0.1 * orc.max_dt(Q, 1.0)
+    for solver in (F.SSPRK33, F.CarpenterKennedy2N54):
+        outs = []
+        for plain in (0, 1):
+            disc.set_option(1, plain)
+            Qd = disc.to_device(Q)
+            F.timeintegrate(Qd, disc, equation, solver, 2 * dt, dt=dt)
+            outs.append(disc.to_host(Qd).astype(np.float64))
+        disc.set_option(1, 0)
+        assert rel_err(outs[0], outs[1]) < 1e-5
