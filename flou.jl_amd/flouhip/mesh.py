@@ -271,19 +271,20 @@ class CartesianMesh:
         mp = self.face_pos[:, 0]
         return frames[mp], fjac[mp]
 
-    def node_coords(self, std):
+    def node_coords(self, std, elements=None):
         """Coordinates of every solution node, (ndofs, nd) (Mesh.jl:380-446), mapped when
-        the mesh carries a mapping."""
-        X = self._cartesian_node_coords(std)
+        the mesh carries a mapping; `elements` (an index range) restricts them to those
+        elements' nodes."""
+        X = self._cartesian_node_coords(std, elements)
         fn = getattr(self, "mapping", None)
         return X if fn is None else np.asarray(fn(X), dtype=np.float64)
 
-    def _cartesian_node_coords(self, std):
+    def _cartesian_node_coords(self, std, elements=None):
         nd, n = self.nd, std.n
         r = (std.basis.ξ + 1.0) / 2.0
-        ne = self.nelements
         nel = self.nelements_dir
-        eidx = np.arange(ne)
+        eidx = np.arange(self.nelements) if elements is None else np.asarray(elements)
+        ne = eidx.size
         ei = [eidx % nel[0]]
         if nd >= 2:
             ei.append((eidx // nel[0]) % nel[1])
@@ -392,12 +393,60 @@ def curved_geometry(mesh, std):
     nd, n = mesh.nd, std.n
     ne, npe = mesh.nelements, n ** nd
     D = np.asarray(std.D, dtype=np.float64)
-    X = mesh.node_coords(std).reshape(ne, npe, nd)
+    Ja = np.zeros((ne, npe, nd, nd))  # Ja[..., r, c] = (Ja^c)_r = metric[r, c]
+    J = np.zeros((ne, npe))
+    # element chunks keep the temporaries of the metric products in cache
+    chunk = max(1, (1 << 16) // npe)
+    for e0 in range(0, ne, chunk):
+        es = slice(e0, min(ne, e0 + chunk))
+        Ja[es], J[es] = _curved_chunk(mesh.node_coords(std, range(es.start, es.stop)), D, nd, n)
+    if np.any(J <= 0):
+        raise ValueError("mapped mesh has a non-positive Jacobian")
+    w = np.asarray(std.ω, dtype=np.float64)
+    evol = (J * w[None, :]).sum(axis=1)
+    metric = np.transpose(Ja, (0, 1, 3, 2)).reshape(ne * npe, nd * nd)  # column-major blocks
+    # faces, from the master element
+    npf = n ** (nd - 1)
+    nf = mesh.nfaces
+    frames = np.zeros((nf, npf, 3, nd))
+    fjac = np.zeros((nf, npf))
+    e_m, pos = mesh.face_elems[:, 0], mesh.face_pos[:, 0]
+    for p in range(2 * nd):
+        sel = np.nonzero(pos == p)[0]
+        if sel.size == 0:
+            continue
+        d, lr = p // 2, p % 2
+        nodes = np.array([_line_node(nd, n, d, f_, n - 1 if lr else 0) for f_ in range(npf)])
+        v = Ja[e_m[sel][:, None], nodes[None, :], :, d]  # Ja^d at the face nodes (nsel, npf, nd)
+        nrm = np.sqrt(np.sum(v * v, axis=2))
+        nv = (1.0 if lr else -1.0) * v / nrm[..., None]
+        fjac[sel] = nrm
+        frames[sel, :, 0] = nv
+        if nd == 2:
+            frames[sel, :, 1, 0], frames[sel, :, 1, 1] = -nv[..., 1], nv[..., 0]
+        elif nd == 3:
+            ref = np.zeros_like(nv)
+            big = np.abs(nv[..., 0]) < 0.9
+            ref[..., 0] = big
+            ref[..., 1] = ~big
+            t = ref - np.sum(ref * nv, axis=2)[..., None] * nv
+            t /= np.sqrt(np.sum(t * t, axis=2))[..., None]
+            frames[sel, :, 1] = t
+            frames[sel, :, 2] = np.cross(nv, t)
+    return (J.reshape(-1), metric, evol, frames.reshape(nf * npf, 3, nd), fjac.reshape(-1))
+
+
+def _curved_chunk(Xc, D, nd, n):
+    """Curl-form metric terms and J of a chunk of elements from their mapped nodes Xc
+    (nelem*n^nd, nd); see curved_geometry."""
+    npe = n ** nd
+    ne = Xc.shape[0] // npe
+    X = Xc.reshape(ne, npe, nd)
     # The curl form is invariant under a constant shift of X; shifting each element to its
     # first node keeps the products X_l ∂X_m small and their round-off at the metric's scale.
     x = [X[:, :, c] - X[:, :1, c] for c in range(nd)]
     dX = [[_dref(x[c], D, nd, n, k) for k in range(nd)] for c in range(nd)]  # dX[c][k] = ∂_k X_c
-    Ja = np.zeros((ne, npe, nd, nd))  # Ja[..., r, c] = (Ja^c)_r = metric[r, c]
+    Ja = np.zeros((ne, npe, nd, nd))
     if nd == 1:
         Ja[:, :, 0, 0] = 1.0
         J = dX[0][0]
@@ -412,37 +461,9 @@ def curved_geometry(mesh, std):
             Ja[:, :, nn, 0] = -(_dref(V[2], D, 3, n, 1) - _dref(V[1], D, 3, n, 2))
             Ja[:, :, nn, 1] = -(_dref(V[0], D, 3, n, 2) - _dref(V[2], D, 3, n, 0))
             Ja[:, :, nn, 2] = -(_dref(V[1], D, 3, n, 0) - _dref(V[0], D, 3, n, 1))
-        a = [np.stack([dX[c][k] for c in range(3)], -1) for k in range(3)]
-        J = np.einsum("eni,eni->en", a[0], np.cross(a[1], a[2]))
-    if np.any(J <= 0):
-        raise ValueError("mapped mesh has a non-positive Jacobian")
-    w = np.asarray(std.ω, dtype=np.float64)
-    evol = (J * w[None, :]).sum(axis=1)
-    metric = np.transpose(Ja, (0, 1, 3, 2)).reshape(ne * npe, nd * nd)  # column-major blocks
-    # faces, from the master element
-    npf = n ** (nd - 1)
-    nf = mesh.nfaces
-    frames = np.zeros((nf, npf, 3, nd))
-    fjac = np.zeros((nf, npf))
-    e_m, pos = mesh.face_elems[:, 0], mesh.face_pos[:, 0]
-    for f_ in range(npf):
-        for p in range(2 * nd):
-            sel = np.nonzero(pos == p)[0]
-            if sel.size == 0:
-                continue
-            d, lr = p // 2, p % 2
-            node = _line_node(nd, n, d, f_, n - 1 if lr else 0)
-            v = Ja[e_m[sel], node, :, d]  # Ja^d at the face node
-            nrm = np.linalg.norm(v, axis=1)
-            nv = (1.0 if lr else -1.0) * v / nrm[:, None]
-            fjac[sel, f_] = nrm
-            frames[sel, f_, 0] = nv
-            if nd == 2:
-                frames[sel, f_, 1] = np.stack([-nv[:, 1], nv[:, 0]], 1)
-            elif nd == 3:
-                ref = np.where(np.abs(nv[:, :1]) < 0.9, np.array([[1.0, 0, 0]]), np.array([[0, 1.0, 0]]))
-                t = ref - np.sum(ref * nv, 1)[:, None] * nv
-                t /= np.linalg.norm(t, axis=1)[:, None]
-                frames[sel, f_, 1] = t
-                frames[sel, f_, 2] = np.cross(nv, t)
-    return (J.reshape(-1), metric, evol, frames.reshape(nf * npf, 3, nd), fjac.reshape(-1))
+        # J = a_1·(a_2 × a_3) with a_k = (∂_k X_0, ∂_k X_1, ∂_k X_2)
+        a = [[dX[c][k] for c in range(3)] for k in range(3)]
+        J = (a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1])
+             + a[0][1] * (a[1][2] * a[2][0] - a[1][0] * a[2][2])
+             + a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]))
+    return Ja, J
