@@ -5,6 +5,10 @@ Usage: python flou.jl_amd/build_lib.py [-j JOBS]
 Every translation unit is compiled for gfx950 only.  The FP64 plain kernels and the
 reductions are built with -ffp-contract=off so that no a*b+c is fused where the reference
 rounds twice; packed FP32 instructions are disabled for all device code.
+
+Also builds libflouvtkhdf.so, the host-side VTKHDF writer, with g++ against the HDF5 C
+library found under $FLOU_HDF5_DIR (default /opt/conda); without HDF5 that library is
+skipped and only VTKHDF output is unavailable.
 """
 import argparse
 import concurrent.futures as cf
@@ -17,6 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libflouhip.so")
+VTKHDF_LIB = os.path.join(HERE, "libflouvtkhdf.so")
+HDF5_DIR = os.environ.get("FLOU_HDF5_DIR", "/opt/conda")
 
 ARCH = "gfx950"
 BASE = ["hipcc", f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
@@ -27,6 +33,7 @@ EXACT = ["-ffp-contract=off"]
 
 def units():
     u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT), ("limiter.o", "limiter.hip", EXACT),
+         ("output.o", "output.hip", []),
          ("rhs_fast3d.o", "rhs_fast3d.hip", ["-ffp-contract=fast"])]
     for prec in ("F64", "F32"):
         for nd in (1, 2, 3):
@@ -74,7 +81,29 @@ def build(jobs=None, verbose=True):
     os.replace(tmp, LIB)
     if verbose:
         print(f"  built {LIB}")
+    build_vtkhdf(verbose)
     return LIB
+
+
+def build_vtkhdf(verbose=True):
+    """libflouvtkhdf.so: g++, HDF5's C library by run path, libstdc++ linked statically so
+    that the HDF5 directory's own C++ runtime is never picked up in its place."""
+    inc, libdir = os.path.join(HDF5_DIR, "include"), os.path.join(HDF5_DIR, "lib")
+    if not os.path.isfile(os.path.join(inc, "hdf5.h")):
+        if verbose:
+            print(f"  HDF5 not found under {HDF5_DIR}: {os.path.basename(VTKHDF_LIB)} not built "
+                  "(VTKHDF output unavailable)")
+        return None
+    tmp = VTKHDF_LIB + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", f"-I{inc}", os.path.join(CSRC, "vtkhdf.cpp"),
+           f"-L{libdir}", "-lhdf5", f"-Wl,-rpath,{libdir}", "-static-libstdc++", "-static-libgcc", "-o", tmp]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("libflouvtkhdf build failed:\n" + p.stdout + p.stderr)
+    os.replace(tmp, VTKHDF_LIB)
+    if verbose:
+        print(f"  built {VTKHDF_LIB}")
+    return VTKHDF_LIB
 
 
 if __name__ == "__main__":

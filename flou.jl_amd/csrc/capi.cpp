@@ -32,6 +32,8 @@ struct flou_hip_handle {
     int lim_kind = FLOU_HIP_LIMITER_NONE;  // stage limiter applied by flou_hip_rk_stage
     double lim_min = 0.0;
     double* d_evol = nullptr;  // element volumes (limiter means)
+    double* d_eqA = nullptr;   // equispaced interpolation matrix of the last output call
+    std::vector<double> eqA;   // its host copy (source of the pending upload)
     // device tables
     void* d_ngeom = nullptr;
     void* d_ops = nullptr;
@@ -106,7 +108,7 @@ void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
                     h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
-                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol};
+                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol, h->d_eqA};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -790,6 +792,41 @@ int flou_hip_monitor(flou_hip_handle* h, const void* Q, int32_t kind, double* ou
     HIPCHECK(h, hipStreamSynchronize(h->stream));
     *out = h->h_out[0];
     return flou_hip_check(h);
+}
+
+int flou_hip_equispaced(flou_hip_handle* h, int32_t m, const double* node2eq, const void* Q, int32_t ncols,
+                        void* Qe) {
+    if (!h || !node2eq || !Q || !Qe) return FLOU_HIP_EINVAL;
+    if (m < 1) return fail(h, FLOU_HIP_EINVAL, "the number of equispaced points must be positive");
+    if (ncols < 1) return fail(h, FLOU_HIP_EINVAL, "ncols must be positive");
+    const size_t na = (size_t)m * h->n;
+    for (size_t k = 0; k < na; ++k)
+        if (!std::isfinite(node2eq[k])) return fail(h, FLOU_HIP_EINVAL, "node2eq must be finite");
+    if (equispaced_lds_bytes(h->nd, h->n, m, bytes_of(h), nullptr) > 65536)
+        return fail(h, FLOU_HIP_EUNSUPPORTED, "too many points per element for the equispaced projection");
+    if (h->eqA.size() != na || !std::equal(h->eqA.begin(), h->eqA.end(), node2eq)) {
+        // the previous upload may still be reading the host copy
+        HIPCHECK(h, hipStreamSynchronize(h->stream));
+        if (h->eqA.size() < na || !h->d_eqA) {
+            if (h->d_eqA) HIPCHECK(h, hipFree(h->d_eqA));
+            h->d_eqA = nullptr;
+            if (hipMalloc(&h->d_eqA, na * sizeof(double)) != hipSuccess) {
+                h->eqA.clear();
+                return fail(h, FLOU_HIP_ENOMEM, "device allocation failed");
+            }
+        }
+        h->eqA.assign(node2eq, node2eq + na);
+        HIPCHECK(h, hipMemcpyAsync(h->d_eqA, h->eqA.data(), na * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    }
+    hipError_t e;
+    if (h->prec == FLOU_HIP_FP64)
+        e = launch_equispaced<double>((const double*)Q, (double*)Qe, h->d_eqA, h->nd, h->n, m, h->ne, ncols,
+                                      h->stream);
+    else
+        e = launch_equispaced<float>((const float*)Q, (float*)Qe, h->d_eqA, h->nd, h->n, m, h->ne, ncols,
+                                     h->stream);
+    if (e != hipSuccess) return fail(h, FLOU_HIP_EDEVICE, hipGetErrorString(e));
+    return FLOU_HIP_OK;
 }
 
 int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double dt, double t,

@@ -31,6 +31,16 @@ hipError_t launch_monitor(const T* Q, int64_t ndofs, int wnpe, int nd, double ga
                           const double* jac, const double* w, double* partial, int max_blocks,
                           double* out, int* err, hipStream_t s);
 
+// Element-wise projection onto m^nd equispaced points (output.hip): for each of the ncols
+// columns c and elements e, Qe[(c*ne + e)*m^nd + i] = ((A ⊗ .. ⊗ A) Q_e)_i with
+// Q_e = Q[(c*ne + e)*n^nd + :] and A the m x n row-major interpolation matrix (device
+// memory).  hipErrorInvalidValue when the generic kernel's LDS footprint exceeds 64 KB.
+template <typename T>
+hipError_t launch_equispaced(const T* Q, T* Qe, const double* A, int nd, int n, int m, int64_t ne, int ncols,
+                             hipStream_t s);
+// LDS bytes of the generic kernel and its units per block
+int equispaced_lds_bytes(int nd, int n, int m, int elem_bytes, int* upb_out);
+
 // Number of partials the reductions may write (size of the partial buffer).
 int reduce_max_blocks();
 

@@ -1,7 +1,8 @@
 """flouhip -- MI355X (gfx950) DGSEM hot path of Flou.jl behind a C ABI.
 
 Host-side mirror of the Flou types a user composes to reach rhs! / timeintegrate; every
-kernel runs in libflouhip.so (flou.jl_amd/libflouhip.so, see include/flouhip.h).  Importing
+kernel runs in libflouhip.so (flou.jl_amd/libflouhip.so, see include/flouhip.h).  VTKHDF
+files are written by libflouvtkhdf.so (include/flouvtkhdf.h), loaded on first use.  Importing
 this package fails with ImportError when the library is missing: there is no CPU path.
 """
 from . import _capi
@@ -13,8 +14,10 @@ from .disc import (CarpenterKennedy2N54, EquationConfig, MonitorOutput, Multiele
 from .equations import (ChandrasekharAverage, EulerEquation, EulerInflowBC, EulerOutflowBC,
                         EulerSlipBC, GenericBC, HybridDivOperator, KennedyGruber, LinearAdvection, LxF,
                         MatrixDissipation, ScalarDissipation, SplitDivOperator, StdAverage, StrongDivOperator,
-                        normal_shockwave, nvariables, pressure, soundvelocity, vars_cons2prim,
-                        vars_prim2cons)
+                        normal_shockwave, nvariables, pressure, soundvelocity, variablenames,
+                        vars_cons2prim, vars_prim2cons)
+from .io import (FlouFile, add_celldata_, add_fielddata_, add_pointdata_, add_solution_, close_file_,
+                 get_save_callback, open_for_write, pointdata2VTKHDF)
 from .mesh import CartesianMesh, apply_periodicBCs
 from .stdregions import DGSEMrec, LagrangeBasis, StdHex, StdQuad, StdSegment
 
@@ -28,11 +31,11 @@ def version():
 __all__ = [
     "CarpenterKennedy2N54", "CartesianMesh", "ChandrasekharAverage", "DGSEMrec", "DomainError",
     "EquationConfig", "EulerEquation", "EulerInflowBC", "EulerOutflowBC", "EulerSlipBC",
-    "GenericBC", "HybridDivOperator", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
+    "FlouFile", "GenericBC", "HybridDivOperator", "KennedyGruber", "LagrangeBasis", "LinearAdvection", "LxF", "MatrixDissipation",
     "MonitorOutput", "MultielementDisc", "ORK256", "SSPRK33", "ScalarDissipation", "SplitDivOperator", "StdAverage", "StdHex",
-    "StdQuad", "StdSegment", "StrongDivOperator", "ZhangShuLimiter", "apply_periodicBCs",
-    "get_limiter", "get_limiter_callback", "get_max_dt", "get_monitor", "get_monitor_callback",
+    "StdQuad", "StdSegment", "StrongDivOperator", "ZhangShuLimiter", "add_celldata_", "add_fielddata_",
+    "add_pointdata_", "add_solution_", "apply_periodicBCs", "close_file_", "get_limiter", "get_limiter_callback", "get_max_dt", "get_monitor", "get_monitor_callback",
     "integrate", "list_limiters", "list_monitors", "make_callback_list", "normal_shockwave",
-    "nvariables", "pressure", "rhs_", "rk_stage", "soundvelocity",
-    "timeintegrate", "vars_cons2prim", "vars_prim2cons", "version",
+    "nvariables", "open_for_write", "pointdata2VTKHDF", "pressure", "rhs_", "rk_stage", "soundvelocity",
+    "timeintegrate", "variablenames", "vars_cons2prim", "vars_prim2cons", "version",
 ]

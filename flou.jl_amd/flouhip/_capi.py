@@ -80,6 +80,7 @@ SYMBOLS = {
     "flou_hip_set_limiter": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double]),
     "flou_hip_limit": (ctypes.c_int, [_vp, _vp, ctypes.c_double]),
     "flou_hip_monitor": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, _d]),
+    "flou_hip_equispaced": (ctypes.c_int, [_vp, ctypes.c_int32, _d, _vp, ctypes.c_int32, _vp]),
     "flou_hip_rhs_vjp": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "flou_hip_rk_step_record": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                                ctypes.POINTER(_vp), _vp]),

@@ -377,6 +377,10 @@ def _timeintegrate_steps(Q0, disc, equation, code, tfinal, dt, cfl, t0, cbs):
     t = ctypes.c_double(float(t0))
     tic = time.perf_counter()
     it = 0
+    for cb in cbs:  # DiscreteCallback's `initialize` (the save callback writes iteration 0)
+        init = getattr(cb, "initialize", None)
+        if init is not None:
+            init(Q0, disc, equation, t.value, it)
     while t.value < float(tfinal) * (1 - 1e-14):
         step = min(float(dt), float(tfinal) - t.value)
         C.check(C.lib.flou_hip_timeintegrate(disc.h, code, ptr, step, 1, float(cfl or 0.0),

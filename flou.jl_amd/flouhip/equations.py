@@ -22,6 +22,10 @@ class LinearAdvection:
         self.nd = len(velocity)
         self.nvariables = 1
 
+    def variablenames(self, unicode=False):
+        """FlouCommon/LinearAdvection.jl:34-40."""
+        return ("u",)
+
 
 class EulerEquation:
     """EulerEquation{ND}(γ) -- nvariables = ND + 2."""
@@ -34,9 +38,21 @@ class EulerEquation:
         self.gamma = self.γ
         self.nvariables = nd + 2
 
+    def variablenames(self, unicode=False):
+        """FlouCommon/Euler.jl:33-50."""
+        vel = ("u", "v", "w")[:self.nd]
+        if unicode:
+            return ("ρ",) + tuple("ρ" + c for c in vel) + ("ρe",)
+        return ("rho",) + tuple("rho" + c for c in vel) + ("rhoe",)
+
 
 def nvariables(eq):
     return eq.nvariables
+
+
+def variablenames(eq, unicode=False):
+    """variablenames(equation; unicode=false) (FlouCommon/Equations.jl:59-66)."""
+    return eq.variablenames(unicode=unicode)
 
 
 # ---------------------------------------------------------------- numerical fluxes

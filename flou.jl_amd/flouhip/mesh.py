@@ -275,13 +275,22 @@ class CartesianMesh:
         """Coordinates of every solution node, (ndofs, nd) (Mesh.jl:380-446), mapped when
         the mesh carries a mapping; `elements` (an index range) restricts them to those
         elements' nodes."""
-        X = self._cartesian_node_coords(std, elements)
+        X = self._cartesian_coords(std.basis.ξ, elements)
         fn = getattr(self, "mapping", None)
         return X if fn is None else np.asarray(fn(X), dtype=np.float64)
 
-    def _cartesian_node_coords(self, std, elements=None):
-        nd, n = self.nd, std.n
-        r = (std.basis.ξ + 1.0) / 2.0
+    def equispaced_coords(self, std, elements=None):
+        """Coordinates of the equispaced output points of every element, (ne*m^nd, nd):
+        phys_coords(ξ, mesh, ie) for ξ in std.ξe (src/FlouSpatial/IO.jl:41-47), mapped when
+        the mesh carries a mapping."""
+        X = self._cartesian_coords(std.ξe1, elements)
+        fn = getattr(self, "mapping", None)
+        return X if fn is None else np.asarray(fn(X), dtype=np.float64)
+
+    def _cartesian_coords(self, xi1, elements=None):
+        nd = self.nd
+        n = len(xi1)
+        r = (np.asarray(xi1, dtype=np.float64) + 1.0) / 2.0
         nel = self.nelements_dir
         eidx = np.arange(self.nelements) if elements is None else np.asarray(elements)
         ne = eidx.size

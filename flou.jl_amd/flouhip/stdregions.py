@@ -8,6 +8,8 @@ with the barycentric formulas (Flou fits monomials, ApproximationBases.jl:154-18
 adds round-off of ~1e-15..1e-12 to the same quantities); the reconstruction vectors and the
 Ds / D# combinations are then formed in double exactly as the reference forms them.
 """
+from fractions import Fraction
+
 import numpy as np
 
 LD = np.longdouble
@@ -137,10 +139,21 @@ class DGSEMrec:
         return (-b.interp_vector(-1.0) / b.ω, b.interp_vector(1.0) / b.ω)
 
 
+def _equispaced_1d(m):
+    """range(-1, 1, m) as StdSegment.jl:56 forms it: the double nearest to -1 + 2k/(m-1) for
+    each k (Julia's float ranges are exact up to the final rounding); [0.0] for one point."""
+    if m < 1:
+        raise ValueError("nequispaced must be positive")
+    if m == 1:
+        return np.zeros(1)
+    return np.array([float(Fraction(-1) + Fraction(2 * k, m - 1)) for k in range(m)])
+
+
 class _StdBase:
     ND = 0
+    VTK_TYPE = 0
 
-    def __init__(self, basis, rec, nvars=1):
+    def __init__(self, basis, rec, nvars=1, nequispaced=None):
         if rec.basis is not basis:
             raise ValueError("over-integration (different flux basis) is not supported")
         n = basis.n
@@ -165,20 +178,78 @@ class _StdBase:
         else:
             self.ω = np.array([(wx * wy) * wz for wz in w1 for wy in w1 for wx in w1])
         self.ω1 = w1
+        # equispaced output points (StdSegment.jl:55-58, StdQuad.jl:51-53, StdHex.jl:56-61):
+        # per direction ξe and node2eq = interp_matrix(ξe, basis); the ND-D points x-fastest
+        m = n if nequispaced is None else int(nequispaced)
+        self.ξe1 = _equispaced_1d(m)
+        self.node2eq1 = np.array([basis.interp_vector(x) for x in self.ξe1])
+        grids = np.meshgrid(*([self.ξe1] * self.ND), indexing="ij")
+        self.ξe = np.stack([g.ravel(order="F") for g in grids], axis=1)
 
     # Flou's D♯ and ∂g are `Dsharp` and `dg` here (♯ and ∂ are not valid in Python names)
 
     def ndofs(self):
         return self.n ** self.ND
 
+    def nequispaced(self):
+        """Number of equispaced output points of the region (StdRegions.jl:136-138)."""
+        return self.ξe1.size ** self.ND
+
+    def equisize(self):
+        return (self.ξe1.size,) * self.ND
+
+    def vtk_type(self):
+        """VTK cell type: Lagrange curve 68, quadrilateral 70, hexahedron 72."""
+        return self.VTK_TYPE
+
+    def vtk_connectivities(self):
+        """The cell's points in VTK Lagrange order (vertices, edges, faces, interior), 0-based
+        indices into the x-fastest lattice of equispaced points -- the order of
+        StdSegment.jl:171-175, StdQuad.jl:185-196 and StdHex.jl:174-199.  The reference
+        lists the n-node solution lattice while writing the equispaced points; the two agree
+        for the default nequispaced = n, and here the listing always follows the points."""
+        m = self.ξe1.size
+        if m < 2:
+            raise ValueError("a VTK Lagrange cell needs at least two points per direction")
+        return _vtk_lagrange_order(self.ND, m)
+
+
+def _vtk_lagrange_order(nd, m):
+    L = np.arange(m ** nd).reshape((m,) * nd, order="F")  # L[i, j, k] = i + m j + m² k
+    inner = slice(1, m - 1)
+
+    def F(a):  # column-major flattening (Julia's vec)
+        return np.asarray(a).ravel(order="F")
+
+    if nd == 1:
+        parts = [[L[0], L[-1]], L[inner]]
+    elif nd == 2:
+        corners = [L[0, 0], L[-1, 0], L[-1, -1], L[0, -1]]
+        edges = [L[inner, 0], L[-1, inner], L[inner, -1], L[0, inner]]
+        parts = [corners] + edges + [F(L[inner, inner])]
+    else:
+        corners = [L[0, 0, 0], L[-1, 0, 0], L[-1, -1, 0], L[0, -1, 0],
+                   L[0, 0, -1], L[-1, 0, -1], L[-1, -1, -1], L[0, -1, -1]]
+        edges = []
+        for z in (0, -1):
+            edges += [L[inner, 0, z], L[-1, inner, z], L[inner, -1, z], L[0, inner, z]]
+        edges += [L[0, 0, inner], L[-1, 0, inner], L[-1, -1, inner], L[0, -1, inner]]
+        faces = [F(L[0, inner, inner]), F(L[-1, inner, inner]), F(L[inner, 0, inner]),
+                 F(L[inner, -1, inner]), F(L[inner, inner, 0]), F(L[inner, inner, -1])]
+        parts = [corners] + edges + faces + [F(L[inner, inner, inner])]
+    return np.concatenate([np.asarray(p, dtype=np.int64).ravel() for p in parts])
+
 
 class StdSegment(_StdBase):
     ND = 1
+    VTK_TYPE = 68
 
 
 class StdQuad(_StdBase):
     ND = 2
+    VTK_TYPE = 70
 
 
 class StdHex(_StdBase):
     ND = 3
+    VTK_TYPE = 72

@@ -229,6 +229,23 @@ int flou_hip_limit(flou_hip_handle* h, void* Q, double minval);
  * p or ρ is negative somewhere (log of a negative number). */
 int flou_hip_monitor(flou_hip_handle* h, const void* Q, int32_t kind, double* out);
 
+/* The point values of a VTKHDF file: FlouBiz.pointdata2VTKHDF(Q, disc)
+ * (src/FlouSpatial/IO.jl:78-121) projects every element's nodal values onto m equispaced
+ * points per direction, Qe_e = node2eq * Q_e with node2eq = A ⊗ ... ⊗ A
+ * (StdQuad.jl:53, StdHex.jl:61; StdRegions.jl:232-238), for each column of Q.
+ *   node2eq  host, m x n row-major FP64: A[i][j] = l_j(ξe_i), the basis interpolated at
+ *            the equispaced points ξe (interp_matrix(ξe, basis), StdSegment.jl:56-58)
+ *   Q        device, (ndofs, ncols) column-major in the handle's precision: a state
+ *            (ncols = nvars) or a block of nvars*ND columns (IO.jl:102-121)
+ *   Qe       device, (ne*m^d, ncols) column-major, same precision: column c is the point
+ *            data of variable c in the order the reference appends it (element-major,
+ *            equispaced points x-fastest)
+ * Asynchronous on the handle's stream (A is copied to the device first; a call with a
+ * different A synchronises the stream).  EUNSUPPORTED when an element's points do not fit
+ * the kernel's 64 KB of LDS (max(n, m)^d > 4096 / bytes per value). */
+int flou_hip_equispaced(flou_hip_handle* h, int32_t m, const double* node2eq, const void* Q,
+                        int32_t ncols, void* Qe);
+
 /* Synchronises, returns FLOU_HIP_EDOMAIN if a kernel met a non-physical state since the
  * last call (sqrt or log of a negative number in the reference), and clears the flag. */
 int flou_hip_check(flou_hip_handle* h);

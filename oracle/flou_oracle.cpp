@@ -83,6 +83,8 @@ Ops build_ops(int nodetype, int n) {
             if (k != j) bw[j] /= (x[j] - x[k]);
     Ops o;
     o.n = n;
+    o.xl = x;
+    o.bw = bw;
     o.xi.resize(n); o.w.resize(n); o.D.resize(n * n);
     o.lm.resize(n); o.lp.resize(n); o.gm.resize(n); o.gp.resize(n);
     o.Ds.resize(n * n); o.Dsh.resize(n * n);
@@ -497,7 +499,122 @@ struct oracle_handle {
     std::string err;
 };
 
+// ------------------------------------------------------------------------------------
+// Output to equispaced points
+
+void oracle::interp_matrix(const Ops& o, int m, const double* xe, double* A) {
+    const int n = o.n;
+    for (int i = 0; i < m; ++i) {
+        const LD t = (LD)xe[i];
+        int hit = -1;
+        for (int j = 0; j < n; ++j) if (o.xl[j] == t) hit = j;
+        if (hit >= 0) {
+            for (int j = 0; j < n; ++j) A[i * n + j] = (j == hit) ? 1.0 : 0.0;
+            continue;
+        }
+        LD den = 0.0L;
+        for (int j = 0; j < n; ++j) den += o.bw[j] / (t - o.xl[j]);
+        for (int j = 0; j < n; ++j) A[i * n + j] = (double)((o.bw[j] / (t - o.xl[j])) / den);
+    }
+}
+
+void oracle::equispaced(int nd, int n, int m, const double* A, int64_t ne, int ncols, const double* Q,
+                int64_t ldq, double* Qe, int64_t lde) {
+    int npe = 1, mpe = 1;
+    for (int k = 0; k < nd; ++k) { npe *= n; mpe *= m; }
+    // node2eq, row-major mpe x npe
+    std::vector<double> K((size_t)mpe * npe);
+    for (int i = 0; i < mpe; ++i)
+        for (int j = 0; j < npe; ++j) {
+            const int i1 = i % m, j1 = j % n;
+            if (nd == 1) { K[(size_t)i * npe + j] = A[i1 * n + j1]; continue; }
+            const int i2 = (i / m) % m, j2 = (j / n) % n;
+            if (nd == 2) { K[(size_t)i * npe + j] = A[i2 * n + j2] * A[i1 * n + j1]; continue; }
+            const int i3 = i / (m * m), j3 = j / (n * n);
+            K[(size_t)i * npe + j] = (A[i3 * n + j3] * A[i2 * n + j2]) * A[i1 * n + j1];
+        }
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < ne; ++e)
+        for (int c = 0; c < ncols; ++c)
+            for (int i = 0; i < mpe; ++i) {
+                double acc = 0.0;
+                for (int j = 0; j < npe; ++j) acc += K[(size_t)i * npe + j] * Q[c * ldq + e * npe + j];
+                Qe[c * lde + e * mpe + i] = acc;
+            }
+}
+
+std::vector<int64_t> oracle::vtk_connectivities(int nd, int n) {
+    // li(i, j, k): 1-based lattice position -> 0-based linear index (LinearIndices)
+    auto li = [n](int i, int j = 1, int k = 1) -> int64_t { return (i - 1) + (int64_t)n * ((j - 1) + (int64_t)n * (k - 1)); };
+    std::vector<int64_t> c;
+    if (nd == 1) {
+        // [1, n, 2..n-1] .- 1
+        c.push_back(li(1)); c.push_back(li(n));
+        for (int i = 2; i <= n - 1; ++i) c.push_back(li(i));
+    } else if (nd == 2) {
+        // corners (1,1) (n,1) (n,n) (1,n); edges y=1, x=n, y=n, x=1; interior x fastest
+        c = {li(1, 1), li(n, 1), li(n, n), li(1, n)};
+        for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, 1));
+        for (int j = 2; j <= n - 1; ++j) c.push_back(li(n, j));
+        for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, n));
+        for (int j = 2; j <= n - 1; ++j) c.push_back(li(1, j));
+        for (int j = 2; j <= n - 1; ++j)
+            for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, j));
+    } else {
+        c = {li(1, 1, 1), li(n, 1, 1), li(n, n, 1), li(1, n, 1),
+             li(1, 1, n), li(n, 1, n), li(n, n, n), li(1, n, n)};
+        auto run_i = [&](int j, int k) { for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, j, k)); };
+        auto run_j = [&](int i, int k) { for (int j = 2; j <= n - 1; ++j) c.push_back(li(i, j, k)); };
+        auto run_k = [&](int i, int j) { for (int k = 2; k <= n - 1; ++k) c.push_back(li(i, j, k)); };
+        // edges: the four of z=1, the four of z=n, then the four along z
+        run_i(1, 1); run_j(n, 1); run_i(n, 1); run_j(1, 1);
+        run_i(1, n); run_j(n, n); run_i(n, n); run_j(1, n);
+        run_k(1, 1); run_k(n, 1); run_k(n, n); run_k(1, n);
+        // faces x=1, x=n (y fastest), y=1, y=n (x fastest), z=1, z=n (x fastest)
+        for (int xf : {1, n})
+            for (int k = 2; k <= n - 1; ++k)
+                for (int j = 2; j <= n - 1; ++j) c.push_back(li(xf, j, k));
+        for (int yf : {1, n})
+            for (int k = 2; k <= n - 1; ++k)
+                for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, yf, k));
+        for (int zf : {1, n})
+            for (int j = 2; j <= n - 1; ++j)
+                for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, j, zf));
+        for (int k = 2; k <= n - 1; ++k)
+            for (int j = 2; j <= n - 1; ++j)
+                for (int i = 2; i <= n - 1; ++i) c.push_back(li(i, j, k));
+    }
+    return c;
+}
+
 extern "C" {
+
+int oracle_interp_matrix(int nodetype, int n, int m, const double* xe, double* A) {
+    try {
+        Ops o = build_ops(nodetype, n);
+        interp_matrix(o, m, xe, A);
+        return 0;
+    } catch (...) {
+        return -1;
+    }
+}
+
+int oracle_equispaced(int nd, int n, int m, const double* A, int64_t ne, int ncols, const double* Q,
+                      double* Qe) {
+    if (nd < 1 || nd > 3 || n < 1 || m < 1 || ne < 0 || ncols < 1) return -1;
+    int64_t npe = 1, mpe = 1;
+    for (int k = 0; k < nd; ++k) { npe *= n; mpe *= m; }
+    equispaced(nd, n, m, A, ne, ncols, Q, ne * npe, Qe, ne * mpe);
+    return 0;
+}
+
+// writes n^nd ids into out; returns their count
+int64_t oracle_vtk_connectivities(int nd, int n, int64_t* out) {
+    if (nd < 1 || nd > 3 || n < 2) return -1;
+    std::vector<int64_t> c = vtk_connectivities(nd, n);
+    std::copy(c.begin(), c.end(), out);
+    return (int64_t)c.size();
+}
 
 int oracle_build_ops(int nodetype, int n, double* xi, double* w, double* D, double* Ds,
                      double* Dsh, double* lm, double* lp, double* gm, double* gp) {

@@ -111,6 +111,7 @@ enum { RK_SSPRK33 = 0, RK_CK2N54 = 1, RK_ORK256 = 2 };
 struct Ops {
     int n = 0;
     std::vector<double> xi, w, D, Ds, Dsh, lm, lp, gm, gp;
+    std::vector<long double> xl, bw;  // nodes and barycentric weights in long double
 };
 
 // Nodes, weights, Lagrange derivative and face interpolation, computed in long double
@@ -119,6 +120,24 @@ struct Ops {
 // Ds / D# combinations are then formed in double exactly as StdSegment.jl:76-89 and
 // Reconstruction.jl:125-134 do.
 Ops build_ops(int nodetype, int n);
+
+// ------------------------------------------------------------------------------------
+// Output to equispaced points (src/FlouSpatial/IO.jl:78-121)
+
+// A[i][j] = l_j(xe[i]) (m x n, row-major): interp_matrix(ξe, basis)
+// (StdRegions/ApproximationBases.jl:85-87, StdSegment.jl:56-58), barycentric in long double.
+void interp_matrix(const Ops& o, int m, const double* xe, double* A);
+
+// node2eq = A ⊗ ... ⊗ A as kron builds it (StdQuad.jl:53, StdHex.jl:61): entry
+// (i, j) = (A[i3][j3] * A[i2][j2]) * A[i1][j1] with index 1 fastest; then per element and
+// column Qe = node2eq * Q_e with j ascending (StdRegions.jl:232-238, IO.jl:86-93).
+void equispaced(int nd, int n, int m, const double* A, int64_t ne, int ncols, const double* Q,
+                int64_t ldq, double* Qe, int64_t lde);
+
+// vtk_connectivities(std): the reference's listing of a cell's points in VTK Lagrange order
+// -- corners, edges, faces, interior -- as 0-based lattice indices (x fastest) of an
+// n^nd lattice (StdSegment.jl:171-175, StdQuad.jl:185-196, StdHex.jl:174-199).
+std::vector<int64_t> vtk_connectivities(int nd, int n);
 
 // ------------------------------------------------------------------------------------
 // Discretisation: Cartesian mesh (CartesianMesh.jl) + periodic rewiring (Mesh.jl:236-322)

@@ -68,6 +68,11 @@ def lib():
         L.oracle_numflux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
         L.oracle_cons2entropy.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_entropy2cons.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
+        L.oracle_interp_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
+        L.oracle_equispaced.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int64,
+                                        ctypes.c_int, _dp, _dp]
+        L.oracle_vtk_connectivities.restype = ctypes.c_int64
+        L.oracle_vtk_connectivities.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
         L.oracle_last_error.restype = ctypes.c_char_p
         L.oracle_last_error.argtypes = [ctypes.c_void_p]
         _lib = L
@@ -254,6 +259,36 @@ def entropy2cons(nd, gamma, W):
     if lib().oracle_entropy2cons(nd, float(gamma), _p(W), _p(Q)) != 0:
         raise ArithmeticError("entropy2cons: pow of a negative number")
     return Q
+
+
+def interp_matrix(n, xe, nodes="GLL"):
+    """A[i, j] = l_j(xe[i]) of the n-node Lagrange basis (interp_matrix, ApproximationBases.jl:85-87)."""
+    xe = np.ascontiguousarray(xe, dtype=np.float64)
+    A = np.zeros((xe.size, n))
+    if lib().oracle_interp_matrix(NODES_GLL if nodes == "GLL" else NODES_GL, n, xe.size, _p(xe), _p(A)) != 0:
+        raise ValueError("oracle_interp_matrix failed")
+    return A
+
+
+def equispaced(nd, n, A, ne, Q):
+    """Element-wise Qe = node2eq * Q (StdRegions.jl:232-238) for Q of shape (ne*n^nd, ncols),
+    column-major as the state matrix; returns (ne*m^nd, ncols)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    m = A.shape[0]
+    Q = np.asfortranarray(Q, dtype=np.float64)
+    ncols = 1 if Q.ndim == 1 else Q.shape[1]
+    Qe = np.zeros((ne * m ** nd, ncols), order="F")
+    if lib().oracle_equispaced(nd, n, m, _p(A), ne, ncols, _p(Q), _p(Qe)) != 0:
+        raise ValueError("oracle_equispaced failed")
+    return Qe
+
+
+def vtk_connectivities(nd, n):
+    """vtk_connectivities(std) (StdSegment.jl:171-175, StdQuad.jl:185-196, StdHex.jl:174-199)."""
+    out = np.zeros(n ** nd, dtype=np.int64)
+    cnt = lib().oracle_vtk_connectivities(nd, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    assert cnt == n ** nd, cnt
+    return out
 
 
 def tableau_2n(scheme):

@@ -1,6 +1,7 @@
 """CPU tests of the drop-in boundary and of the host-side mirror (no GPU needed).
 
-* libflouhip.so loads and exports every function include/flouhip.h declares;
+* libflouhip.so loads and exports every function include/flouhip.h declares, and
+  libflouvtkhdf.so every function of include/flouvtkhdf.h;
 * flou_hip_create rejects invalid descriptions before touching a device;
 * the host mirror's operators, mesh connectivity, geometry and node coordinates equal the
   oracle's restatement of Flou's bit for bit (index work is exact).
@@ -34,6 +35,19 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     # and the ctypes table covers them all
     assert set(names) <= set(C.SYMBOLS), set(names) - set(C.SYMBOLS)
+
+
+def test_vtkhdf_library_exports_every_declared_symbol():
+    from flouhip import io as FIO
+    if not os.path.exists(FIO.VTKHDF_LIB_PATH):
+        pytest.skip("libflouvtkhdf.so not built (no HDF5)")
+    text = re.sub(r"/\*.*?\*/", "", open(os.path.join(ROOT, "include", "flouvtkhdf.h")).read(), flags=re.S)
+    names = sorted(set(re.findall(r"\b(flou_vtkhdf_\w+)\s*\(", text)))
+    assert len(names) == 5
+    lib = FIO._lib()
+    for name in names:
+        assert hasattr(lib, name), name
+    assert FIO.hdf5_version().startswith("1.")
 
 
 def test_version_string():

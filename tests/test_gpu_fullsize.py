@@ -9,6 +9,8 @@ C4  2-D Euler 128^2, N=5, FP64, 200 SSPRK33 steps: the checkpointed adjoint grad
     L2 density misfit against a central finite difference along a random direction.
 C5  3-D curved 32^3, N=6 (x + 0.05 sin 2πx sin 2πy sin 2πz): free stream and conservation
     on the mapped mesh, FP32 against FP64.
+Output: the equispaced projection of VTKHDF files (pointdata2VTKHDF) on the C3 mesh
+    reproduces polynomial fields of degree <= N exactly; its kernel time is printed.
 Each check prints its figure.
 """
 import math
@@ -36,11 +38,13 @@ def prim2cons(prim):
     return Q.contiguous()
 
 
-def node_xyz(disc, std, L):
-    """Node coordinates of a Cartesian [0, L]^d mesh on the device, x fastest."""
-    nd, n = disc.nd, std.n
+def node_xyz(disc, std, L, xi1=None):
+    """Node coordinates of a Cartesian [0, L]^d mesh on the device, x fastest (at the
+    reference points xi1 of every element instead of the solution nodes if given)."""
+    nd = disc.nd
+    xi = torch.tensor(std.basis.ξ if xi1 is None else xi1, dtype=torch.float64, device="cuda")
+    n = xi.numel()
     nel = disc.mesh.nelements_dir
-    xi = torch.tensor(std.basis.ξ, dtype=torch.float64, device="cuda")
     e = torch.arange(disc.nelements, device="cuda")
     a = torch.arange(n ** nd, device="cuda")
     out, estride, astride = [], 1, 1
@@ -70,6 +74,47 @@ def euler_disc(nd, n, nel, L, op, precision, mapping=None):
         mesh.apply_mapping(mapping)
     eq = F.EulerEquation(nd, GAMMA)
     return F.MultielementDisc(mesh, std, eq, op, precision=precision), eq, std
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_c3_full_size_output_projection(precision):
+    import ctypes
+
+    from flouhip import _capi as C
+    from flouhip import io as FIO
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    L = 2 * math.pi
+    disc, eq, std = euler_disc(3, 6, 64, L, op, precision)
+
+    def fields(x, y, z):  # degree <= 5 per direction in every element
+        return [1 + 0.1 * x, x - 2 * y + 0.5 * z, x * y * z / L ** 2, (x / L) ** 5 - y * z / L + 1,
+                (y / L) ** 3 * (z / L) ** 2]
+
+    Q = torch.stack(fields(*node_xyz(disc, std, L))).to(disc.torch_dtype()).contiguous()
+    want = torch.stack(fields(*node_xyz(disc, std, L, std.ξe1)))
+    out = torch.empty_like(Q)
+    disc._sync_stream()
+    A = np.ascontiguousarray(std.node2eq1)
+    args = (disc.h, A.shape[0], A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.c_void_p(Q.data_ptr()),
+            5, ctypes.c_void_p(out.data_ptr()))
+    C.check(C.lib.flou_hip_equispaced(*args), disc.h)
+    torch.cuda.synchronize()
+    reps = 10
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        C.lib.flou_hip_equispaced(*args)
+    t1.record()
+    torch.cuda.synchronize()
+    ms = t0.elapsed_time(t1) / reps
+    err = float((out.double() - want).abs().max() / want.abs().max())
+    gbs = 2 * Q.numel() * Q.element_size() / (ms * 1e-3) / 1e9
+    print(f"C3 output projection {precision}: rel err {err:.2e}; kernel {ms:.3f} ms "
+          f"({gbs:.0f} GB/s of algorithmic bytes: read Q + write Qe)")
+    assert err <= (1e-13 if precision == "fp64" else 2e-6)
+    # the Python entry point gives the same columns on the host
+    cols = FIO.pointdata2VTKHDF(Q, disc)
+    assert len(cols) == 5 and np.array_equal(cols[3], out[3].cpu().numpy())
 
 
 def test_c3_full_size_free_stream_conservation_and_fp32_vs_fp64():
