@@ -400,9 +400,6 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         if (d->l_minus[j] != (j == 0 ? 1.0 : 0.0)) h->gather = 0;
         if (d->l_plus[j] != (j == n - 1 ? 1.0 : 0.0)) h->gather = 0;
     }
-    if (h->vol == FLOU_HIP_HYBRID && !h->gather)
-        return bad(FLOU_HIP_EUNSUPPORTED,
-                   "HybridDivOperator: only nodes that include the element boundaries (GLL) are supported");
 
     // geometry
     const int NG = nd * nd + 1;
@@ -514,8 +511,12 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     }
 
     // kernels
-    // the split form on nodes without boundaries (Gauss) has entropy-projected surface terms
-    const int kvol = (h->vol == FLOU_HIP_SPLIT && !h->gather) ? (int)VOL_SPLIT_NB : h->vol;
+    // the split and hybrid forms on nodes without boundaries (Gauss) have entropy-projected
+    // surface terms
+    const int kvol = h->gather ? h->vol
+                     : h->vol == FLOU_HIP_SPLIT ? (int)VOL_SPLIT_NB
+                     : h->vol == FLOU_HIP_HYBRID ? (int)VOL_HYBRID_NB
+                                                 : h->vol;
     if (kvol == VOL_SPLIT_NB && (h->eq != FLOU_HIP_EULER || h->curved))
         return bad(FLOU_HIP_EUNSUPPORTED,
                    "the split form on Gauss nodes is implemented for the Euler equations on Cartesian elements");

@@ -9,9 +9,10 @@ namespace flouhip {
 
 enum { EQ_ADVECTION = 0, EQ_EULER = 1 };
 enum { VOL_STRONG = 0, VOL_SPLIT = 1, VOL_HYBRID = 2 };
-// Kernel-internal variant: the split form on nodes without boundaries (Gauss), whose
-// surface terms use entropy-projected end states (OpDivergence.jl:300-436).
-enum { VOL_SPLIT_NB = 3 };
+// Kernel-internal variants on nodes without boundaries (Gauss): the split form, whose
+// surface terms use entropy-projected end states (OpDivergence.jl:300-436), and the hybrid
+// form, whose subcell fluxes telescope from those terms (OpDivergence.jl:633-779).
+enum { VOL_SPLIT_NB = 3, VOL_HYBRID_NB = 4 };
 enum { TP_STDAVG = 0, TP_CHANDRASEKHAR = 1, TP_KENNEDY_GRUBER = 2 };
 enum { AVG_STD = 0, AVG_CHANDRASEKHAR = 1 };
 enum { DISS_NONE = 0, DISS_LXF = 1, DISS_MATRIX = 2, DISS_SCALAR = 3 };

@@ -22,6 +22,104 @@
 
 namespace flouhip {
 
+// HybridDivOperator on Gauss nodes (_surf_hybrid_nb_tensorproduct!, OpDivergence.jl:732-779),
+// one direction, after the two-point fluxes with the projected end states (sNf) and the row
+// sums (D♯ F♯)_i ω_i (sHs) are in LDS:
+//   (a) Fl_i -= l·Fl + Fn_l, Fr_i -= r·Fr - Fn_r, in place once every node has read the line;
+//   (b) one thread per line runs the subcell recurrence from F̄[0] = -Fn_l:
+//         F̄[p+1] = ((F̄[p] + (D♯F♯)_p ω_p) - l[p] Fl_p) + r[p] Fr_p, blended with the FV flux
+//       of positions p, p+1; F̄[p+1] replaces the consumed row sum of position p;
+//   (c) dQ[i] += (F̄[ii] - F̄[ii+1]) / ω[ii] with F̄[N] = Fn_r.
+// Every thread of the block calls it (it holds barriers); the caller synchronises after it.
+template <typename T, int ND, int N, int NV, int EPB, int TPB>
+__device__ void hybrid_nb_line_phases(const RhsArgs<T>& a, const T* sOps, T (&sQ)[EPB][NV][Cfg<ND, N>::NPE],
+                                      T (&sNf)[EPB][2][NV][Cfg<ND, N>::NPE], T (&sHs)[EPB][NV][Cfg<ND, N>::NPE],
+                                      T (&sFn)[EPB][2 * ND][NV][Cfg<ND, N>::NPF], int dir, int e0, int le, int node,
+                                      bool valid, int tid, T (&acc)[NV]) {
+    constexpr int NPF = Cfg<ND, N>::NPF;
+    const T* lm = sOps + ops_lm(N);
+    const T* lp = sOps + ops_lp(N);
+    const T* w = sOps + ops_w(N);
+    int f = 0, ii = 0;
+    T fl[NV], fr[NV];
+    if (valid) {
+        node_line<ND, N>(node, dir, f, ii);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const int n0 = line_node<ND, N>(dir, f, 0);
+            T lF = lm[0] * sNf[le][0][v][n0], rF = lp[0] * sNf[le][1][v][n0];
+            for (int k = 1; k < N; ++k) {
+                const int nk = line_node<ND, N>(dir, f, k);
+                lF = lF + lm[k] * sNf[le][0][v][nk];
+                rF = rF + lp[k] * sNf[le][1][v][nk];
+            }
+            fl[v] = sNf[le][0][v][node] - (lF + sFn[le][2 * dir][v][f]);
+            fr[v] = sNf[le][1][v][node] - (rF - sFn[le][2 * dir + 1][v][f]);
+        }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            sNf[le][0][v][node] = fl[v];
+            sNf[le][1][v][node] = fr[v];
+        }
+    }
+    __syncthreads();
+    T gs[3 * ND + 1];
+    plain::subgrid_frame<T, ND>(dir, a.sjac[dir], gs);
+    for (int it = tid; it < EPB * NPF; it += TPB) {
+        const int le2 = it / NPF, f2 = it % NPF;
+        if (e0 + le2 >= a.ne) continue;
+        T Fc[NV], Wl[NV], q0[NV];
+        const int n0 = line_node<ND, N>(dir, f2, 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            Fc[v] = -sFn[le2][2 * dir][v][f2];
+            q0[v] = sQ[le2][v][n0];
+        }
+        plain::cons2entropy<T, ND>(q0, a.gamma, a.errflag, Wl);
+        for (int p = 0; p + 1 < N; ++p) {
+            const int ni = line_node<ND, N>(dir, f2, p), nr = line_node<ND, N>(dir, f2, p + 1);
+            T Fx[NV], Ql[NV], Qr[NV], Rl[NV], Rr[NV], Fn[NV], Fv[NV], Wr[NV];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                Fx[v] = ((Fc[v] + sHs[le2][v][ni]) - lm[p] * sNf[le2][0][v][ni]) + lp[p] * sNf[le2][1][v][ni];
+                Ql[v] = sQ[le2][v][ni];
+                Qr[v] = sQ[le2][v][nr];
+            }
+            plain::rotate2face<T, ND>(Ql, gs, Rl);
+            plain::rotate2face<T, ND>(Qr, gs, Rr);
+            plain::euler_numericalflux<T, ND>(Rl, Rr, a.savg, a.sdiss, a.typo, a.intensity, a.gamma, a.errflag,
+                                              Fn);
+            plain::rotate2phys<T, ND>(Fn, gs, Fv);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) Fv[v] = Fv[v] * gs[3 * ND];
+            plain::cons2entropy<T, ND>(Qr, a.gamma, a.errflag, Wr);
+            T b = (Wr[0] - Wl[0]) * (Fx[0] - Fv[0]);
+#pragma unroll
+            for (int v = 1; v < NV; ++v) b = b + (Wr[v] - Wl[v]) * (Fx[v] - Fv[v]);
+            const T dl = plain::hybrid_delta(b, a.blend, a.errflag);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                Fx[v] = (T(1) - dl) * Fv[v] + dl * Fx[v];
+                sHs[le2][v][ni] = Fx[v];
+                Fc[v] = Fx[v];
+                Wl[v] = Wr[v];
+            }
+        }
+    }
+    __syncthreads();
+    if (valid) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const T l = ii == 0 ? -sFn[le][2 * dir][v][f] : sHs[le][v][line_node<ND, N>(dir, f, ii - 1)];
+            const T r = ii == N - 1 ? sFn[le][2 * dir + 1][v][f] : sHs[le][v][node];
+            acc[v] = acc[v] + (l - r) / w[ii];
+        }
+    }
+}
+
 template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a) {
     using C = Cfg<ND, N>;
@@ -41,11 +139,15 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     // HybridDivOperator: subcell-interface fluxes of one direction, [le][v][line][s]
     constexpr int HB = VOL == VOL_HYBRID ? 1 : 0;
     __shared__ T sFb[HB ? EPB : 1][HB ? NV : 1][HB ? NPF : 1][HB ? N : 1];
-    // Split form on Gauss nodes: projected end states (Q, primitives, β) of every line and
-    // the two-point fluxes of every node with them, one direction at a time
-    constexpr int NB = VOL == VOL_SPLIT_NB ? 1 : 0;
+    // Split and hybrid forms on Gauss nodes: projected end states (Q, primitives, β) of every
+    // line and the two-point fluxes of every node with them, one direction at a time
+    constexpr int NB = (VOL == VOL_SPLIT_NB || VOL == VOL_HYBRID_NB) ? 1 : 0;
     __shared__ T sNb[NB ? EPB : 1][NB ? NPF : 1][2][NB ? 2 * NV + 1 : 1];
     __shared__ T sNf[NB ? EPB : 1][2][NB ? NV : 1][NB ? NPE : 1];
+    // Hybrid form on Gauss nodes: per node the row sum (D♯ F♯)_i ω_i, overwritten along each
+    // line by the subcell flux of the interface to the node's right
+    constexpr int HN = VOL == VOL_HYBRID_NB ? 1 : 0;
+    __shared__ T sHs[HN ? EPB : 1][HN ? NV : 1][HN ? NPE : 1];
 
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -281,7 +383,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         M = Mnode;
     }
 #pragma unroll
-    for (int dir = 0; dir < (VOL == VOL_HYBRID ? 0 : ND) && valid; ++dir) {
+    for (int dir = 0; dir < ((VOL == VOL_HYBRID || VOL == VOL_HYBRID_NB) ? 0 : ND) && valid; ++dir) {
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         if constexpr (VOL == VOL_STRONG) {
@@ -398,9 +500,44 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
 #pragma unroll
                     for (int v = 0; v < NV; ++v) sNf[le][side][v][node] = Fs[v];
                 }
+                if constexpr (HN) {
+                    // (D♯[ii,:]·F♯[:,i]) ω[ii]: left fold over the line, F♯ diagonal = the
+                    // contravariant flux, pairs with the lower position first
+                    const T* Dsh = &sOps[ops_Dsh(N)];
+                    const T* w = &sOps[ops_w(N)];
+                    T Fd[NV], S[NV];
+                    plain::contravariant_flux<T, ND, EQ>(qi, M, dir, a.gamma, avel, Fd);
+                    for (int j = 0; j < N; ++j) {
+                        const int nj = line_node<ND, N>(dir, f, j);
+                        T F[NV];
+                        if (j == ii) {
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) F[v] = Fd[v];
+                        } else {
+                            T qj[NV], Pj[NV];
+#pragma unroll
+                            for (int v = 0; v < NV; ++v) { qj[v] = sQ[le][v][nj]; Pj[v] = sP[le][v][nj]; }
+                            const T bj = sP[le][NV][nj];
+                            if (j > ii)
+                                plain::twopointflux<T, ND, EQ, TP>(qi, qj, Pi, Pj, bi, bj, M + ND * dir, M + ND * dir,
+                                                                   a.gamma, avel, a.errflag, F);
+                            else
+                                plain::twopointflux<T, ND, EQ, TP>(qj, qi, Pj, Pi, bj, bi, M + ND * dir, M + ND * dir,
+                                                                   a.gamma, avel, a.errflag, F);
+                        }
+                        const T Dij = Dsh[ii * N + j];
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) S[v] = j == 0 ? Dij * F[v] : S[v] + Dij * F[v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sHs[le][v][node] = S[v] * w[ii];
+                }
             }
             __syncthreads();
-            if (valid) {
+            if constexpr (HN) {
+                hybrid_nb_line_phases<T, ND, N, NV, EPB, TPB>(a, sOps, sQ, sNf, sHs, sFn, dir, e0, le, node, valid,
+                                                              tid, acc);
+            } else if (valid) {
                 int f, ii;
                 node_line<ND, N>(node, dir, f, ii);
 #pragma unroll
@@ -473,7 +610,7 @@ static hipError_t launch_plain(const RhsArgs<T>& a, hipStream_t s) {
 template <typename T, int ND, int N, bool CURV>
 static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipStream_t) {
     if (eq == EQ_ADVECTION) {
-        if (vol == VOL_HYBRID || vol == VOL_SPLIT_NB) return nullptr;
+        if (vol == VOL_HYBRID || vol == VOL_SPLIT_NB || vol == VOL_HYBRID_NB) return nullptr;
         return vol == VOL_STRONG ? &launch_plain<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
                                  : &launch_plain<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
     }
@@ -484,6 +621,15 @@ static hipError_t (*pick_plain(int eq, int vol, int tp))(const RhsArgs<T>&, hipS
             case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_STDAVG, false>;
             case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_CHANDRASEKHAR, false>;
             case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_SPLIT_NB, TP_KENNEDY_GRUBER, false>;
+            default: return nullptr;
+        }
+    }
+    if (vol == VOL_HYBRID_NB) {
+        if constexpr (CURV) return nullptr;  // subcell geometry is defined for Cartesian elements
+        switch (tp) {
+            case TP_STDAVG: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID_NB, TP_STDAVG, false>;
+            case TP_CHANDRASEKHAR: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID_NB, TP_CHANDRASEKHAR, false>;
+            case TP_KENNEDY_GRUBER: return &launch_plain<T, ND, N, EQ_EULER, VOL_HYBRID_NB, TP_KENNEDY_GRUBER, false>;
             default: return nullptr;
         }
     }

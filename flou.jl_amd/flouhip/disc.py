@@ -49,9 +49,6 @@ class MultielementDisc:
             raise ValueError("mesh, standard element and equation dimensions differ")
         if len(bcs) != mesh.nboundaries():
             raise ValueError("The number of BCs does not match the number of boundaries.")
-        if operator.code == C.HYBRID and not std.basis.hasboundaries:
-            raise NotImplementedError(
-                "the hybrid form on Gauss nodes (entropy-projected subcell fluxes) is not on the GPU path")
         self.mesh, self.std, self.equation, self.operator = mesh, std, equation, operator
         self.precision = precision
         self.dtype = {"fp64": np.float64, "fp32": np.float32}[precision]

@@ -47,9 +47,10 @@ enum { FLOU_HIP_ADVECTION = 0, FLOU_HIP_EULER = 1 };
 
 /* Volume operators: StrongDivOperator, SplitDivOperator, HybridDivOperator (the
  * telescopic subcell form blending two-point and finite-volume fluxes,
- * src/FlouSpatial/Equations/OpDivergence.jl:441-620, 781-801; nodes with boundaries (GLL),
- * Cartesian elements, Euler; the FV flux is the surface flux, as in the two- and three-
- * argument constructors) */
+ * src/FlouSpatial/Equations/OpDivergence.jl:441-801; on GLL nodes in the volume term, on
+ * Gauss nodes entirely in the surface term with entropy-projected end states; Cartesian
+ * elements, Euler; the FV flux is the surface flux, as in the two- and three-argument
+ * constructors) */
 enum { FLOU_HIP_STRONG = 0, FLOU_HIP_SPLIT = 1, FLOU_HIP_HYBRID = 2 };
 
 /* Two-point volume fluxes: StdAverage, ChandrasekharAverage, Kennedy-Gruber */

@@ -635,6 +635,132 @@ void hybrid_line_fluxes(const Disc& d, const T* Qe, const T* We, long e, int dir
     }
 }
 
+// HybridDivOperator on Gauss nodes (_hybrid_nb_surface_contribution!,
+// OpDivergence.jl:633-779), added to dQ.  Per element, direction and line:
+//   F♯ table of the line (diagonal = contravariant flux, pairs with the lower position
+//   first); entropy-projected end states and Fl_i, Fr_i as in the split form, less
+//   (l·Fl + Fn_l) and (r·Fr - Fn_r);
+//   F̄[0] = -Fn_l, F̄[n] = Fn_r and, along the line,
+//     F̄[ii+1] = ((F̄[ii] + (D♯[ii,:]·F♯[:,i]) ω[ii]) - l[ii] Fl_i) + r[ii] Fr_i,
+//     blended with the FV flux of nodes ii, ii+1: F̄[ii+1] = (1-δ) F̄v + δ F̄[ii+1];
+//   dQ[i] += (F̄[ii] - F̄[ii+1]) / ω[ii].
+// `FN(face, side, f)` gives the stored numerical flux of a face side.
+template <class T, class FNF>
+void hybrid_nb_surface(const Disc& d, const T* Q, T* dQ, FNF FN) {
+    const int nd = d.nd, n = d.n, nv = d.nv, npe = d.npe, npf = d.npf;
+    const long ne = d.ne, N = d.ndofs();
+    const Ops& o = d.ops;
+    std::atomic<int> domain_failed(0);
+#pragma omp parallel for schedule(static)
+    for (long e = 0; e < ne; ++e) try {
+        const long base = e * npe;
+        std::vector<T> Qe((size_t)npe * nv), We((size_t)npe * nv), Ft((size_t)npe * nd * nv);
+        std::vector<T> Fs((size_t)n * n * nv), Fl((size_t)n * nv), Fr((size_t)n * nv), Fb((size_t)(n + 1) * nv);
+        for (int i = 0; i < npe; ++i)
+            for (int v = 0; v < nv; ++v) Qe[i * nv + v] = Q[(size_t)v * N + base + i];
+        for (int i = 0; i < npe; ++i)
+            for (int dir = 0; dir < nd; ++dir)
+                contravariant_flux(d, &Qe[i * nv], d.metric(e, i), dir, &Ft[((size_t)i * nd + dir) * nv]);
+        for (int i = 0; i < npe; ++i) cons2entropy(&Qe[i * nv], nd, d.gamma, &We[i * nv]);
+        for (int dir = 0; dir < nd; ++dir) {
+            double fr[9], sjac, nvec[3];
+            subgrid_frame(d, dir, fr, sjac);
+            for (int k = 0; k < nd; ++k) nvec[k] = fr[k] * sjac;
+            const long facel = d.e_face[e * 2 * nd + 2 * dir];
+            const int sidel = d.e_side[e * 2 * nd + 2 * dir];
+            const long facer = d.e_face[e * 2 * nd + 2 * dir + 1];
+            const int sider = d.e_side[e * 2 * nd + 2 * dir + 1];
+            for (int f = 0; f < npf; ++f) {
+                // F♯ of the line (_flux_splitdiv_tensorproduct!, OpDivergence.jl:248-271)
+                for (int a = 0; a < n; ++a) {
+                    const int na = line_node(nd, n, dir, f, a);
+                    for (int v = 0; v < nv; ++v) Fs[((size_t)a * n + a) * nv + v] = Ft[((size_t)na * nd + dir) * nv + v];
+                    for (int b = a + 1; b < n; ++b) {
+                        const int nb = line_node(nd, n, dir, f, b);
+                        T F[5];
+                        twopointflux(d, &Qe[na * nv], &Qe[nb * nv], d.metric(e, na) + nd * dir,
+                                     d.metric(e, nb) + nd * dir, F);
+                        for (int v = 0; v < nv; ++v) {
+                            Fs[((size_t)a * n + b) * nv + v] = F[v];
+                            Fs[((size_t)b * n + a) * nv + v] = F[v];
+                        }
+                    }
+                }
+                // entropy-projected fluxes (_flux_splitdiv_nb_tensorproduct!, 382-427)
+                T Wl[5], Wr[5], Ql[5], Qr[5];
+                for (int v = 0; v < nv; ++v) {
+                    const int n0 = line_node(nd, n, dir, f, 0);
+                    T sl = o.lm[0] * We[n0 * nv + v], sr = o.lp[0] * We[n0 * nv + v];
+                    for (int a = 1; a < n; ++a) {
+                        const int na = line_node(nd, n, dir, f, a);
+                        sl = sl + o.lm[a] * We[na * nv + v];
+                        sr = sr + o.lp[a] * We[na * nv + v];
+                    }
+                    Wl[v] = sl;
+                    Wr[v] = sr;
+                }
+                entropy2cons(Wl, nd, d.gamma, Ql);
+                entropy2cons(Wr, nd, d.gamma, Qr);
+                for (int a = 0; a < n; ++a) {
+                    const int na = line_node(nd, n, dir, f, a);
+                    twopointflux(d, &Qe[na * nv], Ql, d.metric(e, na) + nd * dir, nvec, &Fl[a * nv]);
+                    twopointflux(d, &Qe[na * nv], Qr, d.metric(e, na) + nd * dir, nvec, &Fr[a * nv]);
+                }
+                const T* fnl = FN(facel, sidel, f);
+                const T* fnr = FN(facer, sider, f);
+                for (int v = 0; v < nv; ++v) {
+                    T lF = o.lm[0] * Fl[v], rF = o.lp[0] * Fr[v];
+                    for (int a = 1; a < n; ++a) {
+                        lF = lF + o.lm[a] * Fl[a * nv + v];
+                        rF = rF + o.lp[a] * Fr[a * nv + v];
+                    }
+                    for (int a = 0; a < n; ++a) {
+                        Fl[a * nv + v] = Fl[a * nv + v] - (lF + fnl[v]);
+                        Fr[a * nv + v] = Fr[a * nv + v] - (rF - fnr[v]);
+                    }
+                }
+                // subcell fluxes along the line (_surf_hybrid_nb_tensorproduct!, 732-779)
+                for (int v = 0; v < nv; ++v) {
+                    Fb[v] = -fnl[v];
+                    Fb[(size_t)n * nv + v] = fnr[v];
+                }
+                for (int ii = 0; ii + 1 < n; ++ii) {
+                    const int ni = line_node(nd, n, dir, f, ii), nr = line_node(nd, n, dir, f, ii + 1);
+                    T* F = &Fb[(size_t)(ii + 1) * nv];
+                    for (int v = 0; v < nv; ++v) {
+                        T sum = o.Dsh[ii * n + 0] * Fs[((size_t)0 * n + ii) * nv + v];
+                        for (int a = 1; a < n; ++a) sum = sum + o.Dsh[ii * n + a] * Fs[((size_t)a * n + ii) * nv + v];
+                        F[v] = ((Fb[(size_t)ii * nv + v] + sum * o.w[ii]) - o.lm[ii] * Fl[ii * nv + v]) +
+                               o.lp[ii] * Fr[ii * nv + v];
+                    }
+                    T Rl[5], Rr[5], Fn[5], Fv[5];
+                    rotate2face(nd, &Qe[ni * nv], fr, Rl);
+                    rotate2face(nd, &Qe[nr * nv], fr, Rr);
+                    euler_numericalflux(d, Rl, Rr, Fn);
+                    rotate2phys(nd, Fn, fr, Fv);
+                    for (int v = 0; v < nv; ++v) Fv[v] = Fv[v] * sjac;
+                    const T* Wa = &We[ni * nv];
+                    const T* Wb = &We[nr * nv];
+                    T b = (Wb[0] - Wa[0]) * (F[0] - Fv[0]);
+                    for (int v = 1; v < nv; ++v) b = b + (Wb[v] - Wa[v]) * (F[v] - Fv[v]);
+                    const T dl = hybrid_delta(b, d.blend);
+                    for (int v = 0; v < nv; ++v) F[v] = (1.0 - dl) * Fv[v] + dl * F[v];
+                }
+                for (int ii = 0; ii < n; ++ii) {
+                    const int ni = line_node(nd, n, dir, f, ii);
+                    for (int v = 0; v < nv; ++v) {
+                        T& x = dQ[(size_t)v * N + base + ni];
+                        x = x + (Fb[(size_t)ii * nv + v] - Fb[(size_t)(ii + 1) * nv + v]) / o.w[ii];
+                    }
+                }
+            }
+        }
+    } catch (std::domain_error&) {
+        domain_failed = 1;
+    }
+    if (domain_failed) throw std::domain_error("DomainError in the hybrid surface terms");
+}
+
 // ------------------------------------------------------------------------------------
 // rhs! (Equations/Hyperbolic.jl:31-69).  Q, dQ: (ndofs, nv) column-major.
 template <class T>
@@ -706,9 +832,17 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
     // rethrow after the loop.
     std::atomic<int> domain_failed(0);
 
+    // Nodes without boundaries (Gauss): l and r are not unit vectors
+    bool gauss = false;
+    for (int a = 0; a < n; ++a)
+        if (o.lm[a] != (a == 0 ? 1.0 : 0.0) || o.lp[a] != (a == n - 1 ? 1.0 : 0.0)) gauss = true;
+    // HybridDivOperator on Gauss nodes has no volume term: all of it is in the surface
+    // contribution (OpDivergence.jl:474-491, 614-730)
+    const bool hybrid_nb = d.volop == VOL_HYBRID && gauss;
+
     // volume_contribution! (OpDivergence.jl:109-160, 200-282)
 #pragma omp parallel for schedule(static)
-    for (long e = 0; e < ne; ++e) try {
+    for (long e = 0; e < (hybrid_nb ? 0 : ne); ++e) try {
         const long base = e * npe;
         std::vector<T> Qe((size_t)npe * nv), Ft((size_t)npe * nd * nv), acc((size_t)npe * nv);
         for (int i = 0; i < npe; ++i)
@@ -830,10 +964,9 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
 
     // Split form on nodes without boundaries (Gauss): the entropy-projected surface terms
     // (_splitdiv_nb_surface_contribution!, OpDivergence.jl:300-436) replace the FR lift.
-    bool gauss = false;
-    for (int a = 0; a < n; ++a)
-        if (o.lm[a] != (a == 0 ? 1.0 : 0.0) || o.lp[a] != (a == n - 1 ? 1.0 : 0.0)) gauss = true;
-    if (d.volop == VOL_SPLIT && gauss && d.eq == EQ_EULER) {
+    if (hybrid_nb) {
+        hybrid_nb_surface(d, Q, dQ, FN);
+    } else if (d.volop == VOL_SPLIT && gauss && d.eq == EQ_EULER) {
 #pragma omp parallel for schedule(static)
         for (long e = 0; e < ne; ++e) try {
             const long base = e * npe;

@@ -114,9 +114,8 @@ def test_create_rejects_invalid_descriptions(field, value, code):
     (lambda d, keep: setattr(d, "blend", -1.0), C.EINVAL, b"blending"),
     (lambda d, keep: setattr(d, "blend", float("inf")), C.EINVAL, b"blending"),
     (lambda d, keep: setattr(d, "geometry", C.GEOM_CURVED), C.EUNSUPPORTED, b"Cartesian"),
-    (lambda d, keep: keep[3].__setitem__(slice(None), [0.5, 0.5, 0.0]), C.EUNSUPPORTED, b"GLL"),
     (lambda d, keep: keep[-5].__setitem__(0, 2.0 * keep[-5][0]), C.EUNSUPPORTED, b"share one geometry"),
-], ids=["negative-blend", "inf-blend", "curved", "gauss-traces", "non-uniform"])
+], ids=["negative-blend", "inf-blend", "curved", "non-uniform"])
 def test_create_rejects_unsupported_hybrid(setup, code, text):
     keep = []
     d = _valid_desc(keep)

@@ -9,6 +9,13 @@ Pins:
 * δ ∈ [1/2, 2]: the hybrid rhs differs from the split one when c is small;
 * conservation Σ Jω dQ = 0 on a periodic mesh for any δ (the interface fluxes telescope),
   and free-stream preservation.
+On Gauss nodes (_hybrid_nb_surface_contribution!, OpDivergence.jl:633-779) the subcell
+fluxes telescope through the blended values along each line, starting from the face flux.
+With c -> ∞ the telescoped sum returns exactly the other face flux (W D♯ is skew on Gauss
+nodes and the projected corrections sum to the face fluxes), so the operator must equal
+the Gauss split form (_splitdiv_nb_surface_contribution!).  A wrong row of D♯, a dropped
+ω, a sign on l·Fl / r·Fr or a missing face term breaks that identity; conservation and free
+stream are checked for small c too.
 """
 import numpy as np
 import pytest
@@ -109,6 +116,53 @@ def test_hybrid_conservation_and_free_stream(case):
     for v in range(nd + 2):
         tot = np.sum(wq * dq[:, v])
         assert abs(tot) <= 1e-13 * np.sum(np.abs(wq * dq[:, v]))
+    Q0 = np.zeros_like(Q)
+    Q0[:, 0], Q0[:, 1:nd + 1], Q0[:, nd + 1] = 1.2, [0.3, -0.2, 0.1][:nd], 2.5
+    assert np.max(np.abs(h.rhs(np.asfortranarray(Q0)))) <= 1e-12
+
+
+def _pair_gauss(nd, n, nel, tp, savg, sdiss, blend):
+    args = (nd, n, [nel] * nd, [0.0] * nd, [1.0, 1.5, 2.0][:nd], [1] * nd, O.EQ_EULER, [GAMMA])
+    h = oracle_for(*args, O.VOL_HYBRID, tp, savg, sdiss, 1.0, nodes="GL")
+    h.set_blend(blend)
+    s = oracle_for(*args, O.VOL_SPLIT, tp, savg, sdiss, 1.0, nodes="GL")
+    return h, s
+
+
+GAUSS_CASES = [(1, 4, 6, O.TP_CHANDRASEKHAR, O.AVG_CHANDRASEKHAR, O.DISS_MATRIX),
+               (1, 6, 4, O.TP_KENNEDY_GRUBER, O.AVG_STD, O.DISS_LXF),
+               (2, 4, 3, O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_SCALAR),
+               (2, 5, 2, O.TP_STDAVG, O.AVG_STD, O.DISS_LXF),
+               (3, 3, 2, O.TP_CHANDRASEKHAR, O.AVG_CHANDRASEKHAR, O.DISS_MATRIX),
+               (3, 4, 2, O.TP_KENNEDY_GRUBER, O.AVG_STD, O.DISS_LXF)]
+
+
+@pytest.mark.parametrize("case", GAUSS_CASES, ids=lambda c: f"{c[0]}d-n{c[1]}-tp{c[3]}-d{c[5]}")
+def test_gauss_large_blend_is_the_gauss_split_form(case):
+    nd, n, nel, tp, savg, sdiss = case
+    h, s = _pair_gauss(nd, n, nel, tp, savg, sdiss, 1e300)
+    X, _ = h.coords()
+    Q = random_euler_state(X, nd, seed=3, amp=0.2)
+    a, b = h.rhs(Q), s.rhs(Q)
+    err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+    print(f"Gauss {case}: hybrid(c=1e300) vs Gauss split {err:.2e}")
+    assert err <= 1e-13
+
+
+@pytest.mark.parametrize("case", GAUSS_CASES, ids=lambda c: f"{c[0]}d-n{c[1]}-tp{c[3]}-d{c[5]}")
+def test_gauss_hybrid_conservation_and_free_stream(case):
+    nd, n, nel, tp, savg, sdiss = case
+    h, s = _pair_gauss(nd, n, nel, tp, savg, sdiss, 1e-2)
+    X, _ = h.coords()
+    Q = random_euler_state(X, nd, seed=4, amp=0.2)
+    dq = h.rhs(Q)
+    assert np.max(np.abs(dq - s.rhs(Q))) > 1e-5 * np.max(np.abs(dq))  # the blend acts
+    w = np.ones(1)
+    for _ in range(nd):
+        w = np.kron(O.build_ops(n, "GL")["w"], w)
+    wq = np.repeat(h.geometry()["e_jac"], n ** nd) * np.tile(w, h.ne)
+    for v in range(nd + 2):
+        assert abs(np.sum(wq * dq[:, v])) <= 1e-13 * np.sum(np.abs(wq * dq[:, v]))
     Q0 = np.zeros_like(Q)
     Q0[:, 0], Q0[:, 1:nd + 1], Q0[:, nd + 1] = 1.2, [0.3, -0.2, 0.1][:nd], 2.5
     assert np.max(np.abs(h.rhs(np.asfortranarray(Q0)))) <= 1e-12
