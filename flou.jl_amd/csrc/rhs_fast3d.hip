@@ -79,8 +79,9 @@ struct FCfg {
 //   Kennedy-Gruber: ρ/2, u/2, v/2, w/2 | p/2, e/2
 //   StdAverage:     ρ,   u,   v,   w   | p,   ρe
 // In a line task value 1 is the velocity along the line, values 2, 3 the other two
-// (cyclic order: LDS slots 1+d, 2+d, 3+d for direction d).  Chandrasekhar returns the flux factored as (m, F_1, u_t1, u_t2, h/2):
-// F = (m, F_1, m·u_t1, m·u_t2, m·h/2) with the energy component halved.
+// (cyclic order: LDS slots 1+d, 2+d, 3+d for direction d).  Chandrasekhar returns the
+// flux factored as (m, F_1, u_t1, u_t2, h/2): F = (m, F_1, m·u_t1, m·u_t2, m·h/2) with the
+// energy component halved (line_pairs uses the node velocities in place of u_t1, u_t2).
 template <int TP, int PATH>
 __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float inv_gm1, float* F) {
     if constexpr (TP == TP_CHANDRASEKHAR) {
@@ -169,10 +170,13 @@ __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float
 
 // Rows i and j of the line accumulate -D♯_ij F and -D♯_ji F.  Rows 0 and N-1 already
 // hold the lifted face fluxes; the pairs (0, j) carry the first contribution to every
-// other row, so they set those accumulators.
+// other row, so they set those accumulators.  Chandrasekhar: the transverse momentum
+// components m (u_i + u_j)/2 are split, -Σ_j D♯_ij m_ij u_i = u_i × (the row's mass sum),
+// which saves the two velocity sums of every pair.
 template <int N, int TP, int PATH>
 __device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[N][5],
                                            const float* __restrict__ Dsh, float inv_gm1) {
+    const float f0 = acc[0][0], f1 = acc[N - 1][0];  // face terms preset in rows 0, N-1
 #pragma unroll
     for (int i = 0; i < N; ++i) {
 #pragma unroll
@@ -187,11 +191,15 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[
                 acc[i][1] = fmaf(-cij, F[1], acc[i][1]);
                 acc[j][0] = setj ? -mj : acc[j][0] - mj;
                 acc[j][1] = setj ? -cji * F[1] : fmaf(-cji, F[1], acc[j][1]);
+                // transverse momenta: -m (u_i + u_j)/2 per row; only the partner's half is
+                // accumulated here, the own half is u_i times the row's mass sum (below)
 #pragma unroll
-                for (int v = 2; v < 5; ++v) {
-                    acc[i][v] = fmaf(-mi, F[v], acc[i][v]);
-                    acc[j][v] = setj ? -mj * F[v] : fmaf(-mj, F[v], acc[j][v]);
+                for (int v = 2; v < 4; ++v) {
+                    acc[i][v] = fmaf(-mi, P[j][v], acc[i][v]);
+                    acc[j][v] = setj ? -mj * P[i][v] : fmaf(-mj, P[i][v], acc[j][v]);
                 }
+                acc[i][4] = fmaf(-mi, F[4], acc[i][4]);
+                acc[j][4] = setj ? -mj * F[4] : fmaf(-mj, F[4], acc[j][4]);
             } else {
 #pragma unroll
                 for (int v = 0; v < 5; ++v) {
@@ -199,6 +207,14 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[
                     acc[j][v] = setj ? -cji * F[v] : fmaf(-cji, F[v], acc[j][v]);
                 }
             }
+        }
+    }
+    if constexpr (TP == TP_CHANDRASEKHAR) {
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const float m = p == 0 ? acc[p][0] - f0 : (p == N - 1 ? acc[p][0] - f1 : acc[p][0]);
+            acc[p][2] = fmaf(P[p][2], m, acc[p][2]);
+            acc[p][3] = fmaf(P[p][3], m, acc[p][3]);
         }
     }
 }
