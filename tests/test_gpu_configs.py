@@ -15,7 +15,7 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import GAMMA
+from cases import GAMMA, c2_state
 
 pytestmark = pytest.mark.gpu
 
@@ -47,20 +47,6 @@ def test_c1_advection_full_run():
     assert err <= 1e-13
 
 
-def _c2_state(X, seed=2):
-    rng = np.random.default_rng(seed)
-    x, y = X[:, 0], X[:, 1]
-    rho = 1 + 0.2 * np.sin(np.pi * (x + y))
-    for _ in range(8):
-        kx, ky = rng.integers(1, 4, 2)
-        rho = rho + 0.01 / 8 * np.sin(np.pi * (kx * x + ky * y) + rng.uniform(0, 2 * np.pi))
-    u = 0.5 + 0.05 * np.sin(np.pi * x + rng.uniform(0, 2 * np.pi)) * np.cos(np.pi * y)
-    v = 0.5 + 0.05 * np.cos(np.pi * x) * np.sin(np.pi * y + rng.uniform(0, 2 * np.pi))
-    p = np.ones_like(x)
-    Q = np.stack([rho, rho * u, rho * v, p / (GAMMA - 1) + rho * (u * u + v * v) / 2], 1)
-    return np.asfortranarray(Q)
-
-
 def test_c2_kennedy_gruber_fp64_and_fp32():
     nel, n, nsteps = 32, 8, 10
     b = F.LagrangeBasis("GLL", n)
@@ -70,7 +56,7 @@ def test_c2_kennedy_gruber_fp64_and_fp32():
     op = F.SplitDivOperator(F.KennedyGruber(), F.LxF(F.StdAverage(), 1.0))
     orc = O.OracleDisc(2, n, [nel, nel], [0.0, 0.0], [2.0, 2.0], [1, 1], O.build_ops(n, "GLL"),
                        O.EQ_EULER, [GAMMA], O.VOL_SPLIT, O.TP_KENNEDY_GRUBER, O.AVG_STD, O.DISS_LXF, 1.0)
-    Q0 = _c2_state(mesh.node_coords(std))
+    Q0 = c2_state(mesh.node_coords(std))
     dt = orc.max_dt(Q0, 0.1)
     ref = orc.rk_run(O.RK_SSPRK33, Q0, dt, nsteps)
     out = {}

@@ -10,7 +10,7 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import GAMMA, periodic_euler_state
+from cases import GAMMA, periodic_euler_state, warp
 from flouhip import adjoint as A
 from flouhip import mesh as M
 
@@ -18,14 +18,6 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip("torch")
 
-AMP = 0.05
-
-
-def warp(X):
-    s = np.ones((X.shape[0], 1))
-    for k in range(X.shape[1]):
-        s = s * np.sin(2 * np.pi * X[:, [k]])
-    return X + AMP * s
 
 
 TP = {"ch": (F.ChandrasekharAverage, O.TP_CHANDRASEKHAR), "kg": (F.KennedyGruber, O.TP_KENNEDY_GRUBER),

@@ -12,7 +12,7 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import GAMMA, random_euler_state
+from cases import GAMMA, isapprox, random_euler_state, reference_answer
 
 pytestmark = pytest.mark.gpu
 
@@ -124,10 +124,11 @@ def test_shockwave2d_on_gpu_matches_reference_golden():
     Qd = disc.to_device(Q)
     F.timeintegrate(Qd, disc, eq, F.ORK256(williamson_condition=False), 1.0, dt=1e-2)
     u = disc.to_host(Qd)
-    dev = abs(u.max() - 12.977466260673845) / 12.977466260673845
+    ref = reference_answer("shockwave_2d")
+    dev = abs(u.max() - ref["max"]) / ref["max"]
     print(f"Shockwave2D GPU: min {u.min():.6e} max {u.max():.15f} (rel dev {dev:.2e})")
-    assert u.max() == pytest.approx(12.977466260673845, rel=1e-7)
-    assert abs(u.min() - -4.680030848090326e-13) <= max(abs(u.min()), 4.680030848090326e-13)
+    assert isapprox(u.max(), ref["max"], ref["max_rtol"])
+    assert isapprox(u.min(), ref["min"], ref["min_rtol"])
     assert dev < 1e-10
 
 

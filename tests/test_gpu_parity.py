@@ -9,7 +9,7 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import GAMMA, oracle_for, random_euler_state
+from cases import GAMMA, isapprox, oracle_for, random_euler_state, reference_answer
 
 pytestmark = pytest.mark.gpu
 
@@ -207,10 +207,10 @@ def test_sod_tube_on_gpu_matches_reference_golden():
     Qd = disc.to_device(Q)
     F.timeintegrate(Qd, disc, eq, F.ORK256(williamson_condition=False), 0.018, dt=1e-4)
     u = disc.to_host(Qd)
-    assert u.min() == pytest.approx(-0.1662939230897596, rel=1e-7)
-    assert u.max() == pytest.approx(254.90504152149907, rel=1e-7)
-    print("Sod GPU min/max rel dev:", abs(u.min() + 0.1662939230897596) / 0.1662939230897596,
-          abs(u.max() - 254.90504152149907) / 254.90504152149907)
+    ref = reference_answer("sod_tube_1d")
+    assert isapprox(u.min(), ref["min"], ref["rtol"]) and isapprox(u.max(), ref["max"], ref["rtol"])
+    print("Sod GPU min/max rel dev:", abs(u.min() - ref["min"]) / abs(ref["min"]),
+          abs(u.max() - ref["max"]) / abs(ref["max"]))
 
 
 def test_advection_2d_full_period_gauss_nodes():
@@ -225,7 +225,8 @@ def test_advection_2d_full_period_gauss_nodes():
     Qd = disc.to_device(Q0)
     F.timeintegrate(Qd, disc, eq, F.ORK256(), 0.5, dt=1e-3)
     u = disc.to_host(Qd)
-    assert np.linalg.norm(u - Q0) <= 1e-3 * max(np.linalg.norm(u), np.linalg.norm(Q0))
+    rtol = reference_answer("advection_2d")["rtol"]
+    assert np.linalg.norm(u - Q0) <= rtol * max(np.linalg.norm(u), np.linalg.norm(Q0))
 
 
 FP32_CASES = [

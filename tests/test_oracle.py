@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from cases import GAMMA, oracle_for, random_euler_state
+from cases import GAMMA, isapprox, oracle_for, random_euler_state, reference_answer
 
 
 # ------------------------------------------------------------------ reference known answers
@@ -37,13 +37,13 @@ def _sod():
 
 def test_sod_tube_golden():
     """SodTube1D (test/tests.jl:90-134) -> test/runtests.jl:37-38 at rtol 1e-7."""
+    ref = reference_answer("sod_tube_1d")
     d, Q = _sod()
     u = d.rk_run(O.RK_ORK256, Q, 1e-4, 180)
-    assert u.min() == pytest.approx(-0.1662939230897596, rel=1e-7)
-    assert u.max() == pytest.approx(254.90504152149907, rel=1e-7)
+    assert isapprox(u.min(), ref["min"], ref["rtol"]) and isapprox(u.max(), ref["max"], ref["rtol"])
     # far tighter than the reference asks: only round-off separates the two codes
-    assert abs(u.min() + 0.1662939230897596) / 0.1662939230897596 < 1e-12
-    assert abs(u.max() - 254.90504152149907) / 254.90504152149907 < 1e-13
+    assert abs(u.min() - ref["min"]) / abs(ref["min"]) < 1e-12
+    assert abs(u.max() - ref["max"]) / abs(ref["max"]) < 1e-13
 
 
 def _bump(X, x0, s):
@@ -57,7 +57,8 @@ def test_advection_1d_full_period():
     X, _ = d.coords()
     Q = np.asfortranarray(_bump(X, [0.5], [0.1])[:, None])
     u = d.rk_run(O.RK_ORK256, Q, 1e-3, 500)
-    assert np.linalg.norm(u - Q) <= 1e-3 * max(np.linalg.norm(u), np.linalg.norm(Q))
+    rtol = reference_answer("advection_1d")["rtol"]
+    assert np.linalg.norm(u - Q) <= rtol * max(np.linalg.norm(u), np.linalg.norm(Q))
 
 
 def test_advection_2d_full_period():
@@ -67,7 +68,8 @@ def test_advection_2d_full_period():
     X, _ = d.coords()
     Q = np.asfortranarray(_bump(X, [0.75, 1.0], [0.2, 0.2])[:, None])
     u = d.rk_run(O.RK_ORK256, Q, 1e-3, 500)
-    assert np.linalg.norm(u - Q) <= 1e-3 * max(np.linalg.norm(u), np.linalg.norm(Q))
+    rtol = reference_answer("advection_2d")["rtol"]
+    assert np.linalg.norm(u - Q) <= rtol * max(np.linalg.norm(u), np.linalg.norm(Q))
 
 
 # ------------------------------------------------------------------ operators
