@@ -238,8 +238,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64"], default=None,
-                    help="time one form only (profiling runs)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint"], default=None,
+                    help="time one form only, or only the adjoint/forward measurement (profiling runs)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain copies of 1 GiB (FP32), the reference kernel for "
@@ -253,6 +253,9 @@ def main():
     import flouhip as F
 
     torch.cuda.set_device(0)
+    if args.only == "adjoint":  # profiling run of the adjoint workload alone
+        print(json.dumps({"adjoint": adjoint_ratio()}), flush=True)
+        return
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)

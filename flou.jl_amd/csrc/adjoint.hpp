@@ -4,6 +4,8 @@
 // of physics_plain.hpp (same branches: the log-mean series below u = 0.01, the side that
 // wins the LxF max), accumulated into the caller's cotangent (+=).  Derivations work on
 // primitives (ρ, u, p) and are pulled back to conservative variables by prim_pullback.
+// No reference fixes the rounding of these products, so the hot ones (pair fluxes, face
+// fluxes) multiply by a few shared reciprocals instead of dividing.
 // The functions are __host__ __device__ so that they can be unit-tested on the CPU.
 #pragma once
 
@@ -36,18 +38,34 @@ FLOU_HD inline void prim_pullback(const T* Q, T gamma, T rb, const T* ub, T pb, 
     Qb[ND + 1] += g1 * pb;
 }
 
+// prim_pullback with the reciprocal 1/ρ given
+template <typename T, int ND>
+FLOU_HD inline void prim_pullback_ir(const T* Q, T ir, T gamma, T rb, const T* ub, T pb, T* Qb) {
+    T uu = T(0), ubu = T(0);
+    T u[ND];
+    for (int k = 0; k < ND; ++k) {
+        u[k] = Q[1 + k] * ir;
+        uu += u[k] * u[k];
+        ubu += ub[k] * u[k];
+    }
+    const T g1 = gamma - T(1);
+    Qb[0] += rb - ubu * ir + g1 * pb * uu * T(0.5);
+    for (int k = 0; k < ND; ++k) Qb[1 + k] += ub[k] * ir - g1 * pb * u[k];
+    Qb[ND + 1] += g1 * pb;
+}
+
 // Physical Euler flux along a vector n, f_n(Q) = (m·n, (m·n)u + p n, (E+p) u·n)
 // (FlouCommon/Euler.jl:54-114 contracted as in PhysicalRegions.jl:1001-1033):
 // Qb += (∂f_n/∂Q)^T gb.
 template <typename T, int ND>
-FLOU_HD inline void flux_vjp(const T* Q, const T* n, T gamma, const T* gb, T* Qb) {
-    const T rho = Q[0], ir = T(1) / rho;
+FLOU_HD inline void flux_vjp_ir(const T* Q, T ir, const T* n, T gamma, const T* gb, T* Qb) {
+    const T rho = Q[0];
     T u[ND], m2 = T(0);
     for (int k = 0; k < ND; ++k) {
         u[k] = Q[1 + k] * ir;
-        m2 += Q[1 + k] * Q[1 + k];
+        m2 += Q[1 + k] * u[k];
     }
-    const T p = (gamma - T(1)) * (Q[ND + 1] - m2 * ir * T(0.5));
+    const T p = (gamma - T(1)) * (Q[ND + 1] - m2 * T(0.5));
     T un = T(0), gmn = T(0), gmu = T(0);
     for (int k = 0; k < ND; ++k) {
         un += u[k] * n[k];
@@ -61,7 +79,12 @@ FLOU_HD inline void flux_vjp(const T* Q, const T* n, T gamma, const T* gb, T* Qb
     for (int k = 0; k < ND; ++k) ub[k] = rho * un * gb[1 + k] + unb * n[k];
     const T pb = gmn + gb[ND + 1] * un;
     Qb[ND + 1] += gb[ND + 1] * un;
-    prim_pullback<T, ND>(Q, gamma, un * A, ub, pb, Qb);
+    prim_pullback_ir<T, ND>(Q, ir, gamma, un * A, ub, pb, Qb);
+}
+
+template <typename T, int ND>
+FLOU_HD inline void flux_vjp(const T* Q, const T* n, T gamma, const T* gb, T* Qb) {
+    flux_vjp_ir<T, ND>(Q, T(1) / Q[0], n, gamma, gb, Qb);
 }
 
 // logarithmic_mean(a, b) (FlouCommon/Utilities.jl:34-44) and ∂/∂a, same branch as the
@@ -167,6 +190,123 @@ FLOU_HD inline void twopoint_vjp1(const T* Q1, const T* Q2, const T* n, T gamma,
     }
 }
 
+// logarithmic_mean(a, b) with both partial derivatives from one evaluation (the branch of
+// the forward): ∂L/∂a = 1/(2F) - 2 f b F'/(F² s), ∂L/∂b = 1/(2F) + 2 f a F'/(F² s).  The
+// derivative code has no reference operation order to keep, so it multiplies by
+// reciprocals (two divisions on the series branch).
+template <typename T>
+FLOU_HD inline void logmean_dab(T a, T b, T* L, T* da, T* db) {
+    const T s = a + b, is = T(1) / s;
+    const T f = (a - b) * is;
+    const T u = f * f;
+    T F, dF;
+    if (u < T(0.01)) {
+        F = T(1) + u * (T(1) / T(3) + u * (T(1) / T(5) + u * (T(1) / T(7))));
+        dF = T(1) / T(3) + u * (T(2) / T(5) + u * (T(3) / T(7)));
+    } else {
+        F = log(a / b) / (T(2) * f);
+        dF = (T(1) / (T(1) - u) - F) / (T(2) * u);
+    }
+    const T iF = T(1) / F, h = T(0.5) * iF;
+    *L = s * h;
+    const T g = T(2) * f * dF * iF * iF * is;
+    *da = h - g * b;
+    *db = h + g * a;
+}
+
+// Both sides of the two-point flux VJP at once: Qb1 += (∂F♯/∂Q1)^T Gb and
+// Qb2 += (∂F♯/∂Q2)^T Gb, sharing the symmetric intermediates (means, log-means, p̂, ĥ, the
+// mass flux and their cotangents) and multiplying by reuses of a few reciprocals; equal to
+// twopoint_vjp1(Q1, Q2) and twopoint_vjp1(Q2, Q1) up to rounding.
+template <typename T, int ND, int TP>
+FLOU_HD inline void twopoint_vjp2(const T* Q1, const T* Q2, const T* n, T gamma, const T* Gb, T* Qb1,
+                                  T* Qb2) {
+    if constexpr (TP == TP_STDAVG) {
+        T h[ND + 2];
+        for (int v = 0; v < ND + 2; ++v) h[v] = T(0.5) * Gb[v];
+        flux_vjp<T, ND>(Q1, n, gamma, h, Qb1);
+        flux_vjp<T, ND>(Q2, n, gamma, h, Qb2);
+        return;
+    } else {
+        const T g1 = gamma - T(1);
+        const T ir1 = T(1) / Q1[0], ir2 = T(1) / Q2[0];
+        T u1[ND], u2[ND], um[ND];
+        T q1 = T(0), q2 = T(0);
+        for (int k = 0; k < ND; ++k) {
+            u1[k] = Q1[1 + k] * ir1;
+            u2[k] = Q2[1 + k] * ir2;
+            um[k] = (u1[k] + u2[k]) * T(0.5);
+            q1 += Q1[1 + k] * u1[k];
+            q2 += Q2[1 + k] * u2[k];
+        }
+        const T p1 = g1 * (Q1[ND + 1] - T(0.5) * q1);
+        const T p2 = g1 * (Q2[ND + 1] - T(0.5) * q2);
+        T umn = T(0), Gmn = T(0), Gmu = T(0);
+        for (int k = 0; k < ND; ++k) {
+            umn += um[k] * n[k];
+            Gmn += Gb[1 + k] * n[k];
+            Gmu += Gb[1 + k] * um[k];
+        }
+        T ub1[ND], ub2[ND];
+        T rb1, rb2, pb1, pb2;
+        if constexpr (TP == TP_KENNEDY_GRUBER) {
+            const T rho = (Q1[0] + Q2[0]) * T(0.5);
+            const T p = (p1 + p2) * T(0.5);
+            const T e = (Q1[ND + 1] * ir1 + Q2[ND + 1] * ir2) * T(0.5);
+            const T mn = rho * umn;
+            const T mnb = Gb[0] + Gmu + Gb[ND + 1] * e;
+            const T pbar = Gmn + Gb[ND + 1] * umn;
+            const T eb = Gb[ND + 1] * mn;
+            const T rhob = mnb * umn;
+            for (int k = 0; k < ND; ++k) {
+                ub1[k] = T(0.5) * (mn * Gb[1 + k] + (Gb[ND + 1] * p + mnb * rho) * n[k]);
+                ub2[k] = ub1[k];
+            }
+            const T esb = T(0.5) * eb;
+            rb1 = T(0.5) * rhob - esb * Q1[ND + 1] * ir1 * ir1;
+            rb2 = T(0.5) * rhob - esb * Q2[ND + 1] * ir2 * ir2;
+            pb1 = pb2 = T(0.5) * pbar;
+            Qb1[ND + 1] += esb * ir1;
+            Qb2[ND + 1] += esb * ir2;
+        } else {  // TP_CHANDRASEKHAR
+            const T ip1 = T(1) / p1, ip2 = T(1) / p2;
+            const T b1 = T(0.5) * Q1[0] * ip1, b2 = T(0.5) * Q2[0] * ip2;
+            T rho, beta, drho1, drho2, dbeta1, dbeta2;
+            logmean_dab(Q1[0], Q2[0], &rho, &drho1, &drho2);
+            logmean_dab(b1, b2, &beta, &dbeta1, &dbeta2);
+            const T sb = b1 + b2, isb = T(1) / sb;
+            const T irho = T(1) / rho, ibeta = T(1) / beta, ig1 = T(1) / g1;
+            const T p = T(0.5) * (Q1[0] + Q2[0]) * isb;
+            T uu = T(0), sq = T(0);
+            for (int k = 0; k < ND; ++k) {
+                uu += um[k] * um[k];
+                sq += u1[k] * u1[k] + u2[k] * u2[k];
+            }
+            const T h = T(0.5) * ibeta * ig1 - T(0.25) * sq + p * irho + uu;
+            const T mn = rho * umn;
+            const T hb = Gb[ND + 1] * mn;
+            const T mnb = Gb[0] + Gmu + Gb[ND + 1] * h;
+            const T pbar = Gmn + hb * irho;
+            const T rhob = mnb * umn - hb * p * irho * irho;
+            const T betab = T(-0.5) * hb * ibeta * ibeta * ig1;
+            const T sqb2 = T(-0.5) * hb;
+            for (int k = 0; k < ND; ++k) {
+                const T c = T(0.5) * (mn * Gb[1 + k] + mnb * rho * n[k] + T(2) * hb * um[k]);
+                ub1[k] = c + sqb2 * u1[k];
+                ub2[k] = c + sqb2 * u2[k];
+            }
+            const T rp = T(0.5) * pbar * isb, bp = pbar * p * isb;
+            const T bb1 = betab * dbeta1 - bp, bb2 = betab * dbeta2 - bp;
+            rb1 = rhob * drho1 + rp + T(0.5) * bb1 * ip1;
+            rb2 = rhob * drho2 + rp + T(0.5) * bb2 * ip2;
+            pb1 = -bb1 * b1 * ip1;
+            pb2 = -bb2 * b2 * ip2;
+        }
+        prim_pullback_ir<T, ND>(Q1, ir1, gamma, rb1, ub1, pb1, Qb1);
+        prim_pullback_ir<T, ND>(Q2, ir2, gamma, rb2, ub2, pb2, Qb2);
+    }
+}
+
 // LxF(StdAverage) / StdAverage numerical flux in the face frame, normal = first axis
 // (Equations/Euler.jl:99-165): Rlb += (∂Φ/∂Rl)^T Fb, Rrb += (∂Φ/∂Rr)^T Fb.
 template <typename T, int ND>
@@ -177,8 +317,9 @@ FLOU_HD inline void numflux_vjp(const T* Rl, const T* Rr, bool lxf, T intensity,
     for (int k = 0; k < ND; ++k) e1[k] = k == 0 ? T(1) : T(0);
     T h[NV];
     for (int v = 0; v < NV; ++v) h[v] = T(0.5) * Fb[v];
-    flux_vjp<T, ND>(Rl, e1, gamma, h, Rlb);
-    flux_vjp<T, ND>(Rr, e1, gamma, h, Rrb);
+    const T ir[2] = {T(1) / Rl[0], T(1) / Rr[0]};
+    flux_vjp_ir<T, ND>(Rl, ir[0], e1, gamma, h, Rlb);
+    flux_vjp_ir<T, ND>(Rr, ir[1], e1, gamma, h, Rrb);
     if (!lxf) return;
     const T g1 = gamma - T(1);
     T q[2], a[2], un[2];
@@ -186,9 +327,9 @@ FLOU_HD inline void numflux_vjp(const T* Rl, const T* Rr, bool lxf, T intensity,
         const T* R = s ? Rr : Rl;
         T m2 = T(0);
         for (int k = 0; k < ND; ++k) m2 += R[1 + k] * R[1 + k];
-        const T p = g1 * (R[ND + 1] - m2 / (T(2) * R[0]));
-        un[s] = R[1] / R[0];
-        a[s] = sqrt(gamma * p / R[0]);
+        const T p = g1 * (R[ND + 1] - T(0.5) * m2 * ir[s]);
+        un[s] = R[1] * ir[s];
+        a[s] = sqrt(gamma * p * ir[s]);
         q[s] = fabs(un[s]) + a[s];
     }
     const int w = q[0] < q[1] ? 1 : 0;  // the side the forward max keeps (left on ties)
@@ -207,9 +348,9 @@ FLOU_HD inline void numflux_vjp(const T* Rl, const T* Rr, bool lxf, T intensity,
     for (int k = 0; k < ND; ++k) ub[k] = T(0);
     ub[0] = (un[w] > T(0) ? T(1) : (un[w] < T(0) ? T(-1) : T(0))) * lb;
     const T ab = lb;
-    const T pb = ab * gamma / (T(2) * a[w] * R[0]);
-    const T rb = -ab * a[w] / (T(2) * R[0]);
-    prim_pullback<T, ND>(R, gamma, rb, ub, pb, Rb);
+    const T pb = T(0.5) * ab * gamma * ir[w] / a[w];
+    const T rb = T(-0.5) * ab * a[w] * ir[w];
+    prim_pullback_ir<T, ND>(R, ir[w], gamma, rb, ub, pb, Rb);
 }
 
 }  // namespace adj

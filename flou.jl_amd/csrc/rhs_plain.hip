@@ -148,6 +148,11 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     // line by the subcell flux of the interface to the node's right
     constexpr int HN = VOL == VOL_HYBRID_NB ? 1 : 0;
     __shared__ T sHs[HN ? EPB : 1][HN ? NV : 1][HN ? NPE : 1];
+    // Split form (Euler, GLL): the two-point fluxes of one direction, each pair of a line
+    // evaluated once by the whole block ([le][line][pair][v], pairs (a < b) in row order)
+    constexpr int PS = (VOL == VOL_SPLIT && EQ == EQ_EULER) ? 1 : 0;
+    constexpr int NPAIR = N * (N - 1) / 2;
+    __shared__ T sPair[PS ? EPB : 1][PS ? NPF * NPAIR : 1][PS ? NV : 1];
 
     const int tid = threadIdx.x;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
@@ -367,7 +372,8 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     }
     // The Gauss split form has block-wide phases after the volume terms: its threads past
     // the last element stay, skipping every per-node step.
-    if constexpr (!NB) {
+    // The split form's pair phases are block-wide too.
+    if constexpr (!NB && !PS) {
         if (!valid) return;
     }
 
@@ -377,13 +383,81 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     T J = sG[le][ND * ND];
     T Mnode[ND * ND];
     if constexpr (CURV) {
+        if (valid) {
 #pragma unroll
-        for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + gnode];
-        J = a.ngeom[ND * ND * Nd + gnode];
+            for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + gnode];
+            J = a.ngeom[ND * ND * Nd + gnode];
+        }
         M = Mnode;
     }
+    if constexpr (PS) {
+        // dQ[i] -= D♯[ii, j] F♯[ii, j] for j = 0..N-1 in order (F♯ diagonal = the contravariant
+        // flux, pairs with the lower line position first), as the node loop below computes
+        // it, but each pair flux is evaluated once per block instead of once per node
+        const T* Dsh = &sOps[ops_Dsh(N)];
+        for (int dir = 0; dir < ND; ++dir) {
+            __syncthreads();  // the previous direction's table has been read
+            for (int k = tid; k < EPB * NPF * NPAIR; k += TPB) {
+                const int le2 = k / (NPF * NPAIR), r = k - le2 * (NPF * NPAIR);
+                const int f2 = r / NPAIR;
+                int pa = 0, pb = r - f2 * NPAIR;
+                while (pb >= N - 1 - pa) { pb -= N - 1 - pa; ++pa; }
+                pb += pa + 1;
+                if (e0 + le2 >= a.ne) continue;
+                const int na = line_node<ND, N>(dir, f2, pa), nb = line_node<ND, N>(dir, f2, pb);
+                T qa[NV], qb[NV], Pa[NV], Pb[NV], F[NV];
 #pragma unroll
-    for (int dir = 0; dir < ((VOL == VOL_HYBRID || VOL == VOL_HYBRID_NB) ? 0 : ND) && valid; ++dir) {
+                for (int v = 0; v < NV; ++v) {
+                    qa[v] = sQ[le2][v][na]; qb[v] = sQ[le2][v][nb];
+                    Pa[v] = sP[le2][v][na]; Pb[v] = sP[le2][v][nb];
+                }
+                const T* Ma = &sG[le2][0] + ND * dir;
+                const T* Mb = Ma;
+                T Mac[ND], Mbc[ND];
+                if constexpr (CURV) {
+                    const int64_t ga = (int64_t)(e0 + le2) * NPE + na, gb = (int64_t)(e0 + le2) * NPE + nb;
+#pragma unroll
+                    for (int c = 0; c < ND; ++c) {
+                        Mac[c] = a.ngeom[(c + ND * dir) * Nd + ga];
+                        Mbc[c] = a.ngeom[(c + ND * dir) * Nd + gb];
+                    }
+                    Ma = Mac;
+                    Mb = Mbc;
+                }
+                plain::twopointflux<T, ND, EQ, TP>(qa, qb, Pa, Pb, sP[le2][NV][na], sP[le2][NV][nb], Ma, Mb,
+                                                   a.gamma, avel, a.errflag, F);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sPair[le2][r][v] = F[v];
+            }
+            __syncthreads();
+            if (valid) {
+                int f, ii;
+                node_line<ND, N>(node, dir, f, ii);
+                T qi[NV], Fd[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) qi[v] = sQ[le][v][node];
+                plain::contravariant_flux<T, ND, EQ>(qi, M, dir, a.gamma, avel, Fd);
+                for (int j = 0; j < N; ++j) {
+                    const T Dij = Dsh[ii * N + j];
+                    T F[NV];
+                    if (j == ii) {
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) F[v] = Fd[v];
+                    } else {
+                        const int lo = j < ii ? j : ii, hi = j < ii ? ii : j;
+                        const int r = f * NPAIR + lo * (2 * N - lo - 1) / 2 + (hi - lo - 1);
+#pragma unroll
+                        for (int v = 0; v < NV; ++v) F[v] = sPair[le][r][v];
+                    }
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) acc[v] = acc[v] - Dij * F[v];
+                }
+            }
+        }
+        if (!valid) return;
+    }
+#pragma unroll
+    for (int dir = 0; dir < ((VOL == VOL_HYBRID || VOL == VOL_HYBRID_NB || PS) ? 0 : ND) && valid; ++dir) {
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         if constexpr (VOL == VOL_STRONG) {

@@ -37,6 +37,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     __shared__ T sQ[EPB][NV][NPE];
     __shared__ T sW[EPB][NV][NPE];
     __shared__ T sTb[EPB][NF][NV][NPF];
+    // Split form (Euler): both sides of every pair flux's VJP of one direction, each pair
+    // evaluated once per block ([le][line][pair][side][v]), when the table fits in 64 KB
+    constexpr int NPAIR = N * (N - 1) / 2;
+    constexpr bool PS = VOL == VOL_SPLIT && EQ == EQ_EULER &&
+                        EPB * NPF * NPAIR * 2 * NV * (int)sizeof(T) <= 65536;
+    __shared__ T sPb[PS ? EPB : 1][PS ? NPF * NPAIR : 1][2][PS ? NV : 1];
 
     const int tid = threadIdx.x;
     const int e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
@@ -180,7 +186,11 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     __syncthreads();
 
     const int le = tid / NPE, node = tid % NPE, e = e0 + le;
-    if (e >= a.ne) return;
+    const bool valid = e < a.ne;
+    // the pair phases are block-wide: threads past the last element stay until they end
+    if constexpr (!PS) {
+        if (!valid) return;
+    }
 
     T qi[NV], wi[NV], V[NV];
 #pragma unroll
@@ -192,18 +202,80 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     const T* M = &sG[le][0];
     T Mnode[ND * ND];
     if constexpr (CURV) {
+        if (valid) {
 #pragma unroll
-        for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + (int64_t)e * NPE + node];
+            for (int c = 0; c < ND * ND; ++c) Mnode[c] = a.ngeom[c * Nd + (int64_t)e * NPE + node];
+        }
         M = Mnode;
     }
 #pragma unroll
     for (int dir = 0; dir < ND; ++dir) {
+        if constexpr (PS) {
+            // pair phase: Gb = -D♯_ab w_a - D♯_ba w_b of the pair (a < b), pulled back to both
+            const T* Dsh = &sOps[ops_Dsh(N)];
+            __syncthreads();  // the previous direction's table has been read
+            for (int k = tid; k < EPB * NPF * NPAIR; k += TPB) {
+                const int le2 = k / (NPF * NPAIR), r = k - le2 * (NPF * NPAIR);
+                const int f2 = r / NPAIR;
+                int pa = 0, pb = r - f2 * NPAIR;
+                while (pb >= N - 1 - pa) { pb -= N - 1 - pa; ++pa; }
+                pb += pa + 1;
+                if (e0 + le2 >= a.ne) continue;
+                const int na = line_node<ND, N>(dir, f2, pa), nbn = line_node<ND, N>(dir, f2, pb);
+                const T dab = Dsh[pa * N + pb], dba = Dsh[pb * N + pa];
+                T qa[NV], qb[NV], Gb[NV], Qba[NV], Qbb[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    qa[v] = sQ[le2][v][na];
+                    qb[v] = sQ[le2][v][nbn];
+                    Gb[v] = -dab * sW[le2][v][na] - dba * sW[le2][v][nbn];
+                    Qba[v] = T(0);
+                    Qbb[v] = T(0);
+                }
+                T nbar[ND];  // averaged metric (Ja_a + Ja_b)/2 of the pair
+#pragma unroll
+                for (int c = 0; c < ND; ++c) {
+                    if constexpr (CURV) {
+                        const int64_t ga = (int64_t)(e0 + le2) * NPE;
+                        nbar[c] = (a.ngeom[(c + ND * dir) * Nd + ga + na] + a.ngeom[(c + ND * dir) * Nd + ga + nbn]) /
+                                  T(2);
+                    } else {
+                        nbar[c] = sG[le2][c + ND * dir];
+                    }
+                }
+                adj::twopoint_vjp2<T, ND, TP>(qa, qb, nbar, a.gamma, Gb, Qba, Qbb);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    sPb[le2][r][0][v] = Qba[v];
+                    sPb[le2][r][1][v] = Qbb[v];
+                }
+            }
+            __syncthreads();
+            if (!valid) continue;
+        }
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         T n[ND];
 #pragma unroll
         for (int k = 0; k < ND; ++k) n[k] = M[k + ND * dir];
-        if constexpr (VOL == VOL_STRONG) {
+        if constexpr (PS) {
+            const T* Dsh = &sOps[ops_Dsh(N)];
+            for (int j = 0; j < N; ++j) {
+                if (j == ii) {
+                    T Gb[NV];
+                    const T d = Dsh[ii * N + ii];
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) Gb[v] = -d * wi[v];
+                    adj::flux_vjp<T, ND>(qi, n, a.gamma, Gb, V);
+                } else {
+                    const int lo = j < ii ? j : ii, hi = j < ii ? ii : j;
+                    const int r = f * NPAIR + lo * (2 * N - lo - 1) / 2 + (hi - lo - 1);
+                    const int side = ii < j ? 0 : 1;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) V[v] = V[v] + sPb[le][r][side][v];
+                }
+            }
+        } else if constexpr (VOL == VOL_STRONG) {
             const T* Ds = &sOps[ops_Ds(N)];
             T cb[NV];
 #pragma unroll
@@ -269,6 +341,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             V[v] = V[v] + lmi * sTb[le][2 * dir][v][f] + lpi * sTb[le][2 * dir + 1][v][f];
     }
 
+    if (!valid) return;
     // epilogue
     const int64_t gi = (int64_t)e * NPE + node;
 #pragma unroll

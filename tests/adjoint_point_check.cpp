@@ -1,7 +1,8 @@
 // CPU check of the point-wise reverse-mode products of flou.jl_amd/csrc/adjoint.hpp
 // against forward-mode dual-number JVPs of the oracle (oracle/flou_oracle.hpp): for random
 // states, directions V and cotangents G, <G, J V> (oracle) must equal <J^T G, V> (adjoint)
-// to round-off.  Prints the worst relative mismatch per function, dimension and spread;
+// to round-off (the two-sided two-point VJP with both states perturbed at once).  Prints the
+// worst relative mismatch per function, dimension and spread;
 // built and run by tests/test_adjoint_pointwise.py.
 #include <cstdio>
 #include <random>
@@ -41,6 +42,20 @@ template <int ND> double run(int tp, double spread, int which) {
         if (tp == 0) flouhip::adj::twopoint_vjp1<double, ND, 0>(Q1, Q2, n, 1.4, G, Qb);
         if (tp == 1) flouhip::adj::twopoint_vjp1<double, ND, 1>(Q1, Q2, n, 1.4, G, Qb);
         if (tp == 2) flouhip::adj::twopoint_vjp1<double, ND, 2>(Q1, Q2, n, 1.4, G, Qb);
+    } else if (which == 4) {  // two-point flux, both states at once (twopoint_vjp2)
+        double V2[NV];
+        for (int v = 0; v < NV; ++v) { V2[v] = U(-1, 1); q2[v].d[0] = V2[v]; }
+        oracle::twopointflux(d, q1, q2, M + ND * dir, M + ND * dir, F);
+        if (tp == 0) flouhip::adj::twopoint_vjp2<double, ND, 0>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+        if (tp == 1) flouhip::adj::twopoint_vjp2<double, ND, 1>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+        if (tp == 2) flouhip::adj::twopoint_vjp2<double, ND, 2>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+        double jvp = 0, vjp = 0, sc = 0;
+        for (int v = 0; v < NV; ++v) {
+            jvp += G[v] * F[v].d[0];
+            vjp += V[v] * Qb[v] + V2[v] * Qb2[v];
+            sc += fabs(G[v] * F[v].d[0]);
+        }
+        return fabs(jvp - vjp) / (sc + 1e-300);
     } else if (which == 3) {  // LxF, derivative in the right state
         D1 r1[NV], r2[NV];
         for (int v = 0; v < NV; ++v) { r1[v] = D1(Q1[v]); r2[v] = D1(Q2[v]); r2[v].d[0] = V[v]; }
@@ -58,10 +73,10 @@ template <int ND> double run(int tp, double spread, int which) {
     return fabs(jvp - vjp) / (sc + 1e-300);
 }
 int main() {
-    const char* nm[] = {"flux", "twopoint", "lxf-left", "lxf-right"};
-    for (int which = 0; which < 4; ++which)
+    const char* nm[] = {"flux", "twopoint", "lxf-left", "lxf-right", "twopoint-both"};
+    for (int which = 0; which < 5; ++which)
         for (int tp = 0; tp < 3; ++tp) {
-            if (which != 1 && tp > 0) continue;
+            if (which != 1 && which != 4 && tp > 0) continue;
             for (double spread : {1e-4, 0.05, 0.5}) {
                 double e1 = 0, e2 = 0, e3 = 0;
                 for (int t = 0; t < 200; ++t) {
