@@ -271,22 +271,32 @@ __host__ __device__ constexpr int lslot(int d, int v) { return (v == 0 || v == 4
 
 // Numerical flux on the face where a line ends (row R of the line) and its lift, which
 // starts row R: acc[R] = -coef·[s(f_own + f_ext) + λ(Q_own - Q_ext)], in the line frame
-// (slot 1 = momentum along the line); the energy entry is scaled by ehalf.
-template <int N, int R>
-__device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[5], float s, float coef,
-                                          float gm1, float gamma, float diss, float ehalf,
+// (slot 1 = momentum along the line); the energy entry is scaled by ehalf.  With the
+// Chandrasekhar inputs (PO: ρ/2, u_n/2, H = p/(2ρ) of the own node in Po) the own side
+// takes u_n, p and c² from them instead of dividing by ρ; the own node's positivity is
+// already checked at staging.
+template <int N, int R, bool PO>
+__device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[5], const float* Po, float s,
+                                          float coef, float gm1, float gamma, float diss, float ehalf,
                                           float (&acc)[N][5], bool& bad) {
     float fo[5], fx[5], ao, ax;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const float* Q = side ? x : o;
         float* f = side ? fx : fo;
-        const float ir = rcp(Q[0]);
-        const float m2 = fmaf(Q[1], Q[1], fmaf(Q[2], Q[2], Q[3] * Q[3]));
-        const float p = gm1 * fmaf(-0.5f * m2, ir, Q[4]);
-        const float un = Q[1] * ir;
-        const float c2 = gamma * p * ir;
-        bad |= !(c2 >= 0.0f);
+        float p, un, c2;
+        if (PO && !side) {
+            un = Po[1] + Po[1];
+            p = 4.0f * Po[0] * Po[6];
+            c2 = (2.0f * gamma) * Po[6];
+        } else {
+            const float ir = rcp(Q[0]);
+            const float m2 = fmaf(Q[1], Q[1], fmaf(Q[2], Q[2], Q[3] * Q[3]));
+            p = gm1 * fmaf(-0.5f * m2, ir, Q[4]);
+            un = Q[1] * ir;
+            c2 = gamma * p * ir;
+            bad |= !(c2 >= 0.0f);
+        }
         const float a = fabsf(un) + sq(c2);
         f[0] = Q[1];
         f[1] = fmaf(Q[1], un, p);
@@ -481,13 +491,14 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         load_vars<NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
         const float2 g0 = sF[lle * 6 + 2 * ldir], g1 = sF[lle * 6 + 2 * ldir + 1];
         const float ehalf = TP == TP_CHANDRASEKHAR ? 0.5f : 1.0f;  // energy row accumulated halved
+        constexpr bool PO = TP == TP_CHANDRASEKHAR;
+        float P[N][7];
+        load_line<N, NN, PO>(sP, L, P);
         bool bad = false;
-        face_init<N, 0>(o0, xl[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
-        face_init<N, N - 1>(o1, xl[1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        face_init<N, 0, PO>(o0, xl[0], P[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        face_init<N, N - 1, PO>(o1, xl[1], P[N - 1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
         if (bad && line_ok) err |= ERR_SQRT_NEG;
 
-        float P[N][7];
-        load_line<N, NN, TP == TP_CHANDRASEKHAR>(sP, L, P);
         if constexpr (TP == TP_CHANDRASEKHAR) {
             float amax = P[0][0], amin = P[0][0], bmax = P[0][4], bmin = P[0][4];
 #pragma unroll

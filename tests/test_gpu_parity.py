@@ -329,20 +329,27 @@ def test_non_periodic_boundaries_match_oracle():
     assert rel_err(got, ref) <= 1e-13
 
 
+@pytest.mark.parametrize("axis", [0, 2], ids=["x", "z"])
 @pytest.mark.parametrize("kind", ["inflow-outflow", "dirichlet", "slip-outflow"])
-def test_fp32_fast_kernel_boundaries_match_oracle(kind):
-    """Every boundary kind on the FP32 fast path (3-D, N=5, x non-periodic), against the
-    FP64 oracle and against the FP32 plain kernel on the same handle."""
-    mesh, std, equation, op, orc = build(3, 6, [3, 2, 2], "split", "ch", periodic=[0, 1, 1],
+def test_fp32_fast_kernel_boundaries_match_oracle(kind, axis):
+    """Every boundary kind on the FP32 fast path (3-D, N=5, x or z non-periodic), against
+    the FP64 oracle and against the FP32 plain kernel on the same handle.  z: the lines of
+    the block's last wave (z lines of its second element) end on the boundaries."""
+    periodic = [1, 1, 1]
+    periodic[axis] = 0
+    nel = [2, 2, 2]
+    nel[axis] = 3
+    mesh, std, equation, op, orc = build(3, 6, nel, "split", "ch", periodic=periodic,
                                          finish=[2 * np.pi] * 3)
+    lo, hi = str(2 * axis + 1), str(2 * axis + 2)
     qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7])
     X, FX = orc.coords()
     if kind == "inflow-outflow":
-        bcs = {"1": F.EulerInflowBC(qin), "2": F.EulerOutflowBC()}
+        bcs = {lo: F.EulerInflowBC(qin), hi: F.EulerOutflowBC()}
         orc.set_bc(0, O.BC_INFLOW, qin)
         orc.set_bc(1, O.BC_OUTFLOW)
     elif kind == "slip-outflow":
-        bcs = {"1": F.EulerSlipBC(), "2": F.EulerOutflowBC()}
+        bcs = {lo: F.EulerSlipBC(), hi: F.EulerOutflowBC()}
         orc.set_bc(0, O.BC_SLIP)
         orc.set_bc(1, O.BC_OUTFLOW)
     else:
@@ -350,8 +357,8 @@ def test_fp32_fast_kernel_boundaries_match_oracle(kind):
             r = 1.0 + 0.1 * np.sin(x[1]) * np.cos(x[2])
             return np.array([r, 0.2 * r, 0.05 * r, -0.03 * r, 2.5 + 0.1 * np.cos(x[0] + x[2])])
 
-        bcs = {"1": F.GenericBC(lambda _q, x, _f, _t, _e: qext(x)),
-               "2": F.GenericBC(lambda _q, x, _f, _t, _e: qext(x))}
+        bcs = {lo: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x)),
+               hi: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x))}
         bc = np.zeros((orc.nf, orc.npf, 5))
         for f in range(orc.nf):
             for k in range(orc.npf):
