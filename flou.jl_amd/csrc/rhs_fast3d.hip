@@ -7,7 +7,8 @@
 //   * EPB elements per block, one thread per (element, direction, line) -- every thread
 //     does the same work.  Each thread first loads the face tables and then the two
 //     exterior traces of its line (neighbour nodes or boundary states, into registers),
-//     issued after the block's own nodes so that the staging waits for those only;
+//     issued after the block's own nodes so that the staging waits for those only; a
+//     neighbour inside the block is read from LDS after staging instead;
 //   * staging: every thread loads all five variables of its nodes (coalesced, one
 //     uniform base per variable) and writes the node states and the two-point flux
 //     inputs to LDS;
@@ -363,8 +364,8 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
         const int slot = (e0 + lle) * 6 + 2 * ldir + sd;
-        xnb[sd] = line_in ? a.nb[slot] : -2;
-        xinf[sd] = line_in ? a.finfo[slot] : 0;
+        xnb[sd] = line_ok ? a.nb[slot] : -2;
+        xinf[sd] = line_ok ? a.finfo[slot] : 0;
     }
 
     // ---- staging ---------------------------------------------------------------------
@@ -378,14 +379,18 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         for (int r = 0; r < NPT; ++r) q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
     }
     // exterior traces at both line ends: the neighbour's node on interior faces
-    // (branch-free; other lanes read 0), issued after the element nodes so that the
-    // staging below waits only for those
+    // (branch-free; other lanes, including those past the block's line tasks, read 0
+    // without a memory access), issued after the element nodes so that the staging below
+    // waits only for those
     uint32_t xo[2];
+    bool inblk[2];  // neighbour is one of this block's elements: read from LDS after staging
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
         const int kp = finfo_nbpos(xinf[sd]);
-        xo[sd] = xnb[sd] >= 0 ? ((uint32_t)xnb[sd] * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
-                              : 0xffffffffu;
+        inblk[sd] = xnb[sd] >= e0 && xnb[sd] < e0 + nel;
+        xo[sd] = xnb[sd] >= 0 && !inblk[sd]
+                     ? ((uint32_t)xnb[sd] * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
+                     : 0xffffffffu;
     }
     float xr[2][5];
 #pragma unroll
@@ -461,6 +466,18 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
             for (int v = 0; v < 5; ++v) xr[sd][v] = src[v * vs];
         }
     }
+    __syncthreads();
+    // neighbours in this block (the x neighbour of every other element): their node
+    // states are already in LDS
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        if (inblk[sd]) {
+            const int kp = finfo_nbpos(xinf[sd]);
+            const int node = (xnb[sd] - e0) * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+#pragma unroll
+            for (int v = 0; v < 5; ++v) xr[sd][v] = sQ[v * NN + node];
+        }
+    }
     // exterior traces in the line frame; a slip wall reflects the normal momentum
     float xl[2][5];
 #pragma unroll
@@ -473,7 +490,6 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         xl[sd][3] = ldir == 0 ? xr[sd][3] : (ldir == 1 ? xr[sd][1] : xr[sd][2]);
         xl[sd][4] = xr[sd][4];
     }
-    __syncthreads();
 
     // ---- line tasks ------------------------------------------------------------------
     LineAddr L;
