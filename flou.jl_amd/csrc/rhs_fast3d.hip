@@ -368,6 +368,16 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         xinf[sd] = line_ok ? a.finfo[slot] : 0;
     }
 
+    // per face (lanes tid < 6*EPB): the face-geometry index and the side, loaded with the
+    // face tables so that the dependent geometry loads below wait for these only
+    const int fle = tid / 6, fk = tid - (tid / 6) * 6;
+    const bool face_lane = tid < 6 * EPB && fle < nel;
+    int fgi = 0, fside = 0;
+    if (face_lane) {
+        fgi = a.fgi[(e0 + fle) * 6 + fk];
+        fside = finfo_side(a.finfo[(e0 + fle) * 6 + fk]);
+    }
+
     // ---- staging ---------------------------------------------------------------------
     // element nodes, all five variables per node (lanes past the block's last element read 0)
     const uint32_t blk_bytes = (uint32_t)(nel * NPE) * 4u;
@@ -377,6 +387,16 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         const rsrc_t rq = make_rsrc(a.Q + v * Nd + n0, blk_bytes);
 #pragma unroll
         for (int r = 0; r < NPT; ++r) q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
+    }
+    // own outward normal sign along the axis and the lift coefficient of each face,
+    // issued before the exterior traces so that waiting for them does not wait for those
+    float fsn = 0.0f, fjac = 0.0f, feg = 0.0f, fed = 0.0f;
+    if (face_lane) {
+        const float* eg = a.egeom + (int64_t)(e0 + fle) * a.egeom_stride;
+        fsn = a.fgeom[(int64_t)fgi * 10 + (fk >> 1)];
+        fjac = a.fgeom[(int64_t)fgi * 10 + 9];
+        feg = eg[4 + (fk >> 1)];
+        if (fk < 3) fed = eg[fk];
     }
     // exterior traces at both line ends: the neighbour's node on interior faces
     // (branch-free; other lanes, including those past the block's line tasks, read 0
@@ -398,21 +418,6 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) xr[sd][v] = bload(rv, xo[sd]);
-    }
-    // per face: own outward normal sign along the axis and the lift coefficient
-    if (tid < 6 * EPB) {
-        const int le = tid / 6, k = tid - (tid / 6) * 6;
-        if (le < nel) {
-            const int e = e0 + le;
-            const float* eg = a.egeom + (int64_t)e * a.egeom_stride;
-            const int gi = a.fgi[e * 6 + k];
-            const int side = finfo_side(a.finfo[e * 6 + k]);
-            const float sn = a.fgeom[(int64_t)gi * 10 + (k >> 1)];
-            const float jac = a.fgeom[(int64_t)gi * 10 + 9];
-            const float lift = (k & 1) ? a.lift_p : a.lift_m;
-            sF[tid] = make_float2(side ? -sn : sn, 0.5f * eg[4 + (k >> 1)] * lift * jac);
-            if (k < 3) sG[le * 4 + k] = eg[k];
-        }
     }
     // flux inputs and LDS writes
 #pragma unroll
@@ -444,6 +449,11 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         sP[6 * NN + j] = in[4];
         sP[7 * NN + j] = in[5];
         if constexpr (TP == TP_CHANDRASEKHAR) sP[8 * NN + j] = 0.5f * p * ir;
+    }
+    if (face_lane) {
+        const float lift = (fk & 1) ? a.lift_p : a.lift_m;
+        sF[tid] = make_float2(fside ? -fsn : fsn, 0.5f * feg * lift * fjac);
+        if (fk < 3) sG[fle * 4 + fk] = fed;
     }
     // boundary faces: Dirichlet or inflow states, or the own node (outflow, slip)
 #pragma unroll
