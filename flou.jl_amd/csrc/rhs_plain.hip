@@ -160,15 +160,46 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     const int64_t Nd = a.ndofs;
     const T avel[3] = {a.a0, a.a1, a.a2};
 
+    // The face node this thread takes first in the face phase (it = tid): its face tables
+    // are loaded before the element's nodes, and on GLL nodes (gather) the neighbour's
+    // trace after them, so that both are in flight during the staging instead of after it.
+    constexpr int NFN = EPB * NF * NPF;  // face nodes of the block
+    const int pfk = (tid % (NF * NPF)) / NPF, pff = tid % NPF, pfe = e0 + tid / (NF * NPF);
+    const bool pf = tid < NFN && pfe < a.ne;
+    int pf_info = 0, pf_nbe = -1, pf_fgi = 0;
+    if (pf) {
+        const int slot = pfe * NF + pfk;
+        pf_info = a.finfo[slot];
+        pf_nbe = a.nb[slot];
+        pf_fgi = a.fgi[slot];
+    }
+
+    constexpr int NLD = (NV * EPB * NPE + TPB - 1) / TPB;  // staging loads per thread
+    T ld[NLD];
+#pragma unroll
+    for (int r = 0; r < NLD; ++r) {
+        const int i = tid + r * TPB;
+        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+        ld[r] = (i < NV * EPB * NPE && e0 + j / NPE < a.ne) ? a.Q[v * Nd + (int64_t)e0 * NPE + j] : T(0);
+    }
+    T pf_ext[NV];
+    if (pf && a.gather && pf_nbe >= 0) {
+        const int kp = finfo_nbpos(pf_info);
+        const T* Qn = a.Q + (int64_t)pf_nbe * NPE;
+        const int nn = line_node<ND, N>(kp >> 1, pff, (kp & 1) ? N - 1 : 0);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pf_ext[v] = Qn[v * Nd + nn];
+    }
     for (int i = tid; i < NOPS; i += TPB) sOps[i] = a.ops[i];
     for (int i = tid; i < EPB * NG; i += TPB) {
         const int le = i / NG, c = i % NG, e = e0 + le;
         if (e < a.ne) sG[le][c] = a.egeom[(int64_t)e * a.egeom_stride + c];
     }
-    for (int i = tid; i < NV * EPB * NPE; i += TPB) {
+#pragma unroll
+    for (int r = 0; r < NLD; ++r) {
+        const int i = tid + r * TPB;
         const int v = i / (EPB * NPE), j = i % (EPB * NPE);
-        const int le = j / NPE, nd = j % NPE;
-        if (e0 + le < a.ne) sQ[le][v][nd] = a.Q[v * Nd + (int64_t)e0 * NPE + j];
+        if (i < NV * EPB * NPE && e0 + j / NPE < a.ne) sQ[j / NPE][v][j % NPE] = ld[r];
     }
     __syncthreads();
 
@@ -224,13 +255,18 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
             }
         }
         const int slot = e2 * NF + k;
-        const int info = a.finfo[slot];
-        const int nbe = a.nb[slot];
-        const T* g = a.fgeom + ((int64_t)a.fgi[slot] * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
+        const bool first = it == tid;  // prefetched before the staging barrier
+        const int info = first ? pf_info : a.finfo[slot];
+        const int nbe = first ? pf_nbe : a.nb[slot];
+        const int fgi = first ? pf_fgi : a.fgi[slot];
+        const T* g = a.fgeom + ((int64_t)fgi * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
         if (nbe >= 0) {
             const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
             const T* Qn = a.Q + (int64_t)nbe * NPE;
-            if (a.gather) {
+            if (a.gather && first) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ext[v] = pf_ext[v];
+            } else if (a.gather) {
                 const int nn = line_node<ND, N>(ndir, f, nlr ? N - 1 : 0);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ext[v] = Qn[v * Nd + nn];
