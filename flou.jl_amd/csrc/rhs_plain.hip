@@ -18,6 +18,7 @@
 //      (OpDivergence.jl:42-100) and divides by J, one thread per node.
 // Compile with -DFLOUHIP_PREC_F64 or -DFLOUHIP_PREC_F32 and -DFLOUHIP_ND=1|2|3.
 #include "common.hpp"
+#include <type_traits>
 #include "physics_plain.hpp"
 
 namespace flouhip {
@@ -160,12 +161,16 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     const int64_t Nd = a.ndofs;
     const T avel[3] = {a.a0, a.a1, a.a2};
 
-    // The face node this thread takes first in the face phase (it = tid): its face tables
-    // are loaded before the element's nodes, and on GLL nodes (gather) the neighbour's
-    // trace after them, so that both are in flight during the staging instead of after it.
+    // FP64 split form, Euler, 6 or more nodes per direction (occupancy is set by LDS there,
+    // so the registers are free): the face node this thread takes first in the face phase
+    // (it = tid) has its face tables loaded before the element's nodes, and on GLL nodes
+    // (gather) its neighbour trace, face frame and surface J after them, together with the
+    // stage epilogue's second operand (RK register or SSP base) of this thread's node.  All
+    // are then in flight during the staging instead of after it.
+    constexpr bool PF = std::is_same<T, double>::value && EQ == EQ_EULER && VOL == VOL_SPLIT && N >= 6;
     constexpr int NFN = EPB * NF * NPF;  // face nodes of the block
     const int pfk = (tid % (NF * NPF)) / NPF, pff = tid % NPF, pfe = e0 + tid / (NF * NPF);
-    const bool pf = tid < NFN && pfe < a.ne;
+    const bool pf = PF && tid < NFN && pfe < a.ne;
     int pf_info = 0, pf_nbe = -1, pf_fgi = 0;
     if (pf) {
         const int slot = pfe * NF + pfk;
@@ -173,16 +178,17 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         pf_nbe = a.nb[slot];
         pf_fgi = a.fgi[slot];
     }
-
-    constexpr int NLD = (NV * EPB * NPE + TPB - 1) / TPB;  // staging loads per thread
+    constexpr int NLD = PF ? (NV * EPB * NPE + TPB - 1) / TPB : 1;  // staging loads per thread
     T ld[NLD];
+    if constexpr (PF) {
 #pragma unroll
-    for (int r = 0; r < NLD; ++r) {
-        const int i = tid + r * TPB;
-        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
-        ld[r] = (i < NV * EPB * NPE && e0 + j / NPE < a.ne) ? a.Q[v * Nd + (int64_t)e0 * NPE + j] : T(0);
+        for (int r = 0; r < NLD; ++r) {
+            const int i = tid + r * TPB;
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            ld[r] = (i < NV * EPB * NPE && e0 + j / NPE < a.ne) ? a.Q[v * Nd + (int64_t)e0 * NPE + j] : T(0);
+        }
     }
-    T pf_ext[NV];
+    T pf_ext[NV], pf_g[3 * ND + 1], pf_aux[NV];
     if (pf && a.gather && pf_nbe >= 0) {
         const int kp = finfo_nbpos(pf_info);
         const T* Qn = a.Q + (int64_t)pf_nbe * NPE;
@@ -190,16 +196,35 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
 #pragma unroll
         for (int v = 0; v < NV; ++v) pf_ext[v] = Qn[v * Nd + nn];
     }
+    if (pf) {
+        const T* g = a.fgeom + ((int64_t)pf_fgi * (CURV ? NPF : 1) + (CURV ? pff : 0)) * (3 * ND + 1);
+#pragma unroll
+        for (int c = 0; c < 3 * ND + 1; ++c) pf_g[c] = g[c];
+    }
+    const T* aux = (a.mode == MODE_2N && !a.first) ? a.reg
+                 : ((a.mode == MODE_SSP2 || a.mode == MODE_SSP3) ? a.ubase : nullptr);
+    if (PF && aux && e0 + tid / NPE < a.ne) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) pf_aux[v] = aux[v * Nd + (int64_t)e0 * NPE + tid];
+    }
     for (int i = tid; i < NOPS; i += TPB) sOps[i] = a.ops[i];
     for (int i = tid; i < EPB * NG; i += TPB) {
         const int le = i / NG, c = i % NG, e = e0 + le;
         if (e < a.ne) sG[le][c] = a.egeom[(int64_t)e * a.egeom_stride + c];
     }
+    if constexpr (PF) {
 #pragma unroll
-    for (int r = 0; r < NLD; ++r) {
-        const int i = tid + r * TPB;
-        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
-        if (i < NV * EPB * NPE && e0 + j / NPE < a.ne) sQ[j / NPE][v][j % NPE] = ld[r];
+        for (int r = 0; r < NLD; ++r) {
+            const int i = tid + r * TPB;
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            if (i < NV * EPB * NPE && e0 + j / NPE < a.ne) sQ[j / NPE][v][j % NPE] = ld[r];
+        }
+    } else {
+        for (int i = tid; i < NV * EPB * NPE; i += TPB) {
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            const int le = j / NPE, nd = j % NPE;
+            if (e0 + le < a.ne) sQ[le][v][nd] = a.Q[v * Nd + (int64_t)e0 * NPE + j];
+        }
     }
     __syncthreads();
 
@@ -255,11 +280,17 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
             }
         }
         const int slot = e2 * NF + k;
-        const bool first = it == tid;  // prefetched before the staging barrier
+        const bool first = PF && it == tid;  // prefetched before the staging barrier
         const int info = first ? pf_info : a.finfo[slot];
         const int nbe = first ? pf_nbe : a.nb[slot];
-        const int fgi = first ? pf_fgi : a.fgi[slot];
-        const T* g = a.fgeom + ((int64_t)fgi * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
+        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) + (CURV ? f : 0)) *
+                                   (3 * ND + 1);
+        T gbuf[PF ? 3 * ND + 1 : 1];
+        if constexpr (PF) {
+#pragma unroll
+            for (int c = 0; c < 3 * ND + 1; ++c) gbuf[c] = first ? pf_g[c] : g[c];
+            g = gbuf;
+        }
         if (nbe >= 0) {
             const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
             const T* Qn = a.Q + (int64_t)nbe * NPE;
@@ -691,7 +722,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
                 a.out[idx] = kv;
                 break;
             case MODE_2N: {
-                const T t = a.first ? a.dt * kv : a.A * a.reg[idx] + a.dt * kv;
+                const T t = a.first ? a.dt * kv : a.A * (PF ? pf_aux[v] : a.reg[idx]) + a.dt * kv;
                 a.reg[idx] = t;
                 a.out[idx] = uin + a.B * t;
                 break;
@@ -700,10 +731,10 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
                 a.out[idx] = uin + a.dt * kv;
                 break;
             case MODE_SSP2:
-                a.out[idx] = ((T(3) * a.ubase[idx] + uin) + a.dt * kv) / T(4);
+                a.out[idx] = ((T(3) * (PF ? pf_aux[v] : a.ubase[idx]) + uin) + a.dt * kv) / T(4);
                 break;
             default:
-                a.out[idx] = ((a.ubase[idx] + T(2) * uin) + (T(2) * a.dt) * kv) / T(3);
+                a.out[idx] = (((PF ? pf_aux[v] : a.ubase[idx]) + T(2) * uin) + (T(2) * a.dt) * kv) / T(3);
                 break;
         }
     }

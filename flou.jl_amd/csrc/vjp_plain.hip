@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "physics_plain.hpp"
 #include "adjoint.hpp"
+#include <type_traits>
 
 namespace flouhip {
 
@@ -49,20 +50,94 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     const int64_t Nd = a.ndofs;
     const T avel[3] = {a.a0, a.a1, a.a2};
 
+    // FP64 split form, Euler, affine, 6-7 nodes per direction in 2-D and 6 in 3-D (occupancy
+    // is set by LDS there and the registers fit without spills or a lower occupancy):
+    // everything a thread reads from HBM is issued
+    // up front -- the element's nodes of Q and Y with their J, the face tables of the face
+    // node it takes first in the face phase (it = tid), then (GLL: gather) that face node's
+    // neighbour trace, the neighbour line of Y, the neighbour's J and the face frame, and
+    // the epilogue operands of its own node -- so that one barrier ends the staging and the
+    // face phase waits for nothing.  The arithmetic and its order are unchanged.
+    constexpr bool PF = std::is_same<T, double>::value && EQ == EQ_EULER && VOL == VOL_SPLIT && !CURV &&
+                        ((ND == 2 && N >= 6 && N <= 7) || (ND == 3 && N == 6));
+    constexpr int NFN = EPB * NF * NPF;  // face nodes of the block
+    const int pfk = (tid % (NF * NPF)) / NPF, pff = tid % NPF, pfe = e0 + tid / (NF * NPF);
+    const bool pf = PF && a.gather && tid < NFN && pfe < a.ne;
+    int pf_info = 0, pf_nbe = -1, pf_fgi = 0;
+    if (pf) {
+        const int slot = pfe * NF + pfk;
+        pf_info = a.finfo[slot];
+        pf_nbe = a.nb[slot];
+        pf_fgi = a.fgi[slot];
+    }
+    constexpr int NLD = PF ? (NV * EPB * NPE + TPB - 1) / TPB : 1;  // staging loads per thread
+    T ldq[NLD], ldy[NLD], ldj[NLD];
+    if constexpr (PF) {
+#pragma unroll
+        for (int r = 0; r < NLD; ++r) {
+            const int i = tid + r * TPB;
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            const bool in = i < NV * EPB * NPE && e0 + j / NPE < a.ne;
+            const int64_t g = v * Nd + (int64_t)e0 * NPE + j;
+            ldq[r] = in ? a.Q[g] : T(0);
+            ldy[r] = in ? A.Y[g] : T(0);
+            ldj[r] = in ? a.egeom[(int64_t)(e0 + j / NPE) * a.egeom_stride + ND * ND] : T(1);
+        }
+    }
+    T pf_ext[NV], pf_yl[N][NV], pf_g[3 * ND + 1], pf_jn = T(1);
+    if (pf && pf_nbe >= 0) {
+        const int kp = finfo_nbpos(pf_info), ndir = kp >> 1;
+        const int64_t base = (int64_t)pf_nbe * NPE;
+        pf_jn = a.egeom[(int64_t)pf_nbe * a.egeom_stride + ND * ND];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            pf_ext[v] = a.Q[v * Nd + base + line_node<ND, N>(ndir, pff, (kp & 1) ? N - 1 : 0)];
+#pragma unroll
+            for (int p = 0; p < N; ++p) pf_yl[p][v] = A.Y[v * Nd + base + line_node<ND, N>(ndir, pff, p)];
+        }
+    }
+    if (pf) {
+        const T* g = a.fgeom + (int64_t)pf_fgi * (3 * ND + 1);
+#pragma unroll
+        for (int c = 0; c < 3 * ND + 1; ++c) pf_g[c] = g[c];
+    }
+    T pf_p0[NV], pf_p1[NV], pf_p2[NV];
+    if (PF && e0 + tid / NPE < a.ne) {
+        const int64_t gi = (int64_t)e0 * NPE + tid;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            if (A.P0) pf_p0[v] = A.P0[v * Nd + gi];
+            if (A.P1) pf_p1[v] = A.P1[v * Nd + gi];
+            if (A.P2) pf_p2[v] = A.P2[v * Nd + gi];
+        }
+    }
+
     for (int i = tid; i < NOPS; i += TPB) sOps[i] = a.ops[i];
     for (int i = tid; i < EPB * NG; i += TPB) {
         const int le = i / NG, c = i % NG, e = e0 + le;
         if (e < a.ne) sG[le][c] = a.egeom[(int64_t)e * a.egeom_stride + c];
     }
-    __syncthreads();
-    for (int i = tid; i < NV * EPB * NPE; i += TPB) {
-        const int v = i / (EPB * NPE), j = i % (EPB * NPE);
-        const int le = j / NPE, nd = j % NPE;
-        if (e0 + le < a.ne) {
-            const int64_t g = v * Nd + (int64_t)e0 * NPE + j;
-            sQ[le][v][nd] = a.Q[g];
-            const T J = CURV ? a.ngeom[ND * ND * Nd + (int64_t)e0 * NPE + j] : sG[le][ND * ND];
-            sW[le][v][nd] = A.c * A.Y[g] / J;
+    if constexpr (PF) {
+#pragma unroll
+        for (int r = 0; r < NLD; ++r) {
+            const int i = tid + r * TPB;
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            if (i < NV * EPB * NPE && e0 + j / NPE < a.ne) {
+                sQ[j / NPE][v][j % NPE] = ldq[r];
+                sW[j / NPE][v][j % NPE] = A.c * ldy[r] / ldj[r];
+            }
+        }
+    } else {
+        __syncthreads();
+        for (int i = tid; i < NV * EPB * NPE; i += TPB) {
+            const int v = i / (EPB * NPE), j = i % (EPB * NPE);
+            const int le = j / NPE, nd = j % NPE;
+            if (e0 + le < a.ne) {
+                const int64_t g = v * Nd + (int64_t)e0 * NPE + j;
+                sQ[le][v][nd] = a.Q[g];
+                const T J = CURV ? a.ngeom[ND * ND * Nd + (int64_t)e0 * NPE + j] : sG[le][ND * ND];
+                sW[le][v][nd] = A.c * A.Y[g] / J;
+            }
         }
     }
     __syncthreads();
@@ -88,10 +163,18 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             Lo[v] = t;
         }
         const int slot = e * NF + k;
-        const int info = a.finfo[slot];
-        const int nbe = a.nb[slot];
+        const bool first = pf && it == tid;  // issued before the staging barrier
+        const int info = first ? pf_info : a.finfo[slot];
+        const int nbe = first ? pf_nbe : a.nb[slot];
         const bool slave = finfo_side(info);
-        const T* g = a.fgeom + ((int64_t)a.fgi[slot] * (CURV ? NPF : 1) + (CURV ? f : 0)) * (3 * ND + 1);
+        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) + (CURV ? f : 0)) *
+                                   (3 * ND + 1);
+        T gbuf[PF ? 3 * ND + 1 : 1];
+        if constexpr (PF) {
+#pragma unroll
+            for (int c = 0; c < 3 * ND + 1; ++c) gbuf[c] = first ? pf_g[c] : g[c];
+            g = gbuf;
+        }
         const T jac = g[3 * ND];
         T Fb[NV];
 #pragma unroll
@@ -101,18 +184,24 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
             const T* lq = &sOps[nlr ? ops_lp(N) : ops_lm(N)];
             const T* gq = &sOps[nlr ? ops_gp(N) : ops_gm(N)];
-            const T Jn = a.egeom[(int64_t)nbe * a.egeom_stride + ND * ND];
+            const T Jn = first ? pf_jn : a.egeom[(int64_t)nbe * a.egeom_stride + ND * ND];
             const int64_t base = (int64_t)nbe * NPE;
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 T s = T(0), t = T(0);
-                for (int p = 0; p < N; ++p) {
-                    const int64_t nn = base + line_node<ND, N>(ndir, f, p);
-                    const int64_t gi = v * Nd + nn;
-                    s = s + lq[p] * a.Q[gi];
-                    t = t + gq[p] * (CURV ? A.Y[gi] / a.ngeom[ND * ND * Nd + nn] : A.Y[gi]);
+                if (first) {
+#pragma unroll
+                    for (int p = 0; p < N; ++p) t = t + gq[p] * pf_yl[p][v];
+                    ext[v] = pf_ext[v];
+                } else {
+                    for (int p = 0; p < N; ++p) {
+                        const int64_t nn = base + line_node<ND, N>(ndir, f, p);
+                        const int64_t gi = v * Nd + nn;
+                        s = s + lq[p] * a.Q[gi];
+                        t = t + gq[p] * (CURV ? A.Y[gi] / a.ngeom[ND * ND * Nd + nn] : A.Y[gi]);
+                    }
+                    ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, f, nlr ? N - 1 : 0)] : s;
                 }
-                ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, f, nlr ? N - 1 : 0)] : s;
                 const T Ln = CURV ? A.c * t : A.c * t / Jn;
                 Fb[v] = Fb[v] + (slave ? -Ln : Ln);  // -σ_nb L_nb, σ_nb = -σ_own
             }
@@ -348,12 +437,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     for (int v = 0; v < NV; ++v) {
         const int64_t idx = v * Nd + gi;
         T o = V[v];
-        if (A.P0) o = o + A.a0 * A.P0[idx];
-        if (A.P1) o = o + A.a1 * A.P1[idx];
+        if (A.P0) o = o + A.a0 * (PF ? pf_p0[v] : A.P0[idx]);
+        if (A.P1) o = o + A.a1 * (PF ? pf_p1[v] : A.P1[idx]);
         if (A.o1) {
             T x = A.b0 * o;
-            if (A.P1) x = x + A.b1 * A.P1[idx];
-            if (A.P2) x = x + A.b2 * A.P2[idx];
+            if (A.P1) x = x + A.b1 * (PF ? pf_p1[v] : A.P1[idx]);
+            if (A.P2) x = x + A.b2 * (PF ? pf_p2[v] : A.P2[idx]);
             A.o1[idx] = x;
         }
         A.o0[idx] = o;

@@ -311,15 +311,6 @@ bool no_gpu_checks(Lib& L) {
     return failures == 0;
 }
 
-double rel_max(const std::vector<double>& a, const std::vector<double>& b) {
-    double num = 0, den = 0;
-    for (size_t i = 0; i < a.size(); ++i) {
-        num = std::fmax(num, std::fabs(a[i] - b[i]));
-        den = std::fmax(den, std::fabs(b[i]));
-    }
-    return num / den;
-}
-
 void advection(Lib& L) {
     Ops o = make_ops(5, false);
     Mesh1D m = periodic_mesh(20, 1.0, o);
