@@ -34,6 +34,8 @@ VJP_CASES = [
     (2, 5, [3, 3], "split", "kg", "lxf", "euler", "GLL", None),
     (2, 4, [3, 3], "split", "std", "avg", "euler", "GLL", None),
     (2, 4, [3, 3], "split", "ch", "lxf", "euler", "GLL", "walls"),
+    (2, 6, [3, 3], "split", "ch", "lxf", "euler", "GLL", "walls"),  # up-front loads, boundaries
+    (3, 6, [2, 1, 2], "split", "ch", "lxf", "euler", "GLL", None),  # up-front loads, 3-D
     (3, 4, [2, 2, 2], "split", "ch", "lxf", "euler", "GLL", None),
     (3, 3, [2, 2, 2], "strong", "std", "lxf", "euler", "GL", None),
     (3, 5, [2, 1, 2], "split", "kg", "lxf", "euler", "GLL", None),

@@ -311,9 +311,11 @@ def test_rhs_is_deterministic():
     assert torch.equal(a, b)
 
 
-def test_non_periodic_boundaries_match_oracle():
-    """Inflow / outflow / slip boundaries on a 2-D box (Equations/Euler.jl:69-94)."""
-    mesh, std, equation, op, orc = build(2, 4, [3, 3], "split", "ch", periodic=[0, 0])
+@pytest.mark.parametrize("n", [4, 6])
+def test_non_periodic_boundaries_match_oracle(n):
+    """Inflow / outflow / slip boundaries on a 2-D box (Equations/Euler.jl:69-94).  n = 6:
+    the FP64 split kernel that loads each thread's first face node before staging."""
+    mesh, std, equation, op, orc = build(2, n, [3, 3], "split", "ch", periodic=[0, 0])
     qin = np.array([1.1, 0.3, 0.0, 2.7])
     bcs = {"1": F.EulerInflowBC(qin), "2": F.EulerOutflowBC(), "3": F.EulerSlipBC(), "4": F.EulerSlipBC()}
     disc = F.MultielementDisc(mesh, std, equation, op, bcs)
