@@ -246,8 +246,8 @@ def main():
                          "scaling FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(one GPU per run; multi-GPU sharding is not part of this build)", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(one GPU per run; the element-slab sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     import torch
     import flouhip as F

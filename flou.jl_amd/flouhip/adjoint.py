@@ -1,7 +1,7 @@
 """Adjoint of rhs! and of the RK time loop, with checkpointed recomputation.
 
-Flou has no AD rules; the north-star binding puts a ChainRulesCore ``rrule`` around
-``rhs!`` and ``timeintegrate``.  Here the same rules are ``torch.autograd.Function``s and
+Flou has no AD rules; its Julia binding puts a ChainRulesCore ``rrule`` around ``rhs!``
+and ``timeintegrate``.  Here the same rules are ``torch.autograd.Function``s and
 plain functions over the C-ABI (include/flouhip.h):
 
 * ``rhs_vjp``        -- Qbar += (d rhs/dQ)^T dQbar            (flou_hip_rhs_vjp)

@@ -1,6 +1,6 @@
 // Flou.jl DGSEM oracle -- CPU restatement of the reference algorithm.
 //
-// TEST INFRASTRUCTURE ONLY.  This file is the checker for the HIP path: only tests/,
+// TEST INFRASTRUCTURE ONLY.  This file is what the HIP path is compared with: only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The product
 // library (libflouhip.so) never links or calls it.
 //

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
 Test infrastructure: only the tests, ``__graft_entry__.smoke()`` and bench.py's
-``cpu_baseline`` leg use it, always as the checker.
+``cpu_baseline`` leg use it, always as the reference the HIP results are compared with.
 """
 import ctypes
 import os

@@ -1,4 +1,4 @@
-"""Pins of the CPU oracle (the checker of the HIP path).
+"""Pins of the CPU oracle (the reference the HIP path is compared with).
 
 Each check is chosen so that a plausible mistake (a dropped term, a wrong sign, a
 transposed operand, a wrong index) fails it:
