@@ -161,13 +161,13 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     const int64_t Nd = a.ndofs;
     const T avel[3] = {a.a0, a.a1, a.a2};
 
-    // FP64 split form, Euler, 6 or more nodes per direction (occupancy is set by LDS there,
-    // so the registers are free): the face node this thread takes first in the face phase
+    // Split form, Euler, 6 or more nodes per direction, FP64, and FP32 except 3-D N=5 (the
+    // registers fit without lowering the occupancy there): the face node this thread takes first in the face phase
     // (it = tid) has its face tables loaded before the element's nodes, and on GLL nodes
     // (gather) its neighbour trace, face frame and surface J after them, together with the
     // stage epilogue's second operand (RK register or SSP base) of this thread's node.  All
     // are then in flight during the staging instead of after it.
-    constexpr bool PF = std::is_same<T, double>::value && EQ == EQ_EULER && VOL == VOL_SPLIT && N >= 6;
+    constexpr bool PF = EQ == EQ_EULER && VOL == VOL_SPLIT && N >= 6 && (std::is_same<T, double>::value || !(ND == 3 && N == 6));
     constexpr int NFN = EPB * NF * NPF;  // face nodes of the block
     const int pfk = (tid % (NF * NPF)) / NPF, pff = tid % NPF, pfe = e0 + tid / (NF * NPF);
     const bool pf = PF && tid < NFN && pfe < a.ne;
