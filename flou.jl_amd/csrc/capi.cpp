@@ -351,7 +351,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         for (int k = 1; k < nd; ++k) a2 = a2 + h->a[k] * h->a[k];
         h->anorm = std::sqrt(a2);
     }
-    h->typo = 0;
+    h->typo = 1;  // as the reference computes (option 0)
 
     // connectivity checks and per element-face tables
     const int64_t ne = h->ne, nf = h->nf;
@@ -839,7 +839,11 @@ int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double 
     if (scheme == FLOU_HIP_SSPRK33) {
         if (stage < 0 || stage > 2) return fail(h, FLOU_HIP_EINVAL, "SSPRK33 has stages 0..2");
         if (stage > 0 && !u_base) return fail(h, FLOU_HIP_EINVAL, "u_base missing");
-        if (stage == 1 && overlaps(u_base, u_out, bytes)) return fail(h, FLOU_HIP_EINVAL, "u_out overlaps u_base");
+        // stage 2 may update u_base in place (u_out == u_base, point-wise); any other
+        // overlap would let one workgroup read what another has written
+        if (overlaps(u_base, u_out, bytes) && (stage == 1 || u_base != u_out))
+            return fail(h, FLOU_HIP_EINVAL, "u_out overlaps u_base");
+        if (overlaps(u_base, u_in, bytes) && u_base != u_in) return fail(h, FLOU_HIP_EINVAL, "u_base overlaps u_in");
         const int mode = stage == 0 ? MODE_SSP1 : (stage == 1 ? MODE_SSP2 : MODE_SSP3);
         int rc = launch_stage(h, mode, 0, dt, 0.0, 0.0, u_in, u_base, nullptr, u_out);
         if (!rc && h->lim_kind != FLOU_HIP_LIMITER_NONE) rc = apply_limiter(h, u_out, h->lim_min);
@@ -986,6 +990,10 @@ int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t
     for (int s = 0; s < S; ++s)
         if (!stages[s]) return fail(h, FLOU_HIP_EINVAL, "stage buffer missing");
     const size_t bytes = flou_hip_state_bytes(h);
+    // every reverse stage reads stages[s] for neighbour traces while writing the result:
+    // ubar must share no byte with a stage buffer
+    for (int s = 0; s < S; ++s)
+        if (overlaps(ubar, stages[s], bytes)) return fail(h, FLOU_HIP_EINVAL, "ubar overlaps a stage buffer");
     for (int i = 0; i < 3; ++i)
         if (!h->d_work[i]) HIPCHECK(h, hipMalloc(&h->d_work[i], bytes));
     void* X = h->d_work[0];

@@ -153,8 +153,11 @@ const char* flou_hip_create_error(void);
 
 /* Options.
  *  0: reproduce two 3-D surface-flux typos of src/FlouSpatial/Equations/Euler.jl: the
- *     ChandrasekharAverage term of line 216, which squares wl where wr is meant, and the
- *     ScalarDissipation left momenta of line 257, taken from Qr (default 0: corrected).
+ *     ChandrasekharAverage numerical flux of line 216 (used alone and as the average of
+ *     MatrixDissipation), which squares wl where wr is meant, and the ScalarDissipation
+ *     left momenta of line 257, taken from Qr.  Default 1: the reference's results;
+ *     0 gives the corrected (entropy-consistent, symmetric) fluxes.  1-D and 2-D runs
+ *     are unaffected.
  *  1: FP32 handles: 1 = always run the plain (reference-order) kernel, 0 = use the fast
  *     FP32 kernel where it applies (default).
  *  2: query, returns 1 if this FP32 handle has the fast kernel, else 0 (value ignored). */
@@ -190,7 +193,8 @@ int flou_hip_rhs(flou_hip_handle* h, void* dQ, const void* Q, double t);
  *            stage 2: u_out = (u_base + 2*u_in + 2*dt*L(u_in)) / 3       (u_out may be u_base)
  *   CK2N54 / ORK256, stage s: reg = A_s*reg + dt*L(u_in) (reg = dt*L(u_in) at s = 0);
  *            u_out = u_in + B_s*reg                                       (u_base unused)
- * u_out must not overlap u_in (neighbouring elements read u_in). */
+ * u_out must not overlap u_in (neighbouring elements read u_in); u_base must not overlap
+ * u_out except as the exact alias allowed at SSPRK33 stage 2 (EINVAL otherwise). */
 int flou_hip_rk_stage(flou_hip_handle* h, int32_t scheme, int32_t stage, double dt, double t,
                       const void* u_in, const void* u_base, void* reg, void* u_out);
 

@@ -5,6 +5,8 @@
 
 #include <map>
 
+#include <omp.h>
+
 namespace oracle {
 
 // ------------------------------------------------------------------------------------
@@ -839,6 +841,10 @@ int oracle_rk_run_jvp(oracle_handle* h, int scheme, double* u, double* du, doubl
     return 0;
 }
 
+// OpenMP team size of the calling thread's later parallel regions (a per-thread setting:
+// callers that run several oracle handles from their own threads set 1).
+void oracle_set_threads(int n) { omp_set_num_threads(n < 1 ? 1 : n); }
+
 // Point-wise exports for closed-form checks of the flux formulas.
 double oracle_logmean(double a, double b) { return logarithmic_mean(a, b); }
 
@@ -874,6 +880,18 @@ int oracle_numflux(oracle_handle* h, const double* Ql, const double* Qr, const d
         h->err = e.what();
         return -5;
     }
+    return 0;
+}
+
+// Exterior state that boundary `ib` gives an interior trace Qin at a face node with
+// frame (n, t, b) (the point form of applyBCs!, Interfaces.jl:25-49).  GenericBC
+// (per-node data) has no point form here: -1.
+int oracle_bc_point(oracle_handle* h, int ib, const double* Qin, const double* frame, double* Qext) {
+    const Disc& d = h->d;
+    if (ib < 0 || ib >= (int)d.bdnames.size()) return -1;
+    const int type = d.bctype[ib];
+    if (type == BC_DIRICHLET) return -1;
+    bc_exterior(d.nd, d.nv, type, type == BC_INFLOW ? d.bcdata[ib].data() : nullptr, Qin, frame, Qext);
     return 0;
 }
 

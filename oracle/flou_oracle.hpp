@@ -768,26 +768,33 @@ struct Work {
     std::vector<T> Qf, Fn;  // [face][side][f][v]
 };
 
+// Exterior state of one boundary face node (applyBCs!, Interfaces.jl:25-49, calling
+// bc(Qin, x, frame, t, eq)): `fixed` is the per-node state of a GenericBC (DIRICHLET) or
+// the constant Qext of an EulerInflowBC; fr is the face frame (n, t, b) at the node.
+template <class T>
+void bc_exterior(int nd, int nv, int type, const double* fixed, const T* Qin, const double* fr, T* Qext) {
+    if (type == BC_DIRICHLET || type == BC_INFLOW) {
+        // GenericBC with a time-independent Qext(x); EulerInflowBC (Equations/Euler.jl:69-80)
+        for (int v = 0; v < nv; ++v) Qext[v] = T(fixed[v]);
+    } else if (type == BC_OUTFLOW) {  // EulerOutflowBC (Equations/Euler.jl:82-86)
+        for (int v = 0; v < nv; ++v) Qext[v] = Qin[v];
+    } else {  // EulerSlipBC (Equations/Euler.jl:88-94)
+        T R[5];
+        rotate2face(nd, Qin, fr, R);
+        R[1] = -R[1];
+        rotate2phys(nd, R, fr, Qext);
+    }
+}
+
 template <class T>
 void apply_bc(const Disc& d, const T* Qin, long face, int fnode, T* Qext) {
     const int nv = d.nv;
     const int ib = d.f_bd[face];
     const int type = d.bctype.at(ib);
-    const double* fr = d.frame(face, fnode);
-    if (type == BC_DIRICHLET) {
-        // exterior state fixed per face node (GenericBC with a time-independent Qext)
-        const double* s = &d.f_bcstate[((size_t)face * d.npf + fnode) * nv];
-        for (int v = 0; v < nv; ++v) Qext[v] = T(s[v]);
-    } else if (type == BC_INFLOW) {
-        for (int v = 0; v < nv; ++v) Qext[v] = T(d.bcdata[ib][v]);
-    } else if (type == BC_OUTFLOW) {
-        for (int v = 0; v < nv; ++v) Qext[v] = Qin[v];
-    } else {  // BC_SLIP (Equations/Euler.jl:88-94)
-        T R[5];
-        rotate2face(d.nd, Qin, fr, R);
-        R[1] = -R[1];
-        rotate2phys(d.nd, R, fr, Qext);
-    }
+    const double* fixed = type == BC_DIRICHLET ? &d.f_bcstate[((size_t)face * d.npf + fnode) * nv]
+                          : type == BC_INFLOW  ? d.bcdata[ib].data()
+                                               : nullptr;
+    bc_exterior(d.nd, nv, type, fixed, Qin, d.frame(face, fnode), Qext);
 }
 
 template <class T>

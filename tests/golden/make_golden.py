@@ -134,9 +134,78 @@ def subcell_forms():
                      state="random_euler_state(X, 2, seed=17, amp=0.2)")
 
 
+C4_SUB = dict(nel=8, n=4, steps=200, seeds=(101, 202), cfl=0.1)
+
+
+def c4_problem():
+    """The C4 loss on its 8x8, N=3 sub-size: 2-D Euler on [0,2]^2 periodic, GLL 4, split
+    Chandrasekhar + LxF(StdAverage, 1), SSPRK33 at dt = 0.1 get_max_dt(Q_A, 1) for 200
+    steps; L(Q_A) = Σ Jω (ρ_A(T) − ρ_B(T))² with Q_A, Q_B seeded smooth periodic states."""
+    from cases import periodic_euler_state
+    c = C4_SUB
+    nel, n = c["nel"], c["n"]
+
+    def make():
+        return O.OracleDisc(2, n, [nel, nel], [0.0, 0.0], [2.0, 2.0], [1, 1], O.build_ops(n, "GLL"), O.EQ_EULER,
+                            [GAMMA], O.VOL_SPLIT, O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+    d = make()
+    X, _ = d.coords()
+    QA = periodic_euler_state(X, 2, 2.0, seed=c["seeds"][0], amp=0.1)
+    QB = periodic_euler_state(X, 2, 2.0, seed=c["seeds"][1], amp=0.1)
+    dt = c["cfl"] * d.max_dt(QA, 1.0)
+    w1 = O.build_ops(n, "GLL")["w"]
+    wnode = np.tile(np.kron(w1, w1) * (2.0 / nel / 2.0) ** 2, nel * nel)
+    rhoB = d.rk_run(O.RK_SSPRK33, QB, dt, c["steps"])[:, 0]
+    uA = d.rk_run(O.RK_SSPRK33, QA, dt, c["steps"])
+    dLdu = np.zeros_like(uA)
+    dLdu[:, 0] = 2.0 * wnode * (uA[:, 0] - rhoB)
+    return make, QA, dt, wnode, rhoB, dLdu
+
+
+def c4_gradient_columns(make, QA, dt, dLdu, cols, workers=None):
+    """∂L/∂Q_A for the flat (column-major) indices `cols`, one dual-number JVP of the
+    200-step run per index (forward mode with one partial, as ForwardDiff with chunk 1);
+    the indices are shared out over threads, each with its own oracle handle."""
+    import concurrent.futures as cf
+    cols = list(cols)
+    workers = workers or min(16, os.cpu_count() or 1)
+    chunks = [cols[i::workers] for i in range(workers)]
+
+    def run(chunk):
+        O.set_threads(1)
+        d = make()
+        E = np.zeros(QA.shape, order="F")
+        flat = E.reshape(-1, order="F")
+        out = {}
+        for i in chunk:
+            flat[:] = 0.0
+            flat[i] = 1.0
+            _, du = d.rk_run_jvp(O.RK_SSPRK33, QA, E, dt, C4_SUB["steps"])
+            out[i] = float(np.sum(dLdu * du))
+        return out
+
+    res = {}
+    with cf.ThreadPoolExecutor(max_workers=workers) as ex:
+        for part in ex.map(run, [c for c in chunks if c]):
+            res.update(part)
+    return np.array([res[i] for i in cols])
+
+
+def c4_gradient():
+    """C4 sub-size: every component of the loss gradient from dual-number JVPs."""
+    make, QA, dt, wnode, rhoB, dLdu = c4_problem()
+    g = c4_gradient_columns(make, QA, dt, dLdu, range(QA.size)).reshape(QA.shape, order="F")
+    return dict(QA=QA, dt=np.array(dt), wnode=wnode, rhoB=rhoB, grad=g), dict(
+        config="C4 sub-size", nd=2, n=C4_SUB["n"], nel=[C4_SUB["nel"]] * 2, domain=[[0, 0], [2, 2]], gamma=GAMMA,
+        operator="SplitDivOperator(ChandrasekharAverage(), LxF(StdAverage(), 1.0))", scheme="SSPRK33",
+        steps=C4_SUB["steps"], cfl=C4_SUB["cfl"],
+        states=f"periodic_euler_state(X, 2, 2.0, seed={C4_SUB['seeds']}, amp=0.1)",
+        loss="sum(wnode * (rho_A(T) - rhoB)^2)", gradient="oracle dual-number JVP per component")
+
+
 FIXTURES = {"c1_advection_2d_n4.npz": c1_advection, "euler3d_n5_2x2x2.npz": euler3d_n5,
             "c2_kennedy_gruber_n7_4x4.npz": c2_kennedy_gruber, "c5_curved_n6_2x2x2.npz": c5_curved,
-            "subcell_forms_2d.npz": subcell_forms}
+            "subcell_forms_2d.npz": subcell_forms, "c4_gradient_8x8_n3.npz": c4_gradient}
 
 
 def generate(name):
@@ -145,8 +214,8 @@ def generate(name):
     return {k: np.asarray(v, dtype=np.float64) for k, v in arrays.items()}, meta
 
 
-def main():
-    for name in FIXTURES:
+def main(names=None):
+    for name in names or FIXTURES:
         arrays, meta = generate(name)
         path = os.path.join(HERE, name)
         np.savez_compressed(path, meta=np.array(json.dumps(meta, sort_keys=True)), **arrays)
@@ -154,4 +223,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

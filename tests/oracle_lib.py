@@ -66,6 +66,8 @@ def lib():
         L.oracle_logmean.argtypes = [ctypes.c_double, ctypes.c_double]
         L.oracle_twopoint.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oracle_numflux.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp]
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_bc_point.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.oracle_cons2entropy.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_entropy2cons.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp]
         L.oracle_interp_matrix.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
@@ -222,6 +224,15 @@ class OracleDisc:
         self._check(lib().oracle_numflux(self.h, *[_p(x) for x in a], _p(F)))
         return F
 
+    def bc_point(self, ib, Qin, frame):
+        """Exterior state boundary ib assigns to the trace Qin at a face node with frame
+        (n, t, b) (inflow, outflow and slip boundaries)."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (Qin, frame)]
+        Q = np.zeros(self.nv)
+        if lib().oracle_bc_point(self.h, int(ib), _p(a[0]), _p(a[1]), _p(Q)) != 0:
+            raise ValueError("no point form for this boundary")
+        return Q
+
     def max_dt(self, Q, cfl):
         Q = np.asfortranarray(Q, dtype=np.float64)
         return lib().oracle_max_dt(self.h, _p(Q), float(cfl))
@@ -240,6 +251,11 @@ class OracleDisc:
         du = np.array(dQ, dtype=np.float64, order="F", copy=True)
         self._check(lib().oracle_rk_run_jvp(self.h, scheme, _p(u), _p(du), float(dt), int(nsteps)))
         return u, du
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's loops started from the calling thread."""
+    lib().oracle_set_threads(int(n))
 
 
 def logmean(a, b):

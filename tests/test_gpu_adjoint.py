@@ -16,7 +16,7 @@ import pytest
 import flouhip as F
 import oracle_lib as O
 from flouhip import adjoint as A
-from cases import periodic_euler_state
+from cases import golden, periodic_euler_state
 from test_gpu_parity import _state, build
 
 pytestmark = pytest.mark.gpu
@@ -43,6 +43,13 @@ VJP_CASES = [
     (2, 5, [4, 4], "split", "std", "lxf", "adv", "GLL", None),
     (2, 4, [4, 3], "strong", "std", "lxf", "adv", "GL", None),
 ]
+
+
+def _find(nd, n, vol, tp, bcs, eq="euler"):
+    """The VJP case with these values (picked by value, so inserting a case moves nothing)."""
+    hits = [c for c in VJP_CASES if (c[0], c[1], c[3], c[4], c[6], c[8]) == (nd, n, vol, tp, eq, bcs)]
+    assert len(hits) == 1, hits
+    return hits[0]
 
 
 def _disc(case, precision="fp64"):
@@ -100,7 +107,7 @@ def test_rhs_vjp_dot_product_with_oracle_jvp(case):
 
 
 def test_rhs_vjp_autograd_function():
-    case = VJP_CASES[5]
+    case = _find(2, 6, "split", "ch", None)
     disc, orc, equation = _disc(case)
     Q = _state(disc, orc, 2, "euler", seed=3)
     Qd = disc.to_device(Q).requires_grad_(True)
@@ -116,8 +123,17 @@ def test_rhs_vjp_autograd_function():
 SCHEMES = [(F.SSPRK33, O.RK_SSPRK33), (F.CarpenterKennedy2N54, O.RK_CK2N54), (F.ORK256, O.RK_ORK256)]
 
 
+RK_VJP_CASES = {
+    "2d-ch": _find(2, 4, "split", "ch", None),
+    "3d-ch": _find(3, 4, "split", "ch", None),
+    "3d-ch-n6": _find(3, 6, "split", "ch", None),    # FP64 3-D N=6: epilogue operands loaded early
+    "2d-walls": _find(2, 4, "split", "ch", "walls"),
+    "2d-walls-n6": _find(2, 6, "split", "ch", "walls"),
+}
+
+
 @pytest.mark.parametrize("scheme", SCHEMES, ids=lambda s: s[0].__name__)
-@pytest.mark.parametrize("case", [VJP_CASES[4], VJP_CASES[9], VJP_CASES[8]], ids=["2d-ch", "3d-ch", "2d-walls"])
+@pytest.mark.parametrize("case", list(RK_VJP_CASES.values()), ids=list(RK_VJP_CASES))
 def test_rk_steps_vjp_dot_product_with_oracle_jvp(case, scheme):
     solver, ocode = scheme
     nd = case[0]
@@ -219,6 +235,31 @@ def test_adjoint_full_gradient_matches_dual_numbers():
     err = np.max(np.abs(g - ref)) / np.max(np.abs(ref))
     print(f"full gradient vs dual numbers: max rel err {err:.2e}")
     assert err <= 1e-11
+
+
+def test_c4_subsize_gradient_matches_dual_number_fixture():
+    """C4 sub-size (8x8, N=3, 200 SSPRK33 steps, checkpoints every 20): every component of
+    the gradient against the oracle's dual-number JVPs stored in
+    tests/golden/c4_gradient_8x8_n3.npz (the ForwardDiff gradient of the reference path)."""
+    g, meta = golden("c4_gradient_8x8_n3.npz")
+    mesh, std, equation, op, orc = build(2, meta["n"], meta["nel"], "split", "ch", "lxf", finish=[2.0, 2.0])
+    disc = F.MultielementDisc(mesh, std, equation, op)
+    wdev = torch.tensor(g["wnode"], device="cuda")
+    rdev = torch.tensor(g["rhoB"], device="cuda")
+
+    def seed(u):
+        s = torch.zeros_like(u)
+        s[0] = 2.0 * wdev * (u[0] - rdev)
+        return s
+
+    grad = disc.to_host(A.gradient(disc, F.SSPRK33, disc.to_device(g["QA"]), float(g["dt"]), meta["steps"], seed,
+                                   checkpoint_every=20))
+    ref = g["grad"]
+    err = np.max(np.abs(grad - ref)) / np.max(np.abs(ref))
+    l2 = np.linalg.norm(grad - ref) / np.linalg.norm(ref)
+    print(f"C4 8x8 N=3, 200 steps: gradient vs dual numbers max rel {err:.2e}, rel L2 {l2:.2e} "
+          f"({ref.size} components)")
+    assert err <= 1e-10
 
 
 def test_fp32_adjoint_against_fp64_adjoint():

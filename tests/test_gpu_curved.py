@@ -69,7 +69,7 @@ def test_curved_rhs_fp64_matches_oracle(case):
     assert err <= 1e-13
 
 
-@pytest.mark.parametrize("case", [CASES[0], CASES[3]], ids=["2d", "3d"])
+@pytest.mark.parametrize("case", [CASES[0], CASES[3], (3, 7, 2, "split", "ch")], ids=["2d", "3d", "3d-n7-ch"])
 def test_curved_rhs_fp32_against_fp64_oracle(case):
     nd, n, nel, vol, tp = case
     disc, orc, eq, X, J, std = build_curved(nd, n, nel, vol, tp, precision="fp32")
@@ -79,6 +79,20 @@ def test_curved_rhs_fp32_against_fp64_oracle(case):
     F.rhs_(dQ, disc.to_device(Q), F.EquationConfig(disc, eq))
     err = rel_err(disc.to_host(dQ).astype(np.float64), orc.rhs(Q))
     print(f"FP32 curved {case}: max rel err {err:.2e}")
+    assert err <= 2e-5
+
+
+def test_curved_fp32_ck2n54_steps_against_fp64_oracle():
+    """3-D N=6 curved, FP32 fused CK2N54 steps (the register operand loaded before the
+    staging barrier) against the FP64 oracle's steps."""
+    disc, orc, eq, X, J, std = build_curved(3, 7, 2, precision="fp32")
+    Q = periodic_euler_state(X, 3, 1.0, seed=5, amp=0.2)
+    dt = 0.1 * orc.max_dt(Q, 1.0)
+    ref = orc.rk_run(O.RK_CK2N54, Q, dt, 3)
+    uT, _ = A.forward(disc, F.CarpenterKennedy2N54, disc.to_device(Q), dt, 3)
+    got = disc.to_host(uT)
+    err = rel_err(got.astype(np.float64), ref)
+    print(f"FP32 curved CK2N54 x3: max rel err {err:.2e}")
     assert err <= 2e-5
 
 

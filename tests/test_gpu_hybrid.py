@@ -41,7 +41,8 @@ def build(nd, n, nel, tp, savg, sdiss, blend, precision="fp64", nodes="GLL"):
     orc = O.OracleDisc(nd, n, [nel] * nd, [0.0] * nd, finish, [1] * nd, O.build_ops(n, nodes), O.EQ_EULER,
                        [GAMMA], O.VOL_HYBRID, tp, savg, sdiss, 1.0)
     orc.set_blend(blend)
-    orc.set_option(0, 0)
+    orc.set_option(0, 0)  # corrected 3-D surface fluxes on both sides
+    disc.set_option(0, 0)
     return disc, eq, orc
 
 
