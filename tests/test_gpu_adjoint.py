@@ -139,7 +139,9 @@ def test_rk_steps_vjp_dot_product_with_oracle_jvp(case, scheme):
     nd = case[0]
     disc, orc, equation = _disc(case)
     Q = _state(disc, orc, nd, "euler", seed=9, amp=0.2)
-    dt = 0.25 * orc.max_dt(Q, 1.0)
+    # CFL 0.1: at 0.25 the 3-D N=5 state of the [2, 1, 2] mesh turns non-physical within
+    # 5 SSPRK33/ORK256 steps (in the oracle as well)
+    dt = 0.1 * orc.max_dt(Q, 1.0)
     nsteps = 5
     rng = np.random.default_rng(2)
     V = rng.standard_normal(Q.shape)

@@ -233,25 +233,29 @@ def test_c4_full_size_adjoint_gradient_against_finite_differences():
     L0 = loss(QA)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(3)
-    h = 1e-6
-    fds, ads = [], []
+    dirs = []
     for k in range(16):
         V = torch.randn(QA.shape, generator=gen, device="cuda", dtype=torch.float64)
-        V /= torch.linalg.norm(V)
-        fds.append((loss(QA + h * V) - loss(QA - h * V)) / (2 * h))
-        ads.append(float((grad * V).sum()))
-    fds, ads = np.array(fds), np.array(ads)
-    # the round-off of L itself (~ eps·L, summed over 2.36M values) divided by 2h bounds
-    # what a central difference can resolve along a unit direction
-    floor = 2.2e-16 * L0 / h
-    rel = np.abs(fds - ads) / np.abs(ads)
+        dirs.append(V / torch.linalg.norm(V))
+    ads = np.array([float((grad * V).sum()) for V in dirs])
+    fd = {h: np.array([(loss(QA + h * V) - loss(QA - h * V)) / (2 * h) for V in dirs]) for h in (1e-6, 1e-4)}
     for k in range(16):
-        print(f"C4 128^2 direction {k:2d}: adjoint {ads[k]: .10e}  FD {fds[k]: .10e}  rel {rel[k]:.2e}")
-    agg = np.linalg.norm(fds - ads) / np.linalg.norm(ads)
-    print(f"C4 128^2, 200 steps: 16 unit directions, h = {h:g}: max rel {rel.max():.2e}, median "
-          f"{np.median(rel):.2e}, aggregate {agg:.2e}; |∇L·V| ~ {np.sqrt(np.mean(ads ** 2)):.2e}, "
-          f"eps·L/h {floor:.1e}; adjoint pass {t_adj:.2f} s")
-    assert agg <= 1e-6
+        print(f"C4 128^2 direction {k:2d}: adjoint {ads[k]: .10e}  FD(1e-6) {fd[1e-6][k]: .10e}  "
+              f"FD(1e-4) {fd[1e-4][k]: .10e}")
+    rel = {h: np.abs(f - ads) / np.abs(ads) for h, f in fd.items()}
+    agg = {h: np.linalg.norm(f - ads) / np.linalg.norm(ads) for h, f in fd.items()}
+    # Along a unit direction of 2.36 M components each value moves by ~h/1536, so the
+    # central difference resolves ∇L·V only to the round-off of L itself: eps·L/(2h) per
+    # evaluation pair, ~1.5e-12 at h = 1e-6 against derivatives of ~4e-7.  At h = 1e-4 that
+    # floor is 100 times lower and the truncation (O(h²)) is still negligible.
+    floor = 2.2e-16 * L0 / 1e-6
+    for h in (1e-6, 1e-4):
+        print(f"C4 128^2, 200 steps: 16 unit directions, h = {h:g}: max rel {rel[h].max():.2e}, median "
+              f"{np.median(rel[h]):.2e}, aggregate {agg[h]:.2e}")
+    print(f"C4 128^2: |∇L·V| ~ {np.sqrt(np.mean(ads ** 2)):.2e}, FD round-off floor at h = 1e-6 "
+          f"eps·L/h = {floor:.1e}; adjoint pass {t_adj:.2f} s")
+    assert np.all(np.abs(fd[1e-6] - ads) <= 1e-6 * np.abs(ads) + 2 * floor)
+    assert agg[1e-4] <= 1e-6 and rel[1e-4].max() <= 1e-6
 
 
 @pytest.mark.timeout(900)
