@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,6 +59,9 @@ struct flou_hip_handle {
     std::vector<double> bcstate, bcconst;
     // time-loop work buffers
     void* d_work[3] = {nullptr, nullptr, nullptr};
+    // this device's address of the static scratch array (flou_hip_rk_step_vjp_noalloc)
+    void* d_scratch = nullptr;
+    std::string scratch_why;
     // kernels
     RhsLauncherF64 l64 = nullptr;
     RhsLauncherF32 l32 = nullptr;
@@ -74,7 +78,30 @@ struct flou_hip_handle {
     std::string err;
 };
 
+// Work states of flou_hip_rk_step_vjp_noalloc: three state buffers of one reverse RK step
+// (FLOU_HIP_NOALLOC_SCRATCH_BYTES).  Each GPU that the library runs on holds its own copy
+// until the process ends.  In host code `scratch` only names the symbol: kernels receive
+// the device address that hipGetSymbolAddress returns for the current device.
+static __device__ __align__(256) unsigned char scratch[FLOU_HIP_NOALLOC_SCRATCH_BYTES];
+
 namespace {
+
+// Device address of `scratch` on `device` (which must be current), looked up once per device.
+hipError_t scratch_address(int device, void** out) {
+    static std::mutex mu;
+    static std::vector<void*> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (device < 0) return hipErrorInvalidDevice;
+    if ((size_t)device >= addr.size()) addr.resize(device + 1, nullptr);
+    if (!addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[device] = p;
+    }
+    *out = addr[device];
+    return hipSuccess;
+}
 
 int fail(flou_hip_handle* h, int code, const std::string& msg) {
     if (h) h->err = msg;
@@ -605,6 +632,15 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
 
     // device uploads
     if (hipGetDevice(&h->device) != hipSuccess) return bad(FLOU_HIP_EDEVICE, "no HIP device");
+    {
+        // the allocation-free adjoint step's work memory on this device (a failure here only
+        // disables that entry point)
+        const hipError_t e = scratch_address(h->device, &h->d_scratch);
+        if (e != hipSuccess) {
+            h->d_scratch = nullptr;
+            h->scratch_why = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        }
+    }
     if (h->lfast) {
         int rcf = upload(h, &h->d_fegeom, fe);
         rcf |= upload(h, &h->d_fdsharp, fds);
@@ -979,26 +1015,28 @@ int flou_hip_rk_step_record(flou_hip_handle* h, int32_t scheme, double dt, doubl
     return rc;
 }
 
-int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t,
-                         const void* const* stages, void* ubar) {
-    (void)t;
+namespace {
+// Checks of the two adjoint-step entry points; S = number of stages.
+int check_step_vjp(flou_hip_handle* h, int32_t scheme, const void* const* stages, void* ubar, int* S) {
     if (!h || !stages || !ubar) return FLOU_HIP_EINVAL;
     if (h->lim_kind != FLOU_HIP_LIMITER_NONE)
         return fail(h, FLOU_HIP_EUNSUPPORTED, "adjoint: the stage limiter is not differentiated");
     if (scheme < FLOU_HIP_SSPRK33 || scheme > FLOU_HIP_ORK256) return fail(h, FLOU_HIP_EINVAL, "unknown scheme");
-    const int S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
-    for (int s = 0; s < S; ++s)
+    *S = scheme == FLOU_HIP_SSPRK33 ? 3 : 5;
+    for (int s = 0; s < *S; ++s)
         if (!stages[s]) return fail(h, FLOU_HIP_EINVAL, "stage buffer missing");
     const size_t bytes = flou_hip_state_bytes(h);
     // every reverse stage reads stages[s] for neighbour traces while writing the result:
     // ubar must share no byte with a stage buffer
-    for (int s = 0; s < S; ++s)
+    for (int s = 0; s < *S; ++s)
         if (overlaps(ubar, stages[s], bytes)) return fail(h, FLOU_HIP_EINVAL, "ubar overlaps a stage buffer");
-    for (int i = 0; i < 3; ++i)
-        if (!h->d_work[i]) HIPCHECK(h, hipMalloc(&h->d_work[i], bytes));
-    void* X = h->d_work[0];
-    void* Y2 = h->d_work[1];
-    void* Z = h->d_work[2];
+    return FLOU_HIP_OK;
+}
+
+// The reverse RK step over the work states X, Y2, Z (each one state, distinct from the
+// stage buffers and ubar; nothing is read from them before it is written).
+int step_vjp(flou_hip_handle* h, int32_t scheme, int S, double dt, const void* const* stages, void* ubar, void* X,
+             void* Y2, void* Z) {
     int rc;
     if (scheme == FLOU_HIP_SSPRK33) {
         // u1 = u + dt L(u); u2 = (3u + u1 + dt L(u1))/4; u3 = (u + 2u2 + 2dt L(u2))/3
@@ -1027,6 +1065,36 @@ int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t
     }
     if (!rc) rc = launch_vjp(h, stages[0], G[cur], dt, X, 1.0, nullptr, 0.0, ubar, nullptr, 0.0, 0.0, 0.0, nullptr);
     return rc;
+}
+}  // namespace
+
+int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                         const void* const* stages, void* ubar) {
+    (void)t;
+    int S = 0;
+    const int rc = check_step_vjp(h, scheme, stages, ubar, &S);
+    if (rc) return rc;
+    const size_t bytes = flou_hip_state_bytes(h);
+    for (int i = 0; i < 3; ++i)
+        if (!h->d_work[i]) HIPCHECK(h, hipMalloc(&h->d_work[i], bytes));
+    return step_vjp(h, scheme, S, dt, stages, ubar, h->d_work[0], h->d_work[1], h->d_work[2]);
+}
+
+int flou_hip_rk_step_vjp_noalloc(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                                 const void* const* stages, void* ubar) {
+    (void)t;
+    int S = 0;
+    const int rc = check_step_vjp(h, scheme, stages, ubar, &S);
+    if (rc) return rc;
+    if (!h->d_scratch) return fail(h, FLOU_HIP_EDEVICE, "no scratch memory on this device: " + h->scratch_why);
+    const size_t bytes = flou_hip_state_bytes(h);
+    const size_t slot = (bytes + 255) / 256 * 256;  // each work state 256-byte aligned
+    if (3 * slot > (size_t)FLOU_HIP_NOALLOC_SCRATCH_BYTES)
+        return fail(h, FLOU_HIP_EINVAL,
+                    "rk_step_vjp_noalloc: three states of " + std::to_string(bytes) + " bytes exceed the " +
+                        std::to_string((size_t)FLOU_HIP_NOALLOC_SCRATCH_BYTES) + "-byte scratch buffer");
+    char* base = (char*)h->d_scratch;
+    return step_vjp(h, scheme, S, dt, stages, ubar, base, base + slot, base + 2 * slot);
 }
 
 }  // extern "C"

@@ -86,6 +86,8 @@ SYMBOLS = {
                                                ctypes.POINTER(_vp), _vp]),
     "flou_hip_rk_step_vjp": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
                                             ctypes.POINTER(_vp), _vp]),
+    "flou_hip_rk_step_vjp_noalloc": (ctypes.c_int, [_vp, ctypes.c_int32, ctypes.c_double, ctypes.c_double,
+                                            ctypes.POINTER(_vp), _vp]),
 }
 
 

@@ -7,6 +7,7 @@ plain functions over the C-ABI (include/flouhip.h):
 * ``rhs_vjp``        -- Qbar += (d rhs/dQ)^T dQbar            (flou_hip_rhs_vjp)
 * ``rk_step_record`` -- one RK step keeping its stage inputs   (flou_hip_rk_step_record)
 * ``rk_step_vjp``    -- cotangent of u_n from that of u_{n+1}  (flou_hip_rk_step_vjp)
+* ``rk_step_vjp_noalloc`` -- the same without device allocation (flou_hip_rk_step_vjp_noalloc)
 * ``gradient``       -- reverse sweep over nsteps fixed-dt steps: checkpoints every
   ``checkpoint_every`` steps during the forward run, then per segment one forward
   recomputation that keeps the stage inputs, followed by the reverse steps.
@@ -66,6 +67,19 @@ def rk_step_vjp(disc, solver, dt, stages, ubar, t=0.0):
     disc._sync_stream()
     C.check(C.lib.flou_hip_rk_step_vjp(disc.h, code, float(dt), float(t), _ptr_array(disc, stages),
                                        disc._dptr(ubar)), disc.h, "rk_step_vjp")
+    return ubar
+
+
+def rk_step_vjp_noalloc(disc, solver, dt, stages, ubar, t=0.0):
+    """rk_step_vjp with its three work states in the library's static per-GPU scratch
+    buffer (flou_hip_rk_step_vjp_noalloc): no device allocation, so the call can be
+    captured in a HIP graph.  Concurrent calls on one GPU race."""
+    code = _solver_code(solver)
+    if len(stages) != _nstages(code):
+        raise ValueError(f"{_nstages(code)} stage buffers expected")
+    disc._sync_stream()
+    C.check(C.lib.flou_hip_rk_step_vjp_noalloc(disc.h, code, float(dt), float(t), _ptr_array(disc, stages),
+                                               disc._dptr(ubar)), disc.h, "rk_step_vjp_noalloc")
     return ubar
 
 

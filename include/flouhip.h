@@ -276,6 +276,23 @@ int flou_hip_rk_step_record(flou_hip_handle* h, int32_t scheme, double dt, doubl
 int flou_hip_rk_step_vjp(flou_hip_handle* h, int32_t scheme, double dt, double t,
                          const void* const* stages, void* ubar);
 
+/* Bytes of the static device buffer behind flou_hip_rk_step_vjp_noalloc, per GPU. */
+#define FLOU_HIP_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* flou_hip_rk_step_vjp without device allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The three work states live in one static device buffer
+ * of FLOU_HIP_NOALLOC_SCRATCH_BYTES that each GPU holds for the life of the process, instead
+ * of handle-owned buffers allocated on first use; the call itself allocates nothing and does
+ * not synchronise.  Same arguments, results and checks as flou_hip_rk_step_vjp, plus
+ * FLOU_HIP_EINVAL when three states (each rounded up to 256 bytes) exceed the buffer: up
+ * to 22.3 MB per state, e.g. 699,050 nodes of FP64 2-D Euler (a 128 x 128 mesh at N = 5
+ * has 589,824) or 1.1 M nodes of FP32 3-D Euler.  Every call on
+ * one GPU shares that buffer, whatever the handle: calls that overlap in time on one GPU
+ * (other streams or host threads) race and give wrong results. */
+int flou_hip_rk_step_vjp_noalloc(flou_hip_handle* h, int32_t scheme, double dt, double t,
+                                 const void* const* stages, void* ubar);
+
 #ifdef __cplusplus
 }
 #endif
