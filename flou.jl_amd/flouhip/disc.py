@@ -40,9 +40,14 @@ class EquationConfig:
 
 class MultielementDisc:
     """GPU MultielementDisc.  `precision` is "fp64" (plain form) or "fp32" (FP32 state and
-    bulk arithmetic; dt and integrals stay FP64)."""
+    bulk arithmetic; dt and integrals stay FP64).  `source` is the reference's source-term
+    argument (MultielementDiscontinuous.jl:29-36, applied per node by apply_sourceterm!,
+    139-146): an arbitrary host function per node has no device form, so any source other
+    than None is refused."""
 
-    def __init__(self, mesh, std, equation, operator, bcs=None, precision="fp64"):
+    def __init__(self, mesh, std, equation, operator, bcs=None, source=None, precision="fp64"):
+        if source is not None:
+            raise NotImplementedError("source terms (apply_sourceterm!) are not available on the GPU path")
         bcs = dict(bcs or {})
         nd = mesh.nd
         if std.ND != nd or equation.nd != nd:

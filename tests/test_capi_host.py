@@ -230,3 +230,19 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(C, "LIB_PATH", str(tmp_path / "nope.so"))
     with pytest.raises(ImportError):
         C._load()
+
+
+def test_source_term_is_refused_explicitly():
+    """The reference's apply_sourceterm! (MultielementDiscontinuous.jl:139-146) runs a host
+    function per node; the GPU path refuses one instead of dropping it."""
+    b = F.LagrangeBasis("GLL", 3)
+    std = F.StdQuad(b, F.DGSEMrec(b), 4)
+    m = F.CartesianMesh(2, (0, 0), (1, 1), (2, 2)).apply_periodicBCs(("1", "2"), ("3", "4"))
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+
+    def source(dQ, Q, x, t):
+        dQ[0] += 1.0
+
+    with pytest.raises(NotImplementedError, match="source"):
+        F.MultielementDisc(m, std, F.EulerEquation(2, 1.4), op, {}, source)
+
