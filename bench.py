@@ -88,7 +88,7 @@ def run_steps(disc, Q, dt, nsteps, bufs):
 def time_form(precision, Q64, dt, steps, warmup):
     import torch
     disc, eq, std = build_disc(precision)
-    Q = Q64.to(disc.torch_dtype()).contiguous()
+    Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()
     bufs = [disc.new_state(), disc.new_state()]
     cur = run_steps(disc, Q, dt, warmup, bufs)
     if cur is not Q:
@@ -173,30 +173,163 @@ def adjoint_ratio(nel=128, n=6, nsteps=200, checkpoint_every=20, seed=(101, 202)
     disc.check()
     rhoB = uB[0].clone()
     stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
     Q = QA.clone()
     F.timeintegrate(Q.clone(), disc, eq, F.SSPRK33, 5 * dt, dt=dt)  # warm-up
     torch.cuda.synchronize()
     ev[0].record(stream)
     F.timeintegrate(Q, disc, eq, F.SSPRK33, nsteps * dt, dt=dt)
     ev[1].record(stream)
+    torch.cuda.synchronize()
+    # the gradient, end to end: checkpointed forward, loss cotangent, reverse sweep
+    ev[2].record(stream)
     uT, cps = A.forward(disc, F.SSPRK33, QA, dt, nsteps, checkpoint_every)
     ubar = torch.zeros_like(uT)
     ubar[0] = 2.0 * wnode * (uT[0] - rhoB)
-    torch.cuda.synchronize()
-    ev[2].record(stream)
-    A.reverse(disc, F.SSPRK33, cps, dt, nsteps, ubar, checkpoint_every)
     ev[3].record(stream)
+    A.reverse(disc, F.SSPRK33, cps, dt, nsteps, ubar, checkpoint_every)
+    ev[4].record(stream)
     torch.cuda.synchronize()
     disc.check()
-    fwd_ms, rev_ms = ev[0].elapsed_time(ev[1]), ev[2].elapsed_time(ev[3])
+    fwd_ms = ev[0].elapsed_time(ev[1])
+    cfwd_ms, rev_ms = ev[2].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4])
     gnorm = float(torch.linalg.norm(ubar))
-    return {"value": rev_ms / fwd_ms, "forward_ms": fwd_ms, "reverse_ms": rev_ms,
-            "gradient_norm": gnorm,
+    return {"value": (cfwd_ms + rev_ms) / fwd_ms, "forward_ms": fwd_ms,
+            "checkpointed_forward_ms": cfwd_ms, "reverse_ms": rev_ms,
+            "reverse_only_ratio": rev_ms / fwd_ms, "gradient_norm": gnorm,
             "workload": "2-D Euler periodic 128^2 quads on [0,2]^2, N=5, FP64, split Chandrasekhar + "
                         "LxF(StdAverage), SSPRK33 x 200 fixed-dt steps (CFL 0.15), L2 density misfit, "
-                        "checkpoints every 20 steps with recompute; value = reverse sweep "
-                        "(recompute + VJP) / forward"}
+                        "checkpoints every 20 steps with recompute; value = one gradient (checkpointed "
+                        "forward + reverse sweep with recompute and VJP) / one forward run"}
+
+
+def _ssprk33_steps(disc, Q, dt, nsteps):
+    """nsteps fused SSPRK33 steps in place on Q (two work states)."""
+    import flouhip as F
+    u1, u2 = disc.new_state(), disc.new_state()
+    for _ in range(nsteps):
+        F.rk_stage(disc, F.SSPRK33.code, 0, dt, Q, u1)
+        F.rk_stage(disc, F.SSPRK33.code, 1, dt, u1, u2, u_base=Q)
+        F.rk_stage(disc, F.SSPRK33.code, 2, dt, u2, Q, u_base=Q)
+    return Q
+
+
+SECONDARY = {
+    "c2": "2-D Euler, periodic 256^2 quads on [0,2]^2, N=7 (4,194,304 nodes), split KennedyGruber volume "
+          "flux + LxF(StdAverage) surface, SSPRK33 fused stages, fixed dt (CFL 0.1)",
+    "c5": "3-D Euler, periodic curved 32^3 hex (x + 0.05 sin 2πx sin 2πy sin 2πz on the unit cube), N=6 "
+          "(11,239,424 nodes, per-node metric and J), split ChandrasekharAverage volume flux + "
+          "LxF(StdAverage) surface, SSPRK33 fused stages, fixed dt (CFL 0.1)",
+}
+
+
+def secondary_workload(name, steps=None, warmup=2):
+    """C2: 2-D Euler on a periodic 256^2 quad mesh over [0,2]^2, N=7, split Kennedy-Gruber +
+    LxF(StdAverage), SSPRK33 (CFL 0.1 of the initial state), seeded state rho = 1 +
+    0.2 sin(π(x+y)) + 8 Fourier modes of 0.01, u, v = 0.5 + 0.05 smooth, p = 1.
+    C5: 3-D Euler on the curved periodic 32^3 mesh x + 0.05 sin 2πx sin 2πy sin 2πz of the
+    unit cube, N=6, split Chandrasekhar + LxF(StdAverage), SSPRK33 (CFL 0.1), seeded smooth
+    state rho in [0.9, 1.1], |u| <= 0.3, p in [0.9, 1.1].
+    FP32 and FP64 over the same steps from the same FP64 state: DOF-updates/s per stage,
+    average stage-kernel time against the HBM roofline of the algorithmic bytes (SSPRK33
+    stages move 2, 3 and 3 states of NV values; C5 adds the per-node metric and J, (d²+1)
+    values per node, to every stage), and the density L2 error of FP32 against FP64."""
+    import numpy as np
+    import torch
+    import flouhip as F
+    if name == "c2":
+        nd, n, nel, L, nv = 2, 8, 256, 2.0, 4
+        nsteps = steps or 50
+        op = F.SplitDivOperator(F.KennedyGruber(), F.LxF(F.StdAverage(), 1.0))
+        mapping = None
+    else:
+        nd, n, nel, L, nv = 3, 7, 32, 1.0, 5
+        nsteps = steps or 20
+        op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+
+        def mapping(X):
+            s = np.ones((X.shape[0], 1))
+            for k in range(3):
+                s = s * np.sin(2 * np.pi * X[:, [k]])
+            return X + 0.05 * s
+    b = F.LagrangeBasis("GLL", n)
+    std = [None, None, F.StdQuad, F.StdHex][nd](b, F.DGSEMrec(b), nv)
+    eq = F.EulerEquation(nd, GAMMA)
+    res, Q64, dt = {}, None, None
+    for form in ("fp64", "fp32"):
+        mesh = F.CartesianMesh(nd, (0.0,) * nd, (L,) * nd, (nel,) * nd)
+        mesh.apply_periodicBCs(*[(str(2 * k + 1), str(2 * k + 2)) for k in range(nd)])
+        if mapping is not None:
+            mesh.apply_mapping(mapping)
+        disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
+        if Q64 is None:
+            X = torch.tensor(mesh.node_coords(std), device="cuda").T
+            g = torch.Generator(device="cpu")
+            g.manual_seed(7)
+            if name == "c2":
+                x, y = X[0], X[1]
+                rho = 1 + 0.2 * torch.sin(math.pi * (x + y))
+                for _ in range(8):
+                    kx, ky = torch.randint(1, 4, (2,), generator=g).tolist()
+                    rho = rho + 0.01 / 8 * torch.sin(math.pi * (kx * x + ky * y) + float(torch.rand(1, generator=g)) * 6.283)
+                ph = (torch.rand(2, generator=g) * 6.283).tolist()
+                u = 0.5 + 0.05 * torch.sin(math.pi * x + ph[0]) * torch.cos(math.pi * y)
+                v = 0.5 + 0.05 * torch.cos(math.pi * x) * torch.sin(math.pi * y + ph[1])
+                prim = torch.stack([rho, u, v, torch.ones_like(x)])
+            else:
+                ph = (torch.rand(5, 3, generator=g) * 6.283).tolist()
+                modes = [torch.sin(2 * math.pi * X[0] + ph[v][0]) * torch.sin(2 * math.pi * X[1] + ph[v][1])
+                         * torch.sin(2 * math.pi * X[2] + ph[v][2]) for v in range(5)]
+                prim = torch.stack([1 + 0.1 * modes[0], 0.17 * modes[1], 0.17 * modes[2], 0.17 * modes[3],
+                                    1 + 0.1 * modes[4]])
+            Q64 = torch.empty_like(prim)
+            Q64[0] = prim[0]
+            Q64[1:nd + 1] = prim[0] * prim[1:nd + 1]
+            Q64[nd + 1] = prim[nd + 1] / (GAMMA - 1) + prim[0] * (prim[1:nd + 1] ** 2).sum(0) / 2
+            del X, prim
+            dt = F.get_max_dt(Q64, disc, eq, 0.1)
+        Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()  # the steps run in place
+        _ssprk33_steps(disc, Q.clone(), dt, warmup)
+        torch.cuda.synchronize()
+        stream = torch.cuda.current_stream()
+        t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0.record(stream)
+        _ssprk33_steps(disc, Q, dt, nsteps)
+        t1.record(stream)
+        torch.cuda.synchronize()
+        disc.check()
+        ms = t0.elapsed_time(t1)
+        res[form] = dict(ms=ms, rho=Q[0].to(torch.float64).clone(), ndofs=disc.ndofs,
+                         fast=disc.uses_fast_kernel() if form == "fp32" else False)
+        del disc, Q
+        torch.cuda.empty_cache()
+    nn = res["fp32"]["ndofs"]
+    stages = 3 * nsteps
+    out = {"workload": SECONDARY[name], "nodes": nn, "steps": nsteps, "scheme": "SSPRK33", "dt": dt}
+    for form in ("fp32", "fp64"):
+        s_bytes = 4 if form == "fp32" else 8
+        geo = (nd * nd + 1) * s_bytes if name == "c5" else 0
+        bytes_per_stage = ((2 + 3 + 3) / 3 * nv * s_bytes + geo) * nn
+        launch_ms = res[form]["ms"] / stages
+        achieved = bytes_per_stage / (launch_ms / 1e3) / 1e9
+        out[form] = {"value": nn * stages / (res[form]["ms"] / 1e3), "unit": "DOF-updates/s",
+                     "avg_stage_ms": launch_ms, "roofline_frac": achieved / 8000.0,
+                     "achieved_GBps": achieved, "bytes_per_stage": bytes_per_stage}
+    out["fp32"]["fast_kernel"] = res["fp32"]["fast"]
+    out["fp32_speedup_vs_fp64"] = res["fp64"]["ms"] / res["fp32"]["ms"]
+    d = res["fp32"]["rho"] - res["fp64"]["rho"]
+    out["density_rel_l2_err_fp32_vs_fp64"] = float(torch.linalg.norm(d) / torch.linalg.norm(res["fp64"]["rho"]))
+    return out
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(nel=32, steps=6):
@@ -226,7 +359,7 @@ def cpu_baseline(nel=32, steps=6):
     t = time.perf_counter() - tic
     cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": d.ndofs * 5 * steps / t, "unit": "DOF-updates/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "cpu": cpu_model(),
             "sample": f"oracle (FP64 C++ restatement of Flou's rhs!, OpenMP) on {nel}^3 elements, N=5 "
                       f"({d.ndofs} nodes), {steps} CK2N54 steps = {5 * steps} stages, {t:.1f} s"}
 
@@ -238,8 +371,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint"], default=None,
-                    help="time one form only, or only the adjoint/forward measurement (profiling runs)")
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5"], default=None,
+                    help="time one form only, only the adjoint/forward measurement, or only the "
+                         "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the C2 and C5 workloads")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint/forward measurement")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 3 plain copies of 1 GiB (FP32), the reference kernel for "
@@ -255,6 +390,9 @@ def main():
     torch.cuda.set_device(0)
     if args.only == "adjoint":  # profiling run of the adjoint workload alone
         print(json.dumps({"adjoint": adjoint_ratio()}), flush=True)
+        return
+    if args.only in ("c2", "c5"):  # profiling run of one secondary workload alone
+        print(json.dumps({args.only: secondary_workload(args.only, args.steps, args.warmup)}), flush=True)
         return
     if args.copy_calibration:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -293,9 +431,13 @@ def main():
     bytes_per_launch = (3 + 4 * 4) / 5 * 5 * s_bytes * nn
     launch_ms = r["gpu_ms"] / stages
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
+    traffic_src = None
     if main_form == "fp32" and r["fast"]:
         kname = "k_fast3d<6, CHANDRASEKHAR, MODE_2N> (FP32 fused rhs + CK2N54 stage)"
         traffic = measured_traffic()
+        if traffic is not None:
+            traffic_src = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this workload, "
+                           "from " + os.path.relpath(TRAFFIC_FILE, ROOT) + " (not measured in this run)")
     else:
         kname = f"k_plain<{'float' if main_form == 'fp32' else 'double'},3,6,EULER,SPLIT,CHANDRASEKHAR> (fused rhs + CK2N54 stage)"
         traffic = None
@@ -317,7 +459,7 @@ def main():
                                "CarpenterKennedy2N54 fused stages, fixed dt (CFL 0.3)",
                    "form": main_form, "nodes": nn, "stages_per_step": 5, "dt": dt},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": traffic,
+                     "frac": achieved / 8000.0, "traffic": traffic, "traffic_source": traffic_src,
                      "kernel": kname, "bytes_per_launch": bytes_per_launch, "avg_launch_ms": launch_ms},
     }
     if "fp64" in res and "fp32" in res:
@@ -328,6 +470,11 @@ def main():
         d = res["fp32"]["rho"] - r64["rho"]
         out["density_rel_l2_err_fp32_vs_fp64"] = float(_t.linalg.norm(d) / _t.linalg.norm(r64["rho"]))
         out["density_err_after_steps"] = args.steps + args.warmup
+    if not args.no_secondary and not args.only:
+        for name in ("c2", "c5"):
+            out[name] = secondary_workload(name)
+            print(f"# {name}: fp32 {out[name]['fp32']['avg_stage_ms']:.3f} ms/stage, "
+                  f"fp64 {out[name]['fp64']['avg_stage_ms']:.3f} ms/stage", file=sys.stderr, flush=True)
     out["adjoint_forward_ratio"] = None
     if not args.no_adjoint:
         adj = adjoint_ratio()
