@@ -329,7 +329,8 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float x) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)off, 0, 0);
 }
 
-template <int N, int TP, int MODE>
+// FIRST: stage 0 of a 2N scheme (the register is written, not read)
+template <int N, int TP, int MODE, bool FIRST>
 __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const FastArgs a) {
     using C = FCfg<N>;
     constexpr int NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB, TPB = C::TPB;
@@ -426,16 +427,22 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         if (j >= NN) continue;
         const float rho = q[r][0];
         const float ir = rcp(rho);
-        const float u = q[r][1] * ir, v = q[r][2] * ir, w = q[r][3] * ir;
-        const float qq = fmaf(u, u, fmaf(v, v, w * w));
-        const float p = gm1 * fmaf(-0.5f * rho, qq, q[r][4]);
+        float in[6], u, v, w, p, hir = 0.0f;
+        if constexpr (TP == TP_CHANDRASEKHAR) {
+            // halved velocities straight from 1/(2ρ): 0.5ρ|u|² = 2ρ|u/2|²
+            hir = 0.5f * ir;
+            u = q[r][1] * hir; v = q[r][2] * hir; w = q[r][3] * hir;
+            p = gm1 * fmaf(-2.0f * rho, fmaf(u, u, fmaf(v, v, w * w)), q[r][4]);
+        } else {
+            u = q[r][1] * ir; v = q[r][2] * ir; w = q[r][3] * ir;
+            p = gm1 * fmaf(-0.5f * rho, fmaf(u, u, fmaf(v, v, w * w)), q[r][4]);
+        }
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f))) err |= ERR_LOG_NEG;
 #pragma unroll
         for (int c = 0; c < 5; ++c) sQ[c * NN + j] = q[r][c];
-        float in[6];
         if constexpr (TP == TP_CHANDRASEKHAR) {
-            in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
-            in[4] = 0.5f * rho * rcp(p); in[5] = (-1.0f / 6.0f) * ir;
+            in[0] = 0.5f * rho; in[1] = u; in[2] = v; in[3] = w;
+            in[4] = in[0] * rcp(p); in[5] = (-1.0f / 6.0f) * ir;
         } else if constexpr (TP == TP_KENNEDY_GRUBER) {
             in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
             in[4] = 0.5f * p; in[5] = 0.5f * q[r][4] * ir;
@@ -448,7 +455,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         sP[5 * NN + j] = in[2];
         sP[6 * NN + j] = in[4];
         sP[7 * NN + j] = in[5];
-        if constexpr (TP == TP_CHANDRASEKHAR) sP[8 * NN + j] = 0.5f * p * ir;
+        if constexpr (TP == TP_CHANDRASEKHAR) sP[8 * NN + j] = p * hir;
     }
     if (face_lane) {
         const float lift = (fk & 1) ? a.lift_p : a.lift_m;
@@ -532,11 +539,13 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
                 amax = fmaxf(amax, P[p][0]); amin = fminf(amin, P[p][0]);
                 bmax = fmaxf(bmax, P[p][4]); bmin = fminf(bmin, P[p][4]);
             }
-            const float fa = (amax - amin) * rcp(amax + amin);
-            const float fb = (bmax - bmin) * rcp(bmax + bmin);
-            const float um = fmaxf(fa * fa, fb * fb) * 1.0001f;  // margin for rcp rounding
-            if (um < 1e-4f) line_pairs<N, TP, 2>(P, acc, a.dsharp, inv_gm1);
-            else if (um < 1e-2f) line_pairs<N, TP, 3>(P, acc, a.dsharp, inv_gm1);
+            // u = ((max - min)/(max + min))² of ρ and β along the line, compared without a
+            // division: (max - min)² < u0 (max + min)² (a margin of 1e-4 relative for the
+            // rounding of the squares)
+            const float da = amax - amin, sa = amax + amin, db = bmax - bmin, sb = bmax + bmin;
+            const float qa = da * da * 1.0001f, pa = sa * sa, qb = db * db * 1.0001f, pb = sb * sb;
+            if (qa < 1e-4f * pa && qb < 1e-4f * pb) line_pairs<N, TP, 2>(P, acc, a.dsharp, inv_gm1);
+            else if (qa < 1e-2f * pa && qb < 1e-2f * pb) line_pairs<N, TP, 3>(P, acc, a.dsharp, inv_gm1);
             else line_pairs<N, TP, 0>(P, acc, a.dsharp, inv_gm1);
 #pragma unroll
             for (int p = 0; p < N; ++p) acc[p][4] += acc[p][4];  // energy row was halved
@@ -547,7 +556,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
 
     // the stage epilogue's second operand (RK register or SSP base), loaded before the
     // barrier so its latency overlaps the other waves' line tasks
-    const float* aux = (MODE == MODE_2N && !a.first) ? a.reg
+    const float* aux = (MODE == MODE_2N && !FIRST) ? a.reg
                      : ((MODE == MODE_SSP2 || MODE == MODE_SSP3) ? a.ubase : nullptr);
     float xv[NPT][5];
 #pragma unroll
@@ -596,7 +605,7 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
             if constexpr (MODE == MODE_RHS) {
                 out = k;
             } else if constexpr (MODE == MODE_2N) {
-                const float t = a.first ? a.dt * k : fmaf(a.A, xx, a.dt * k);
+                const float t = FIRST ? a.dt * k : fmaf(a.A, xx, a.dt * k);
                 bstore(rr, off, t);
                 out = fmaf(a.B, t, uin);
             } else if constexpr (MODE == MODE_SSP1) {
@@ -617,11 +626,14 @@ static hipError_t launch(const FastArgs& a, hipStream_t s) {
     const int nblocks = (a.ne + FCfg<N>::EPB - 1) / FCfg<N>::EPB;
     const dim3 g(nblocks), b(FCfg<N>::TPB);
     switch (a.mode) {
-        case MODE_RHS: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_RHS>), g, b, 0, s, a); break;
-        case MODE_2N: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N>), g, b, 0, s, a); break;
-        case MODE_SSP1: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP1>), g, b, 0, s, a); break;
-        case MODE_SSP2: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP2>), g, b, 0, s, a); break;
-        case MODE_SSP3: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP3>), g, b, 0, s, a); break;
+        case MODE_RHS: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_RHS, false>), g, b, 0, s, a); break;
+        case MODE_2N:
+            if (a.first) hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N, false>), g, b, 0, s, a);
+            break;
+        case MODE_SSP1: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP1, false>), g, b, 0, s, a); break;
+        case MODE_SSP2: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP2, false>), g, b, 0, s, a); break;
+        case MODE_SSP3: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP3, false>), g, b, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

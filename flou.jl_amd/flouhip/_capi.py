@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libflouhip.so")
+# $FLOUHIP_LIB selects another build of the library (same ABI), e.g. for A/B timing
+LIB_PATH = os.environ.get("FLOUHIP_LIB") or os.path.join(os.path.dirname(_HERE), "libflouhip.so")
 
 OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, EDOMAIN = 0, -1, -2, -3, -4, -5
 FP64, FP32 = 0, 1
