@@ -68,7 +68,7 @@ struct flou_hip_handle {
     VjpLauncherF64 v64 = nullptr;
     VjpLauncherF32 v32 = nullptr;
     std::string vjp_why;  // why the adjoint is unavailable for this handle
-    // FP32 fast path (3-D split form on axis-aligned affine elements)
+    // FP32 fast path (2-D/3-D split form on axis-aligned affine elements)
     FastLauncher lfast = nullptr;
     int force_plain = 0;
     void* d_fegeom = nullptr;
@@ -578,14 +578,15 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     }
     if (!h->v64 && !h->v32 && h->vjp_why.empty()) h->vjp_why = "adjoint: no kernel for this combination";
 
-    // FP32 fast path: 3-D split-form Euler with the StdAverage/LxF surface flux on
+    // FP32 fast path: 2-D and 3-D split-form Euler with the StdAverage/LxF surface flux on
     // axis-aligned affine elements (diagonal metric, frames n = ±e_d), GLL traces
     // (l = e_1/e_n and ∂g supported on the end nodes up to 1e-12) and a vanishing D♯
     // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable).  Anything
     // else runs the plain kernel in FP32.
     std::vector<float> fe, fds;
-    if (!h->curved && h->prec == FLOU_HIP_FP32 && nd == 3 && h->eq == FLOU_HIP_EULER && h->vol == FLOU_HIP_SPLIT &&
-        h->savg == FLOU_HIP_AVG_STD && (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
+    if (!h->curved && h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
+        h->vol == FLOU_HIP_SPLIT && h->savg == FLOU_HIP_AVG_STD &&
+        (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
         h->ndofs < (int64_t(1) << 30)) {
         bool ok = true;
         double dmax = 0.0;
@@ -597,26 +598,28 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             if (j < n - 1 && std::fabs(d->dg_plus[j]) > 1e-12 * std::fabs(d->dg_plus[n - 1])) ok = false;
             if (std::fabs(d->Dsharp[j * n + j]) > 1e-10 * dmax) ok = false;
         }
+        const int nd2 = nd * nd;
         for (int64_t e = 0; e < ne && ok; ++e)
-            for (int r = 0; r < 3; ++r)
-                for (int c = 0; c < 3; ++c)
-                    if (r != c && d->elem_metric[e * 9 + r + 3 * c] != 0.0) ok = false;
+            for (int r = 0; r < nd; ++r)
+                for (int c = 0; c < nd; ++c)
+                    if (r != c && d->elem_metric[e * nd2 + r + nd * c] != 0.0) ok = false;
         for (int64_t f = 0; f < nf && ok; ++f) {
             const int dd = d->face_elempos[f * 2] >> 1;
-            for (int c = 0; c < 3; ++c) {
-                const double nc = d->face_frames[f * 9 + c];
+            for (int c = 0; c < nd; ++c) {
+                const double nc = d->face_frames[f * 3 * nd + c];
                 if (c == dd ? std::fabs(nc) != 1.0 : nc != 0.0) ok = false;
             }
         }
-        if (ok) h->lfast = fast3d_launcher(n, h->tp);
+        if (ok) h->lfast = fast_launcher(nd, n, h->tp);
         if (h->lfast) {
+            // per element: Ja_cc/J (slots 0..nd-1), 1/J (slot 3), 1/Ja_cc (slots 4..3+nd)
             const int64_t cnt = uniform ? 1 : ne;
             fe.assign(cnt * 8, 0.0f);
             for (int64_t e = 0; e < cnt; ++e) {
                 const double J = d->elem_jac[e];
-                for (int c = 0; c < 3; ++c) {
-                    const double ja = d->elem_metric[e * 9 + c * 4];
-                    fe[e * 8 + c] = (float)(ja / J);   // Ja_cc / J
+                for (int c = 0; c < nd; ++c) {
+                    const double ja = d->elem_metric[e * nd2 + c * (nd + 1)];
+                    fe[e * 8 + c] = (float)(ja / J);        // Ja_cc / J
                     fe[e * 8 + 4 + c] = (float)(1.0 / ja);  // lift scale 1 / Ja_cc
                 }
                 fe[e * 8 + 3] = (float)(1.0 / J);

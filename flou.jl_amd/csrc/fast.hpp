@@ -7,7 +7,7 @@
 namespace flouhip {
 
 struct FastArgs {
-    const float* Q;     // stage input, (ndofs, 5) column-major, ndofs < 2^30
+    const float* Q;     // stage input, (ndofs, nd + 2) column-major, ndofs < 2^30
     float* out;
     const float* ubase;
     float* reg;
@@ -15,13 +15,13 @@ struct FastArgs {
     int mode, first;
     int64_t ndofs;
     int ne;
-    const int* nb;      // ne*6 neighbour element (-1: boundary)
-    const int* finfo;   // ne*6 (see common.hpp)
-    const int* fgi;     // ne*6 index into fgeom
-    const int* fid;     // ne*6 mesh face index
-    const float* egeom; // per element: Ja_xx/J, Ja_yy/J, Ja_zz/J, 1/J, 1/Ja_xx, 1/Ja_yy, 1/Ja_zz, -
+    const int* nb;      // ne*2nd neighbour element (-1: boundary)
+    const int* finfo;   // ne*2nd (see common.hpp)
+    const int* fgi;     // ne*2nd index into fgeom
+    const int* fid;     // ne*2nd mesh face index
+    const float* egeom; // per element: Ja_dd/J (d < nd, slots 0..2), 1/J (slot 3), 1/Ja_dd (slots 4..6)
     int egeom_stride;   // 0 when every element is alike, else 8
-    const float* fgeom; // per entry: frame (n, t, b) then surface J (10 floats)
+    const float* fgeom; // per entry: frame (n, t, b) then surface J (3nd + 1 floats)
     const float* dsharp;// D♯, row-major n x n
     float lift_m, lift_p; // ∂g_minus[0], ∂g_plus[n-1]
     const float* bcstate;
@@ -34,7 +34,7 @@ struct FastArgs {
 
 typedef hipError_t (*FastLauncher)(const FastArgs&, hipStream_t);
 
-// 3-D Euler split form, n = 4..8 nodes per direction, tp = two-point flux.
-FastLauncher fast3d_launcher(int n, int tp);
+// 2-D and 3-D Euler split form, n = 4..8 nodes per direction, tp = two-point flux.
+FastLauncher fast_launcher(int nd, int n, int tp);
 
 }  // namespace flouhip

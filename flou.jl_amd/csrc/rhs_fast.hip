@@ -1,6 +1,6 @@
-// FP32 ("chosen form") fused element kernel for the 3-D split-form Euler rhs on
-// axis-aligned affine hexahedra (Flou's CartesianMesh), GLL nodes, LxF(StdAverage) or
-// StdAverage surface flux, with the RK stage epilogue.
+// FP32 ("chosen form") fused element kernel for the 2-D and 3-D split-form Euler rhs on
+// axis-aligned affine elements (Flou's CartesianMesh quads and hexahedra), GLL nodes,
+// LxF(StdAverage) or StdAverage surface flux, with the RK stage epilogue.
 //
 // Same mathematics as SplitDivOperator + interface_fluxes! + the FR lift
 // (OpDivergence.jl:200-299, Interfaces.jl:111-136), organised for gfx950:
@@ -9,14 +9,15 @@
 //     exterior traces of its line (neighbour nodes or boundary states, into registers),
 //     issued after the block's own nodes so that the staging waits for those only; a
 //     neighbour inside the block is read from LDS after staging instead;
-//   * staging: every thread loads all five variables of its nodes (coalesced, one
+//   * staging: every thread loads all NV = ND + 2 variables of its nodes (coalesced, one
 //     uniform base per variable) and writes the node states and the two-point flux
 //     inputs to LDS;
 //   * line phase: the numerical flux on the two face nodes where the line ends, lifted
 //     into the two end rows (GLL: ∂g lives on the ends), then all N(N-1)/2 pairs of the
 //     line, each evaluated once (F♯ is symmetric), both rows accumulated in registers
 //     (F♯ never touches memory).  Line-frame velocity slots are consecutive in LDS
-//     (u, v, w, u, v), so one per-lane offset serves every direction;
+//     (u, v, w, u, v in 3-D; u, v, u in 2-D), so one per-lane offset serves every
+//     direction;
 //   * per node: Σ_d (Ja_dd/J)·(row sum of direction d) and the RK stage epilogue.  The row
 //     sums go to LDS in line-frame order over the dead flux inputs and node states.
 // FP32 reformulations (the FP64 plain kernel keeps the reference's operation order):
@@ -38,85 +39,94 @@
 #include "fast.hpp"
 
 namespace flouhip {
-namespace fast3d {
+namespace fast {
 
 __device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ __forceinline__ float lg(float x) { return __logf(x); }
 __device__ __forceinline__ float sq(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 constexpr int r4(int x) { return (x + 3) & ~3; }
-#ifndef FAST_EPB6
-#define FAST_EPB6 2
-#endif
-#ifndef FAST_WAVES
-#define FAST_WAVES
-#endif
+constexpr int ipow(int b, int e) { return e == 0 ? 1 : b * ipow(b, e - 1); }
 
-template <int N>
+// Elements per block: 3-D, the most that keep four waves per SIMD (2 at N = 5); 2-D, a
+// count whose line tasks fill whole waves (or nearly: 126 of 128 lanes at N = 7).
+template <int ND, int N>
+constexpr int fast_epb() {
+    return ND == 3 ? (N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? 2 : 1)))
+                   : (N == 4 ? 16 : (N == 5 ? 32 : (N == 6 ? 16 : (N == 7 ? 9 : 8))));
+}
+
+template <int ND, int N>
 struct FCfg {
-    static constexpr int NPE = N * N * N;
-    static constexpr int NPF = N * N;
-    static constexpr int NL = 3 * N * N;                             // line tasks per element
-    static constexpr int EPB = N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? FAST_EPB6 : 1));
-    static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;          // threads per block
-    static constexpr int NN = EPB * NPE;                             // nodes per block
-    static constexpr int NPT = (NN + TPB - 1) / TPB;                 // nodes per thread
-    // LDS layout in floats.  Q: [5][NN] node states; P: [10][NN] two-point flux inputs
-    // (slots 1..5 hold u, v, w, u, v, so the three velocity slots of a line in any
-    // direction are consecutive; slot 8 is an extra Chandrasekhar input; slot 9 is spare
-    // room for the row sums); G: [EPB][4]; F: [EPB][6] float2.  Once every line is done
-    // the row sums (line-frame variable order) of directions 0 and 1 go to P, those of
-    // direction 2 to Q.  Arrays are indexed by block node.
+    static constexpr int NV = ND + 2;                                 // conserved variables
+    static constexpr int NFC = 2 * ND;                                // faces per element
+    static constexpr int NPE = ipow(N, ND);
+    static constexpr int NPF = ipow(N, ND - 1);
+    static constexpr int NL = ND * NPF;                               // line tasks per element
+    static constexpr int EPB = fast_epb<ND, N>();
+    static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;           // threads per block
+    static constexpr int NN = EPB * NPE;                              // nodes per block
+    static constexpr int NPT = (NN + TPB - 1) / TPB;                  // nodes per thread
+    // LDS layout in floats, arrays indexed by block node.  Q: [NV][NN] node states.
+    // P: two-point flux inputs -- slot 0 density, slots 1 .. 2ND-1 the velocities in cyclic
+    // order (u, v, w, u, v in 3-D; u, v, u in 2-D), so the ND line-frame velocity slots of a
+    // line in any direction are consecutive, then three more scalars (PIN slots).  Once
+    // every line is done the row sums (line-frame variable order) of directions
+    // 0 .. ND-2 go to P, those of direction ND-1 to Q.  G: [EPB][4] Ja_dd/J per element;
+    // F: [EPB][2ND] float2 face sign and lift coefficient.
+    static constexpr int PIN = 2 * ND + 3;
+    static constexpr int PSL = PIN > (ND - 1) * NV ? PIN : (ND - 1) * NV;
     static constexpr int OFF_Q = 0;
-    static constexpr int OFF_P = r4(5 * NN);
-    static constexpr int OFF_G = r4(OFF_P + 10 * NN);
+    static constexpr int OFF_P = r4(NV * NN);
+    static constexpr int OFF_G = r4(OFF_P + PSL * NN);
     static constexpr int OFF_F = OFF_G + 4 * EPB;
-    static constexpr int LDS = OFF_F + 12 * EPB;
+    static constexpr int LDS = OFF_F + 2 * NFC * EPB;
 };
 
-// Node inputs (6 values, stored in 8 LDS slots with u, v repeated after w; a 7th,
-// H = 1/(4β) = p/(2ρ), for the Chandrasekhar flux):
-//   Chandrasekhar:  ρ/2, u/2, v/2, w/2 | β = ρ/(2p), -(1/6)/ρ | H
-//   Kennedy-Gruber: ρ/2, u/2, v/2, w/2 | p/2, e/2
-//   StdAverage:     ρ,   u,   v,   w   | p,   ρe
-// In a line task value 1 is the velocity along the line, values 2, 3 the other two
-// (cyclic order: LDS slots 1+d, 2+d, 3+d for direction d).  Chandrasekhar returns the
-// flux factored as (m, F_1, u_t1, u_t2, h/2): F = (m, F_1, m·u_t1, m·u_t2, m·h/2) with the
-// energy component halved (line_pairs uses the node velocities in place of u_t1, u_t2).
-template <int TP, int PATH>
+// Node inputs (the velocity repeated cyclically as above; a last one, H = 1/(4β) =
+// p/(2ρ), for the Chandrasekhar flux):
+//   Chandrasekhar:  ρ/2, u/2 .. | β = ρ/(2p), -(1/6)/ρ | H
+//   Kennedy-Gruber: ρ/2, u/2 .. | p/2, e/2
+//   StdAverage:     ρ,   u   .. | p,   ρe
+// In a line task value 1 is the velocity along the line, values 2 .. ND the others
+// (cyclic order: LDS slots 1+d .. ND+d for direction d); values ND+1, ND+2, ND+3 the
+// three scalars.  The line-frame input array of a node: Pl[0] density, Pl[1..ND]
+// velocities, Pl[ND+1..ND+3] scalars.  Chandrasekhar returns the flux factored as
+// (m, F_1, -, .., h/2): F = (m, F_1, m·u_t.., m·h/2) with the energy component halved
+// (line_pairs uses the node velocities for the transverse momenta).
+template <int ND, int TP, int PATH>
 __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float inv_gm1, float* F) {
+    constexpr int NV = ND + 2, SB = ND + 1;  // first scalar slot
     if constexpr (TP == TP_CHANDRASEKHAR) {
         const float hs = Pi[0] + Pj[0], dA = Pi[0] - Pj[0];
-        const float db = Pi[4] - Pj[4];
-        const float rb = PATH == 2 ? 0.0f : rcp(Pi[4] + Pj[4]);
+        const float db = Pi[SB] - Pj[SB];
+        const float rb = PATH == 2 ? 0.0f : rcp(Pi[SB] + Pj[SB]);
         const float fb = db * rb;
         const float ub = fb * fb;
         float rho, X;  // X = (F_β/(γ-1) + F_ρ)/2
+        float dot = Pi[ND] * Pj[ND];
+#pragma unroll
+        for (int k = ND - 1; k >= 1; --k) dot = fmaf(Pi[k], Pj[k], dot);
+        const float un = Pi[1] + Pj[1];
         if constexpr (PATH == 2) {
             // Both log-means to first order in u (u < 1e-4: F = 1 + u/3, truncation < 2e-9),
             // without a reciprocal per pair:
             //   1/(β1+β2) = (H1+H2)(1 - f_β²) exactly, and f_β = (β1-β2)/(β1+β2) ≈
             //   (β1-β2)(H1+H2) inside the O(u) terms (relative error 2f⁴ < 2e-8);
-            //   slot 5 holds -(1/6)/ρ, so fr = dA·(slot5_1 + slot5_2) = -f_ρ/3·(1 + O(f²)),
+            //   the second scalar holds -(1/6)/ρ, so fr = dA·(s_1 + s_2) = -f_ρ/3·(1 + O(f²)),
             //   ρ̂ = ρ̄(1 - f_ρ²/3) = hs + dA·fr and the F_ρ term f_ρ²/6 of X is 3/2·fr².
-            const float sh = Pi[6] + Pj[6];
+            const float sh = Pi[SB + 2] + Pj[SB + 2];
             const float fh = db * sh;
             const float uh = fh * fh;
             const float rbh = fmaf(-sh, uh, sh);
-            const float fr = dA * (Pi[5] + Pj[5]);
+            const float fr = dA * (Pi[SB + 1] + Pj[SB + 1]);
             X = fmaf(uh, inv_gm1 * (1.0f / 6.0f), fmaf(fr * fr, 1.5f, 0.5f * (inv_gm1 + 1.0f)));
             rho = fmaf(dA, fr, hs);
             const float p = hs * rbh;
-            const float un = Pi[1] + Pj[1];
-            const float ut1 = Pi[2] + Pj[2];
-            const float ut2 = Pi[3] + Pj[3];
-            const float dot = fmaf(Pi[1], Pj[1], fmaf(Pi[2], Pj[2], Pi[3] * Pj[3]));
             const float m = rho * un;
             F[0] = m;
             F[1] = fmaf(m, un, p);
-            F[2] = ut1;
-            F[3] = ut2;
-            F[4] = fmaf(rbh, X, dot);
+            F[NV - 1] = fmaf(rbh, X, dot);
             return;
         } else {
             const float fr = dA * rcp(hs);
@@ -129,43 +139,35 @@ __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float
                 if (ur < 0.01f) Fr = fmaf(ur, fmaf(ur, fmaf(ur, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
                 else Fr = 0.5f * (lg(Pi[0]) - lg(Pj[0])) * rcp(fr);
                 if (ub < 0.01f) Fb = fmaf(ub, fmaf(ub, fmaf(ub, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
-                else Fb = 0.5f * (lg(Pi[4]) - lg(Pj[4])) * rcp(fb);
+                else Fb = 0.5f * (lg(Pi[SB]) - lg(Pj[SB])) * rcp(fb);
             }
             X = fmaf(Fb, 0.5f * inv_gm1, 0.5f * Fr);
             rho = hs * rcp(Fr);
         }
         const float p = hs * rb;
-        const float un = Pi[1] + Pj[1];
-        const float ut1 = Pi[2] + Pj[2];
-        const float ut2 = Pi[3] + Pj[3];
-        const float dot = fmaf(Pi[1], Pj[1], fmaf(Pi[2], Pj[2], Pi[3] * Pj[3]));
         const float m = rho * un;
-        // factored: F = (m, F[1], m·F[2], m·F[3], m·F[4])
+        // factored: F = (m, F[1], m·u_t.., m·F[NV-1])
         F[0] = m;
         F[1] = fmaf(m, un, p);
-        F[2] = ut1;
-        F[3] = ut2;
-        F[4] = fmaf(rb, X, dot);
+        F[NV - 1] = fmaf(rb, X, dot);
     } else if constexpr (TP == TP_KENNEDY_GRUBER) {
         const float rho = Pi[0] + Pj[0];
         const float un = Pi[1] + Pj[1];
-        const float ut1 = Pi[2] + Pj[2];
-        const float ut2 = Pi[3] + Pj[3];
-        const float p = Pi[4] + Pj[4];
-        const float e = Pi[5] + Pj[5];
+        const float p = Pi[SB] + Pj[SB];
+        const float e = Pi[SB + 1] + Pj[SB + 1];
         const float m = rho * un;
         F[0] = m;
         F[1] = fmaf(m, un, p);
-        F[2] = m * ut1;
-        F[3] = m * ut2;
-        F[4] = fmaf(m, e, p * un);
+#pragma unroll
+        for (int k = 2; k <= ND; ++k) F[k] = m * (Pi[k] + Pj[k]);
+        F[NV - 1] = fmaf(m, e, p * un);
     } else {
         const float mi = Pi[0] * Pi[1], mj = Pj[0] * Pj[1];
         F[0] = 0.5f * (mi + mj);
-        F[1] = 0.5f * (fmaf(mi, Pi[1], Pi[4]) + fmaf(mj, Pj[1], Pj[4]));
-        F[2] = 0.5f * (mi * Pi[2] + mj * Pj[2]);
-        F[3] = 0.5f * (mi * Pi[3] + mj * Pj[3]);
-        F[4] = 0.5f * ((Pi[5] + Pi[4]) * Pi[1] + (Pj[5] + Pj[4]) * Pj[1]);
+        F[1] = 0.5f * (fmaf(mi, Pi[1], Pi[SB]) + fmaf(mj, Pj[1], Pj[SB]));
+#pragma unroll
+        for (int k = 2; k <= ND; ++k) F[k] = 0.5f * (mi * Pi[k] + mj * Pj[k]);
+        F[NV - 1] = 0.5f * ((Pi[SB + 1] + Pi[SB]) * Pi[1] + (Pj[SB + 1] + Pj[SB]) * Pj[1]);
     }
 }
 
@@ -173,17 +175,18 @@ __device__ __forceinline__ void twopoint(const float* Pi, const float* Pj, float
 // hold the lifted face fluxes; the pairs (0, j) carry the first contribution to every
 // other row, so they set those accumulators.  Chandrasekhar: the transverse momentum
 // components m (u_i + u_j)/2 are split, -Σ_j D♯_ij m_ij u_i = u_i × (the row's mass sum),
-// which saves the two velocity sums of every pair.
-template <int N, int TP, int PATH>
-__device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[N][5],
+// which saves the velocity sums of every pair.
+template <int ND, int N, int TP, int PATH>
+__device__ __forceinline__ void line_pairs(const float (&P)[N][ND + 4], float (&acc)[N][ND + 2],
                                            const float* __restrict__ Dsh, float inv_gm1) {
+    constexpr int NV = ND + 2;
     const float f0 = acc[0][0], f1 = acc[N - 1][0];  // face terms preset in rows 0, N-1
 #pragma unroll
     for (int i = 0; i < N; ++i) {
 #pragma unroll
         for (int j = i + 1; j < N; ++j) {
-            float F[5];
-            twopoint<TP, PATH>(P[i], P[j], inv_gm1, F);
+            float F[NV];
+            twopoint<ND, TP, PATH>(P[i], P[j], inv_gm1, F);
             const float cij = Dsh[i * N + j], cji = Dsh[j * N + i];
             const bool setj = i == 0 && j < N - 1;  // first contribution to row j
             if constexpr (TP == TP_CHANDRASEKHAR) {
@@ -195,15 +198,15 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[
                 // transverse momenta: -m (u_i + u_j)/2 per row; only the partner's half is
                 // accumulated here, the own half is u_i times the row's mass sum (below)
 #pragma unroll
-                for (int v = 2; v < 4; ++v) {
+                for (int v = 2; v <= ND; ++v) {
                     acc[i][v] = fmaf(-mi, P[j][v], acc[i][v]);
                     acc[j][v] = setj ? -mj * P[i][v] : fmaf(-mj, P[i][v], acc[j][v]);
                 }
-                acc[i][4] = fmaf(-mi, F[4], acc[i][4]);
-                acc[j][4] = setj ? -mj * F[4] : fmaf(-mj, F[4], acc[j][4]);
+                acc[i][NV - 1] = fmaf(-mi, F[NV - 1], acc[i][NV - 1]);
+                acc[j][NV - 1] = setj ? -mj * F[NV - 1] : fmaf(-mj, F[NV - 1], acc[j][NV - 1]);
             } else {
 #pragma unroll
-                for (int v = 0; v < 5; ++v) {
+                for (int v = 0; v < NV; ++v) {
                     acc[i][v] = fmaf(-cij, F[v], acc[i][v]);
                     acc[j][v] = setj ? -cji * F[v] : fmaf(-cji, F[v], acc[j][v]);
                 }
@@ -214,61 +217,65 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][7], float (&acc)[
 #pragma unroll
         for (int p = 0; p < N; ++p) {
             const float m = p == 0 ? acc[p][0] - f0 : (p == N - 1 ? acc[p][0] - f1 : acc[p][0]);
-            acc[p][2] = fmaf(P[p][2], m, acc[p][2]);
-            acc[p][3] = fmaf(P[p][3], m, acc[p][3]);
+#pragma unroll
+            for (int v = 2; v <= ND; ++v) acc[p][v] = fmaf(P[p][v], m, acc[p][v]);
         }
     }
 }
 
 // Line addressing.  Node p of a line is block node base + p*stride; the momentum along the
-// line and the two others (line-frame slots 1, 2, 3) are physical variables m[0..2] --
+// line and the others (line-frame slots 1 .. ND) are physical variables m[0 .. ND-1] --
 // per-lane offsets, so one instruction stream serves lines of every direction.  vel is
 // the per-lane offset (1 + direction)*NN of the line's first velocity slot in P.
+template <int ND>
 struct LineAddr {
     int base, stride, vel;
-    int m[3];
+    int m[ND];
 };
 
 // flux inputs of the N nodes of a line, line-frame order (slot arrays of NN floats)
-template <int N, int NN, bool H>
-__device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr& L, float (&P)[N][7]) {
+template <int ND, int N, int NN, bool H>
+__device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr<ND>& L,
+                                          float (&P)[N][ND + 4]) {
 #pragma unroll
     for (int p = 0; p < N; ++p) {
         const float* q = sP + L.base + p * L.stride;
         const float* qv = q + L.vel;
         P[p][0] = q[0];
-        P[p][1] = qv[0];
-        P[p][2] = qv[NN];
-        P[p][3] = qv[2 * NN];
-        P[p][4] = q[6 * NN];
-        P[p][5] = q[7 * NN];
-        P[p][6] = H ? q[8 * NN] : 0.0f;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) P[p][1 + k] = qv[k * NN];
+        P[p][ND + 1] = q[(2 * ND) * NN];
+        P[p][ND + 2] = q[(2 * ND + 1) * NN];
+        P[p][ND + 3] = H ? q[(2 * ND + 2) * NN] : 0.0f;
     }
 }
 
-// five variables in line-frame order from an array with variable stride VS
-template <int VS>
-__device__ __forceinline__ void load_vars(const float* __restrict__ s, const LineAddr& L, float (&o)[5]) {
+// the NV variables in line-frame order from an array with variable stride VS
+template <int ND, int VS>
+__device__ __forceinline__ void load_vars(const float* __restrict__ s, const LineAddr<ND>& L, float (&o)[ND + 2]) {
     o[0] = s[0];
-    o[1] = s[L.m[0] * VS];
-    o[2] = s[L.m[1] * VS];
-    o[3] = s[L.m[2] * VS];
-    o[4] = s[4 * VS];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) o[1 + k] = s[L.m[k] * VS];
+    o[ND + 1] = s[(ND + 1) * VS];
 }
 
 // row sums of a line in line-frame variable order (the node phase undoes the permutation)
-template <int N, int NN>
-__device__ __forceinline__ void store_line(float* __restrict__ R, const LineAddr& L, const float (&acc)[N][5]) {
+template <int ND, int N, int NN>
+__device__ __forceinline__ void store_line(float* __restrict__ R, const LineAddr<ND>& L,
+                                           const float (&acc)[N][ND + 2]) {
 #pragma unroll
     for (int p = 0; p < N; ++p) {
         float* r = R + L.base + p * L.stride;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) r[c * NN] = acc[p][c];
+        for (int c = 0; c < ND + 2; ++c) r[c * NN] = acc[p][c];
     }
 }
 
 // line-frame slot of physical variable v in a line of direction d (momenta cycle)
-__host__ __device__ constexpr int lslot(int d, int v) { return (v == 0 || v == 4) ? v : ((v - 1 - d + 3) % 3) + 1; }
+template <int ND>
+__host__ __device__ constexpr int lslot(int d, int v) {
+    return (v == 0 || v == ND + 1) ? v : ((v - 1 - d + ND) % ND) + 1;
+}
 
 // Numerical flux on the face where a line ends (row R of the line) and its lift, which
 // starts row R: acc[R] = -coef·[s(f_own + f_ext) + λ(Q_own - Q_ext)], in the line frame
@@ -276,11 +283,12 @@ __host__ __device__ constexpr int lslot(int d, int v) { return (v == 0 || v == 4
 // Chandrasekhar inputs (PO: ρ/2, u_n/2, H = p/(2ρ) of the own node in Po) the own side
 // takes u_n, p and c² from them instead of dividing by ρ; the own node's positivity is
 // already checked at staging.
-template <int N, int R, bool PO>
-__device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[5], const float* Po, float s,
-                                          float coef, float gm1, float gamma, float diss, float ehalf,
-                                          float (&acc)[N][5], bool& bad) {
-    float fo[5], fx[5], ao, ax;
+template <int ND, int N, int R, bool PO>
+__device__ __forceinline__ void face_init(const float (&o)[ND + 2], const float (&x)[ND + 2], const float* Po,
+                                          float s, float coef, float gm1, float gamma, float diss, float ehalf,
+                                          float (&acc)[N][ND + 2], bool& bad) {
+    constexpr int NV = ND + 2;
+    float fo[NV], fx[NV], ao, ax;
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
         const float* Q = side ? x : o;
@@ -288,12 +296,14 @@ __device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[
         float p, un, c2;
         if (PO && !side) {
             un = Po[1] + Po[1];
-            p = 4.0f * Po[0] * Po[6];
-            c2 = (2.0f * gamma) * Po[6];
+            p = 4.0f * Po[0] * Po[ND + 3];
+            c2 = (2.0f * gamma) * Po[ND + 3];
         } else {
             const float ir = rcp(Q[0]);
-            const float m2 = fmaf(Q[1], Q[1], fmaf(Q[2], Q[2], Q[3] * Q[3]));
-            p = gm1 * fmaf(-0.5f * m2, ir, Q[4]);
+            float m2 = Q[ND] * Q[ND];
+#pragma unroll
+            for (int k = ND - 1; k >= 1; --k) m2 = fmaf(Q[k], Q[k], m2);
+            p = gm1 * fmaf(-0.5f * m2, ir, Q[NV - 1]);
             un = Q[1] * ir;
             c2 = gamma * p * ir;
             bad |= !(c2 >= 0.0f);
@@ -301,16 +311,16 @@ __device__ __forceinline__ void face_init(const float (&o)[5], const float (&x)[
         const float a = fabsf(un) + sq(c2);
         f[0] = Q[1];
         f[1] = fmaf(Q[1], un, p);
-        f[2] = Q[2] * un;
-        f[3] = Q[3] * un;
-        f[4] = (Q[4] + p) * un;
+#pragma unroll
+        for (int k = 2; k <= ND; ++k) f[k] = Q[k] * un;
+        f[NV - 1] = (Q[NV - 1] + p) * un;
         if (side) ax = a; else ao = a;
     }
     const float cs = coef * s, cl = coef * fmaxf(ao, ax) * diss;
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < NV; ++v) {
         const float t = fmaf(-cs, fo[v] + fx[v], -cl * (o[v] - x[v]));
-        acc[R][v] = v == 4 ? ehalf * t : t;
+        acc[R][v] = v == NV - 1 ? ehalf * t : t;
     }
 }
 
@@ -330,11 +340,12 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float x) {
 }
 
 // FIRST: stage 0 of a 2N scheme (the register is written, not read)
-template <int N, int TP, int MODE, bool FIRST>
-__global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const FastArgs a) {
-    using C = FCfg<N>;
-    constexpr int NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB, TPB = C::TPB;
-    constexpr int NN = C::NN, NPT = C::NPT;
+template <int ND, int N, int TP, int MODE, bool FIRST>
+__global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
+    using C = FCfg<ND, N>;
+    constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
+    constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
+    constexpr int FG = 3 * ND + 1;  // face geometry entry: frame (n, t, b) and surface J
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
     float* const sQ = sm + C::OFF_Q;
@@ -364,27 +375,27 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
     int xnb[2], xinf[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
-        const int slot = (e0 + lle) * 6 + 2 * ldir + sd;
+        const int slot = (e0 + lle) * NFC + 2 * ldir + sd;
         xnb[sd] = line_ok ? a.nb[slot] : -2;
         xinf[sd] = line_ok ? a.finfo[slot] : 0;
     }
 
-    // per face (lanes tid < 6*EPB): the face-geometry index and the side, loaded with the
+    // per face (lanes tid < NFC*EPB): the face-geometry index and the side, loaded with the
     // face tables so that the dependent geometry loads below wait for these only
-    const int fle = tid / 6, fk = tid - (tid / 6) * 6;
-    const bool face_lane = tid < 6 * EPB && fle < nel;
+    const int fle = tid / NFC, fk = tid - (tid / NFC) * NFC;
+    const bool face_lane = tid < NFC * EPB && fle < nel;
     int fgi = 0, fside = 0;
     if (face_lane) {
-        fgi = a.fgi[(e0 + fle) * 6 + fk];
-        fside = finfo_side(a.finfo[(e0 + fle) * 6 + fk]);
+        fgi = a.fgi[(e0 + fle) * NFC + fk];
+        fside = finfo_side(a.finfo[(e0 + fle) * NFC + fk]);
     }
 
     // ---- staging ---------------------------------------------------------------------
-    // element nodes, all five variables per node (lanes past the block's last element read 0)
+    // element nodes, all variables per node (lanes past the block's last element read 0)
     const uint32_t blk_bytes = (uint32_t)(nel * NPE) * 4u;
-    float q[NPT][5];
+    float q[NPT][NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < NV; ++v) {
         const rsrc_t rq = make_rsrc(a.Q + v * Nd + n0, blk_bytes);
 #pragma unroll
         for (int r = 0; r < NPT; ++r) q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
@@ -394,10 +405,10 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
     float fsn = 0.0f, fjac = 0.0f, feg = 0.0f, fed = 0.0f;
     if (face_lane) {
         const float* eg = a.egeom + (int64_t)(e0 + fle) * a.egeom_stride;
-        fsn = a.fgeom[(int64_t)fgi * 10 + (fk >> 1)];
-        fjac = a.fgeom[(int64_t)fgi * 10 + 9];
+        fsn = a.fgeom[(int64_t)fgi * FG + (fk >> 1)];
+        fjac = a.fgeom[(int64_t)fgi * FG + 3 * ND];
         feg = eg[4 + (fk >> 1)];
-        if (fk < 3) fed = eg[fk];
+        if (fk < ND) fed = eg[fk];
     }
     // exterior traces at both line ends: the neighbour's node on interior faces
     // (branch-free; other lanes, including those past the block's line tasks, read 0
@@ -410,12 +421,12 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         const int kp = finfo_nbpos(xinf[sd]);
         inblk[sd] = xnb[sd] >= e0 && xnb[sd] < e0 + nel;
         xo[sd] = xnb[sd] >= 0 && !inblk[sd]
-                     ? ((uint32_t)xnb[sd] * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
+                     ? ((uint32_t)xnb[sd] * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
                      : 0xffffffffu;
     }
-    float xr[2][5];
+    float xr[2][NV];
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < NV; ++v) {
         const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) xr[sd][v] = bload(rv, xo[sd]);
@@ -427,40 +438,46 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
         if (j >= NN) continue;
         const float rho = q[r][0];
         const float ir = rcp(rho);
-        float in[6], u, v, w, p, hir = 0.0f;
-        if constexpr (TP == TP_CHANDRASEKHAR) {
-            // halved velocities straight from 1/(2ρ): 0.5ρ|u|² = 2ρ|u/2|²
-            hir = 0.5f * ir;
-            u = q[r][1] * hir; v = q[r][2] * hir; w = q[r][3] * hir;
-            p = gm1 * fmaf(-2.0f * rho, fmaf(u, u, fmaf(v, v, w * w)), q[r][4]);
-        } else {
-            u = q[r][1] * ir; v = q[r][2] * ir; w = q[r][3] * ir;
-            p = gm1 * fmaf(-0.5f * rho, fmaf(u, u, fmaf(v, v, w * w)), q[r][4]);
-        }
+        float in[ND + 4], u[ND], hir = 0.0f;
+        const float uscale = TP == TP_CHANDRASEKHAR ? 0.5f * ir : ir;
+        if constexpr (TP == TP_CHANDRASEKHAR) hir = uscale;  // halved velocities from 1/(2ρ)
+#pragma unroll
+        for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * uscale;
+        float uu = u[ND - 1] * u[ND - 1];
+#pragma unroll
+        for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
+        // 0.5ρ|u|² = 2ρ|u/2|²
+        const float p = gm1 * fmaf((TP == TP_CHANDRASEKHAR ? -2.0f : -0.5f) * rho, uu, q[r][NV - 1]);
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f))) err |= ERR_LOG_NEG;
 #pragma unroll
-        for (int c = 0; c < 5; ++c) sQ[c * NN + j] = q[r][c];
+        for (int c = 0; c < NV; ++c) sQ[c * NN + j] = q[r][c];
         if constexpr (TP == TP_CHANDRASEKHAR) {
-            in[0] = 0.5f * rho; in[1] = u; in[2] = v; in[3] = w;
-            in[4] = in[0] * rcp(p); in[5] = (-1.0f / 6.0f) * ir;
+            in[0] = 0.5f * rho;
+            in[ND + 1] = in[0] * rcp(p);
+            in[ND + 2] = (-1.0f / 6.0f) * ir;
+            in[ND + 3] = p * hir;
         } else if constexpr (TP == TP_KENNEDY_GRUBER) {
-            in[0] = 0.5f * rho; in[1] = 0.5f * u; in[2] = 0.5f * v; in[3] = 0.5f * w;
-            in[4] = 0.5f * p; in[5] = 0.5f * q[r][4] * ir;
-        } else {
-            in[0] = rho; in[1] = u; in[2] = v; in[3] = w; in[4] = p; in[5] = q[r][4];
-        }
+            in[0] = 0.5f * rho;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sP[c * NN + j] = in[c];
-        sP[4 * NN + j] = in[1];
-        sP[5 * NN + j] = in[2];
-        sP[6 * NN + j] = in[4];
-        sP[7 * NN + j] = in[5];
-        if constexpr (TP == TP_CHANDRASEKHAR) sP[8 * NN + j] = p * hir;
+            for (int k = 0; k < ND; ++k) u[k] *= 0.5f;
+            in[ND + 1] = 0.5f * p;
+            in[ND + 2] = 0.5f * q[r][NV - 1] * ir;
+        } else {
+            in[0] = rho;
+            in[ND + 1] = p;
+            in[ND + 2] = q[r][NV - 1];
+        }
+        sP[j] = in[0];
+#pragma unroll
+        for (int c = 0; c < 2 * ND - 1; ++c) sP[(1 + c) * NN + j] = u[c % ND];
+        sP[(2 * ND) * NN + j] = in[ND + 1];
+        sP[(2 * ND + 1) * NN + j] = in[ND + 2];
+        if constexpr (TP == TP_CHANDRASEKHAR) sP[(2 * ND + 2) * NN + j] = in[ND + 3];
     }
     if (face_lane) {
         const float lift = (fk & 1) ? a.lift_p : a.lift_m;
         sF[tid] = make_float2(fside ? -fsn : fsn, 0.5f * feg * lift * fjac);
-        if (fk < 3) sG[fle * 4 + fk] = fed;
+        if (fk < ND) sG[fle * 4 + fk] = fed;
     }
     // boundary faces: Dirichlet or inflow states, or the own node (outflow, slip)
 #pragma unroll
@@ -472,127 +489,141 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
             const float* src;
             int64_t vs = 1;
             if (type == BC_DIRICHLET) {
-                src = a.bcstate + ((int64_t)a.fid[(e0 + lle) * 6 + k] * NPF + lf) * 5;
+                src = a.bcstate + ((int64_t)a.fid[(e0 + lle) * NFC + k] * NPF + lf) * NV;
             } else if (type == BC_INFLOW) {
-                src = a.bcconst + bd * 5;
+                src = a.bcconst + bd * NV;
             } else {
-                src = a.Q + n0 + lle * NPE + line_node<3, N>(ldir, lf, sd ? N - 1 : 0);
+                src = a.Q + n0 + lle * NPE + line_node<ND, N>(ldir, lf, sd ? N - 1 : 0);
                 vs = Nd;
             }
 #pragma unroll
-            for (int v = 0; v < 5; ++v) xr[sd][v] = src[v * vs];
+            for (int v = 0; v < NV; ++v) xr[sd][v] = src[v * vs];
         }
     }
     __syncthreads();
-    // neighbours in this block (the x neighbour of every other element): their node
-    // states are already in LDS
+    // neighbours in this block: their node states are already in LDS
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
         if (inblk[sd]) {
             const int kp = finfo_nbpos(xinf[sd]);
-            const int node = (xnb[sd] - e0) * NPE + line_node<3, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+            const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
 #pragma unroll
-            for (int v = 0; v < 5; ++v) xr[sd][v] = sQ[v * NN + node];
+            for (int v = 0; v < NV; ++v) xr[sd][v] = sQ[v * NN + node];
         }
     }
+    // ---- line tasks ------------------------------------------------------------------
+    LineAddr<ND> L;
+    L.base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
+    L.stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    L.vel = (1 + ldir) * NN;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) L.m[k] = 1 + (ldir + k) % ND;
     // exterior traces in the line frame; a slip wall reflects the normal momentum
-    float xl[2][5];
+    float xl[2][NV];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
         const bool slip = xnb[sd] == -1 && a.bctype[finfo_bd(xinf[sd])] == BC_SLIP;
-        const float un = ldir == 0 ? xr[sd][1] : (ldir == 1 ? xr[sd][2] : xr[sd][3]);
         xl[sd][0] = xr[sd][0];
-        xl[sd][1] = slip ? -un : un;
-        xl[sd][2] = ldir == 0 ? xr[sd][2] : (ldir == 1 ? xr[sd][3] : xr[sd][1]);
-        xl[sd][3] = ldir == 0 ? xr[sd][3] : (ldir == 1 ? xr[sd][1] : xr[sd][2]);
-        xl[sd][4] = xr[sd][4];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            // momentum 1 + (ldir + k) % ND, selected without a per-lane array index
+            float m = xr[sd][1 + k];
+#pragma unroll
+            for (int d = 1; d < ND; ++d)
+                if (ldir == d) m = xr[sd][1 + (d + k) % ND];
+            xl[sd][1 + k] = (k == 0 && slip) ? -m : m;
+        }
+        xl[sd][NV - 1] = xr[sd][NV - 1];
     }
-
-    // ---- line tasks ------------------------------------------------------------------
-    LineAddr L;
-    L.base = lle * NPE + (ldir == 0 ? N * lf : (ldir == 1 ? lf % N + NPF * (lf / N) : lf));
-    L.stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
-    L.vel = (1 + ldir) * NN;
-    L.m[0] = 1 + ldir;
-    L.m[1] = ldir == 2 ? 1 : 2 + ldir;
-    L.m[2] = ldir == 0 ? 3 : ldir;
-    float acc[N][5];
+    float acc[N][NV];
     {
         // face fluxes at the two line ends start rows 0 and N-1
-        float o0[5], o1[5];
-        load_vars<NN>(sQ + L.base, L, o0);
-        load_vars<NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
-        const float2 g0 = sF[lle * 6 + 2 * ldir], g1 = sF[lle * 6 + 2 * ldir + 1];
+        float o0[NV], o1[NV];
+        load_vars<ND, NN>(sQ + L.base, L, o0);
+        load_vars<ND, NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
+        const float2 g0 = sF[lle * NFC + 2 * ldir], g1 = sF[lle * NFC + 2 * ldir + 1];
         const float ehalf = TP == TP_CHANDRASEKHAR ? 0.5f : 1.0f;  // energy row accumulated halved
         constexpr bool PO = TP == TP_CHANDRASEKHAR;
-        float P[N][7];
-        load_line<N, NN, PO>(sP, L, P);
+        float P[N][ND + 4];
+        load_line<ND, N, NN, PO>(sP, L, P);
         bool bad = false;
-        face_init<N, 0, PO>(o0, xl[0], P[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
-        face_init<N, N - 1, PO>(o1, xl[1], P[N - 1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        face_init<ND, N, 0, PO>(o0, xl[0], P[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
+        face_init<ND, N, N - 1, PO>(o1, xl[1], P[N - 1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
         if (bad && line_ok) err |= ERR_SQRT_NEG;
 
         if constexpr (TP == TP_CHANDRASEKHAR) {
-            float amax = P[0][0], amin = P[0][0], bmax = P[0][4], bmin = P[0][4];
+            constexpr int SB = ND + 1;
+            float amax = P[0][0], amin = P[0][0], bmax = P[0][SB], bmin = P[0][SB];
 #pragma unroll
             for (int p = 1; p < N; ++p) {
                 amax = fmaxf(amax, P[p][0]); amin = fminf(amin, P[p][0]);
-                bmax = fmaxf(bmax, P[p][4]); bmin = fminf(bmin, P[p][4]);
+                bmax = fmaxf(bmax, P[p][SB]); bmin = fminf(bmin, P[p][SB]);
             }
             // u = ((max - min)/(max + min))² of ρ and β along the line, compared without a
             // division: (max - min)² < u0 (max + min)² (a margin of 1e-4 relative for the
             // rounding of the squares)
             const float da = amax - amin, sa = amax + amin, db = bmax - bmin, sb = bmax + bmin;
             const float qa = da * da * 1.0001f, pa = sa * sa, qb = db * db * 1.0001f, pb = sb * sb;
-            if (qa < 1e-4f * pa && qb < 1e-4f * pb) line_pairs<N, TP, 2>(P, acc, a.dsharp, inv_gm1);
-            else if (qa < 1e-2f * pa && qb < 1e-2f * pb) line_pairs<N, TP, 3>(P, acc, a.dsharp, inv_gm1);
-            else line_pairs<N, TP, 0>(P, acc, a.dsharp, inv_gm1);
+            if (qa < 1e-4f * pa && qb < 1e-4f * pb) line_pairs<ND, N, TP, 2>(P, acc, a.dsharp, inv_gm1);
+            else if (qa < 1e-2f * pa && qb < 1e-2f * pb) line_pairs<ND, N, TP, 3>(P, acc, a.dsharp, inv_gm1);
+            else line_pairs<ND, N, TP, 0>(P, acc, a.dsharp, inv_gm1);
 #pragma unroll
-            for (int p = 0; p < N; ++p) acc[p][4] += acc[p][4];  // energy row was halved
+            for (int p = 0; p < N; ++p) acc[p][NV - 1] += acc[p][NV - 1];  // energy row was halved
         } else {
-            line_pairs<N, TP, 0>(P, acc, a.dsharp, inv_gm1);
+            line_pairs<ND, N, TP, 0>(P, acc, a.dsharp, inv_gm1);
         }
     }
 
     // the stage epilogue's second operand (RK register or SSP base), loaded before the
-    // barrier so its latency overlaps the other waves' line tasks
+    // barrier so its latency overlaps the other waves' line tasks -- when it is a few
+    // registers per thread (3-D); with more nodes per thread (2-D) after the barriers
     const float* aux = (MODE == MODE_2N && !FIRST) ? a.reg
                      : ((MODE == MODE_SSP2 || MODE == MODE_SSP3) ? a.ubase : nullptr);
-    float xv[NPT][5];
+    constexpr bool EARLY_AUX = NPT * NV <= 12;
+    float xv[NPT][NV];
+    auto load_aux = [&]() {
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
-        const rsrc_t rx = make_rsrc(aux ? aux + v * Nd + n0 : a.Q, aux ? blk_bytes : 0u);
+        for (int v = 0; v < NV; ++v) {
+            const rsrc_t rx = make_rsrc(aux ? aux + v * Nd + n0 : a.Q, aux ? blk_bytes : 0u);
 #pragma unroll
-        for (int r = 0; r < NPT; ++r) xv[r][v] = bload(rx, (uint32_t)(tid + r * TPB) * 4u);
-    }
+            for (int r = 0; r < NPT; ++r) xv[r][v] = bload(rx, (uint32_t)(tid + r * TPB) * 4u);
+        }
+    };
+    if constexpr (EARLY_AUX) load_aux();
     // the stage input of this thread's nodes, before its LDS region takes row sums
-    float uq[NPT][5];
+    float uq[NPT][NV];
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
 #pragma unroll
-        for (int v = 0; v < 5; ++v) uq[r][v] = sQ[v * NN + (j < NN ? j : 0)];
+        for (int v = 0; v < NV; ++v) uq[r][v] = sQ[v * NN + (j < NN ? j : 0)];
     }
     __syncthreads();  // flux inputs and node states are dead from here
-    if (line_ok) store_line<N, NN>(ldir == 0 ? sP : (ldir == 1 ? sP + 5 * NN : sQ), L, acc);
+    if (line_ok) store_line<ND, N, NN>(ldir == ND - 1 ? sQ : sP + ldir * NV * NN, L, acc);
+    if constexpr (!EARLY_AUX) load_aux();
     __syncthreads();
 
     // ---- per node: Σ_d (Ja_dd/J) row sums and the stage epilogue -----------------------
     // (stores of lanes past the block's last element are dropped by the range check)
-    float kv[NPT][5];
+    float kv[NPT][NV];
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
         const int jj = j < NN ? j : 0;
         const int le = jj / NPE;
-        const float s0 = sG[le * 4 + 0], s1 = sG[le * 4 + 1], s2 = sG[le * 4 + 2];
+        float s[ND];
 #pragma unroll
-        for (int v = 0; v < 5; ++v)
-            kv[r][v] = fmaf(s0, sP[lslot(0, v) * NN + jj],
-                            fmaf(s1, sP[(5 + lslot(1, v)) * NN + jj], s2 * sQ[lslot(2, v) * NN + jj]));
+        for (int d = 0; d < ND; ++d) s[d] = sG[le * 4 + d];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            float t = s[ND - 1] * sQ[lslot<ND>(ND - 1, v) * NN + jj];
+#pragma unroll
+            for (int d = ND - 2; d >= 0; --d) t = fmaf(s[d], sP[(d * NV + lslot<ND>(d, v)) * NN + jj], t);
+            kv[r][v] = t;
+        }
     }
 #pragma unroll
-    for (int v = 0; v < 5; ++v) {
+    for (int v = 0; v < NV; ++v) {
         const rsrc_t ro = make_rsrc(a.out + v * Nd + n0, blk_bytes);
         const rsrc_t rr = make_rsrc(a.reg + v * Nd + n0, MODE == MODE_2N ? blk_bytes : 0u);
 #pragma unroll
@@ -621,45 +652,53 @@ __global__ void __launch_bounds__((FCfg<N>::TPB)) FAST_WAVES k_fast3d(const Fast
     if (err) atomicOr(a.errflag, err);
 }
 
-template <int N, int TP>
+template <int ND, int N, int TP>
 static hipError_t launch(const FastArgs& a, hipStream_t s) {
-    const int nblocks = (a.ne + FCfg<N>::EPB - 1) / FCfg<N>::EPB;
-    const dim3 g(nblocks), b(FCfg<N>::TPB);
+    using C = FCfg<ND, N>;
+    const int nblocks = (a.ne + C::EPB - 1) / C::EPB;
+    const dim3 g(nblocks), b(C::TPB);
     switch (a.mode) {
-        case MODE_RHS: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_RHS, false>), g, b, 0, s, a); break;
+        case MODE_RHS: hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_RHS, false>), g, b, 0, s, a); break;
         case MODE_2N:
-            if (a.first) hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N, true>), g, b, 0, s, a);
-            else hipLaunchKernelGGL((k_fast3d<N, TP, MODE_2N, false>), g, b, 0, s, a);
+            if (a.first) hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_2N, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_2N, false>), g, b, 0, s, a);
             break;
-        case MODE_SSP1: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP1, false>), g, b, 0, s, a); break;
-        case MODE_SSP2: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP2, false>), g, b, 0, s, a); break;
-        case MODE_SSP3: hipLaunchKernelGGL((k_fast3d<N, TP, MODE_SSP3, false>), g, b, 0, s, a); break;
+        case MODE_SSP1: hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_SSP1, false>), g, b, 0, s, a); break;
+        case MODE_SSP2: hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_SSP2, false>), g, b, 0, s, a); break;
+        case MODE_SSP3: hipLaunchKernelGGL((k_fast<ND, N, TP, MODE_SSP3, false>), g, b, 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
 
-template <int N>
+template <int ND, int N>
 static FastLauncher pick_tp(int tp) {
     switch (tp) {
-        case TP_CHANDRASEKHAR: return &launch<N, TP_CHANDRASEKHAR>;
-        case TP_KENNEDY_GRUBER: return &launch<N, TP_KENNEDY_GRUBER>;
-        case TP_STDAVG: return &launch<N, TP_STDAVG>;
+        case TP_CHANDRASEKHAR: return &launch<ND, N, TP_CHANDRASEKHAR>;
+        case TP_KENNEDY_GRUBER: return &launch<ND, N, TP_KENNEDY_GRUBER>;
+        case TP_STDAVG: return &launch<ND, N, TP_STDAVG>;
         default: return nullptr;
     }
 }
 
-}  // namespace fast3d
-
-FastLauncher fast3d_launcher(int n, int tp) {
+template <int ND>
+static FastLauncher pick_n(int n, int tp) {
     switch (n) {
-        case 4: return fast3d::pick_tp<4>(tp);
-        case 5: return fast3d::pick_tp<5>(tp);
-        case 6: return fast3d::pick_tp<6>(tp);
-        case 7: return fast3d::pick_tp<7>(tp);
-        case 8: return fast3d::pick_tp<8>(tp);
+        case 4: return pick_tp<ND, 4>(tp);
+        case 5: return pick_tp<ND, 5>(tp);
+        case 6: return pick_tp<ND, 6>(tp);
+        case 7: return pick_tp<ND, 7>(tp);
+        case 8: return pick_tp<ND, 8>(tp);
         default: return nullptr;
     }
+}
+
+}  // namespace fast
+
+FastLauncher fast_launcher(int nd, int n, int tp) {
+    if (nd == 2) return fast::pick_n<2>(n, tp);
+    if (nd == 3) return fast::pick_n<3>(n, tp);
+    return nullptr;
 }
 
 }  // namespace flouhip

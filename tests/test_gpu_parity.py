@@ -255,22 +255,30 @@ def test_advection_2d_full_period_gauss_nodes():
 
 
 FP32_CASES = [
-    # n, tp, surf, periodic, amp
-    (6, "ch", "lxf", [1, 1, 1], 0.1),
-    (6, "ch", "lxf", [1, 1, 1], 0.5),  # large jumps: log branch of the log-mean
-    (5, "kg", "lxf", [1, 1, 1], 0.1),
-    (7, "std", "avg", [1, 1, 1], 0.1),
-    (4, "ch", "lxf", [1, 0, 1], 0.1),  # slip walls on y
-    (8, "ch", "lxf", [1, 1, 1], 0.1),
+    # nd, n, tp, surf, periodic, amp
+    (3, 6, "ch", "lxf", [1, 1, 1], 0.1),
+    (3, 6, "ch", "lxf", [1, 1, 1], 0.5),  # large jumps: log branch of the log-mean
+    (3, 5, "kg", "lxf", [1, 1, 1], 0.1),
+    (3, 7, "std", "avg", [1, 1, 1], 0.1),
+    (3, 4, "ch", "lxf", [1, 0, 1], 0.1),  # slip walls on y
+    (3, 8, "ch", "lxf", [1, 1, 1], 0.1),
+    (2, 8, "kg", "lxf", [1, 1], 0.1),     # the C2 operator
+    (2, 8, "kg", "lxf", [1, 1], 0.5),
+    (2, 7, "ch", "lxf", [1, 1], 0.1),
+    (2, 7, "ch", "lxf", [1, 1], 0.5),
+    (2, 6, "std", "avg", [1, 1], 0.1),
+    (2, 5, "ch", "lxf", [1, 0], 0.1),     # slip walls on y
+    (2, 4, "kg", "lxf", [1, 1], 0.1),
 ]
 
 
 @pytest.mark.parametrize("case", FP32_CASES, ids=lambda c: "-".join(str(x) for x in c))
 def test_fp32_rhs_error_against_fp64_oracle(case):
     """FP32 chosen form (fast kernel) and FP32 plain kernel against the FP64 oracle."""
-    n, tp, surf, periodic, amp = case
-    mesh, std, equation, op, orc = build(3, n, [2, 2, 2], "split", tp, surf, periodic=periodic,
-                                         finish=[2 * np.pi] * 3)
+    nd, n, tp, surf, periodic, amp = case
+    nel = [2, 2, 2] if nd == 3 else [3, 4]
+    mesh, std, equation, op, orc = build(nd, n, nel, "split", tp, surf, periodic=periodic,
+                                         finish=[2 * np.pi] * nd)
     bcs = {}
     if not all(periodic):
         bcs = {"3": F.EulerSlipBC(), "4": F.EulerSlipBC()}
@@ -278,7 +286,7 @@ def test_fp32_rhs_error_against_fp64_oracle(case):
         orc.set_bc(1, O.BC_SLIP)
     disc = F.MultielementDisc(mesh, std, equation, op, bcs, precision="fp32")
     assert disc.uses_fast_kernel()
-    Q = _state(disc, orc, 3, "euler", seed=4, amp=amp)
+    Q = _state(disc, orc, nd, "euler", seed=4, amp=amp)
     ref = orc.rhs(Q)
     Qd = disc.to_device(Q)
     for plain in (0, 1):
@@ -356,20 +364,22 @@ def test_non_periodic_boundaries_match_oracle(n, precision):
     assert rel_err(got, ref) <= (1e-13 if precision == "fp64" else 2e-5)
 
 
-@pytest.mark.parametrize("axis", [0, 2], ids=["x", "z"])
+@pytest.mark.parametrize("nd,axis", [(3, 0), (3, 2), (2, 0), (2, 1)], ids=["3d-x", "3d-z", "2d-x", "2d-y"])
 @pytest.mark.parametrize("kind", ["inflow-outflow", "dirichlet", "slip-outflow"])
-def test_fp32_fast_kernel_boundaries_match_oracle(kind, axis):
-    """Every boundary kind on the FP32 fast path (3-D, N=5, x or z non-periodic), against
-    the FP64 oracle and against the FP32 plain kernel on the same handle.  z: the lines of
-    the block's last wave (z lines of its second element) end on the boundaries."""
-    periodic = [1, 1, 1]
+def test_fp32_fast_kernel_boundaries_match_oracle(kind, nd, axis):
+    """Every boundary kind on the FP32 fast path (3-D N=5 and 2-D N=7, one axis
+    non-periodic), against the FP64 oracle and against the FP32 plain kernel on the same
+    handle.  3-D z: the lines of the block's last wave (z lines of its second element) end
+    on the boundaries."""
+    periodic = [1] * nd
     periodic[axis] = 0
-    nel = [2, 2, 2]
+    nel = [2] * nd if nd == 3 else [4, 5]
     nel[axis] = 3
-    mesh, std, equation, op, orc = build(3, 6, nel, "split", "ch", periodic=periodic,
-                                         finish=[2 * np.pi] * 3)
+    n = 6 if nd == 3 else 8
+    mesh, std, equation, op, orc = build(nd, n, nel, "split", "ch", periodic=periodic,
+                                         finish=[2 * np.pi] * nd)
     lo, hi = str(2 * axis + 1), str(2 * axis + 2)
-    qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7])
+    qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7]) if nd == 3 else np.array([1.1, 0.3, -0.05, 2.7])
     X, FX = orc.coords()
     if kind == "inflow-outflow":
         bcs = {lo: F.EulerInflowBC(qin), hi: F.EulerOutflowBC()}
@@ -381,12 +391,13 @@ def test_fp32_fast_kernel_boundaries_match_oracle(kind, axis):
         orc.set_bc(1, O.BC_OUTFLOW)
     else:
         def qext(x):
-            r = 1.0 + 0.1 * np.sin(x[1]) * np.cos(x[2])
-            return np.array([r, 0.2 * r, 0.05 * r, -0.03 * r, 2.5 + 0.1 * np.cos(x[0] + x[2])])
+            r = 1.0 + 0.1 * np.sin(x[1]) * np.cos(x[-1])
+            mom = [0.2 * r, 0.05 * r, -0.03 * r][:nd]
+            return np.array([r] + mom + [2.5 + 0.1 * np.cos(x[0] + x[-1])])
 
         bcs = {lo: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x)),
                hi: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x))}
-        bc = np.zeros((orc.nf, orc.npf, 5))
+        bc = np.zeros((orc.nf, orc.npf, nd + 2))
         for f in range(orc.nf):
             for k in range(orc.npf):
                 bc[f, k] = qext(FX[f, k])
@@ -394,7 +405,7 @@ def test_fp32_fast_kernel_boundaries_match_oracle(kind, axis):
         orc.set_bc(1, O.BC_DIRICHLET, bc)
     disc = F.MultielementDisc(mesh, std, equation, op, bcs, precision="fp32")
     assert disc.uses_fast_kernel()
-    Q = _state(disc, orc, 3, "euler", seed=21, amp=0.1)
+    Q = _state(disc, orc, nd, "euler", seed=21, amp=0.1)
     ref = orc.rhs(Q)
     Qd = disc.to_device(Q)
     out = {}
@@ -409,16 +420,20 @@ def test_fp32_fast_kernel_boundaries_match_oracle(kind, axis):
     assert e_fast < 2e-5 and e_plain < 2e-5
 
 
-@pytest.mark.parametrize("n,nel", [(6, [3, 3, 3]), (5, [2, 2, 2]), (4, [3, 1, 2]), (6, [1, 1, 1])],
-                         ids=["n6-27el", "n5-8el", "n4-6el", "n6-1el"])
+@pytest.mark.parametrize("n,nel", [(6, [3, 3, 3]), (5, [2, 2, 2]), (4, [3, 1, 2]), (6, [1, 1, 1]),
+                                   (8, [3, 5]), (7, [2, 5]), (5, [5, 7]), (4, [3, 3]), (6, [1, 1])],
+                         ids=["n6-27el", "n5-8el", "n4-6el", "n6-1el",
+                              "2d-n8-15el", "2d-n7-10el", "2d-n5-35el", "2d-n4-9el", "2d-n6-1el"])
 def test_fp32_fast_kernel_partial_blocks(n, nel):
-    """Element counts that are not multiples of the elements per block (2 at N=6, 3 at N=5,
-    4 at N=4), and a single element: the last block's missing elements must neither be
-    read nor written, and the stage forms must agree with the plain FP32 kernel."""
-    mesh, std, equation, op, orc = build(3, n, nel, "split", "ch", finish=[2 * np.pi] * 3)
+    """Element counts that are not multiples of the elements per block (3-D: 2 at N=6, 3 at
+    N=5, 4 at N=4; 2-D: 8 at N=8, 9 at N=7, 32 at N=5, 16 at N=4 and N=6), and a single
+    element: the last block's missing elements must neither be read nor written, and the
+    stage forms must agree with the plain FP32 kernel."""
+    nd = len(nel)
+    mesh, std, equation, op, orc = build(nd, n, nel, "split", "ch", finish=[2 * np.pi] * nd)
     disc = F.MultielementDisc(mesh, std, equation, op, precision="fp32")
     assert disc.uses_fast_kernel()
-    Q = _state(disc, orc, 3, "euler", seed=31, amp=0.1)
+    Q = _state(disc, orc, nd, "euler", seed=31, amp=0.1)
     ref = orc.rhs(Q)
     Qd = disc.to_device(Q)
     dQd = disc.new_state()
