@@ -68,7 +68,7 @@ struct flou_hip_handle {
     VjpLauncherF64 v64 = nullptr;
     VjpLauncherF32 v32 = nullptr;
     std::string vjp_why;  // why the adjoint is unavailable for this handle
-    // FP32 fast path (2-D/3-D split form on axis-aligned affine elements)
+    // FP32 fast path (2-D/3-D split form on axis-aligned affine or curved elements)
     FastLauncher lfast = nullptr;
     int force_plain = 0;
     void* d_fegeom = nullptr;
@@ -263,6 +263,7 @@ int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, d
         a.egeom = (const float*)h->d_fegeom;
         a.egeom_stride = h->fegeom_stride;
         a.fgeom = (const float*)h->d_fgeom;
+        a.ngeom = (const float*)h->d_ngeom;
         a.dsharp = (const float*)h->d_fdsharp;
         a.lift_m = h->lift_m;
         a.lift_p = h->lift_p;
@@ -579,12 +580,13 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     if (!h->v64 && !h->v32 && h->vjp_why.empty()) h->vjp_why = "adjoint: no kernel for this combination";
 
     // FP32 fast path: 2-D and 3-D split-form Euler with the StdAverage/LxF surface flux on
-    // axis-aligned affine elements (diagonal metric, frames n = ±e_d), GLL traces
+    // axis-aligned affine elements (diagonal metric, frames n = ±e_d) or on curved elements
+    // (per-node metric and J, per-face-node frames), GLL traces
     // (l = e_1/e_n and ∂g supported on the end nodes up to 1e-12) and a vanishing D♯
     // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable).  Anything
     // else runs the plain kernel in FP32.
     std::vector<float> fe, fds;
-    if (!h->curved && h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
+    if (h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
         h->vol == FLOU_HIP_SPLIT && h->savg == FLOU_HIP_AVG_STD &&
         (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
         h->ndofs < (int64_t(1) << 30)) {
@@ -599,19 +601,20 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             if (std::fabs(d->Dsharp[j * n + j]) > 1e-10 * dmax) ok = false;
         }
         const int nd2 = nd * nd;
-        for (int64_t e = 0; e < ne && ok; ++e)
+        // affine elements must be axis-aligned; curved ones carry their metric per node
+        for (int64_t e = 0; e < ne && ok && !h->curved; ++e)
             for (int r = 0; r < nd; ++r)
                 for (int c = 0; c < nd; ++c)
                     if (r != c && d->elem_metric[e * nd2 + r + nd * c] != 0.0) ok = false;
-        for (int64_t f = 0; f < nf && ok; ++f) {
+        for (int64_t f = 0; f < nf && ok && !h->curved; ++f) {
             const int dd = d->face_elempos[f * 2] >> 1;
             for (int c = 0; c < nd; ++c) {
                 const double nc = d->face_frames[f * 3 * nd + c];
                 if (c == dd ? std::fabs(nc) != 1.0 : nc != 0.0) ok = false;
             }
         }
-        if (ok) h->lfast = fast_launcher(nd, n, h->tp);
-        if (h->lfast) {
+        if (ok) h->lfast = fast_launcher(nd, n, h->tp, h->curved != 0);
+        if (h->lfast && !h->curved) {
             // per element: Ja_cc/J (slots 0..nd-1), 1/J (slot 3), 1/Ja_cc (slots 4..3+nd)
             const int64_t cnt = uniform ? 1 : ne;
             fe.assign(cnt * 8, 0.0f);
@@ -625,6 +628,8 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                 fe[e * 8 + 3] = (float)(1.0 / J);
             }
             h->fegeom_stride = uniform ? 0 : 8;
+        }
+        if (h->lfast) {
             fds.resize(n * n);
             for (int i = 0; i < n * n; ++i) fds[i] = (float)d->Dsharp[i];
             h->lift_m = (float)d->dg_minus[0];
@@ -645,7 +650,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         }
     }
     if (h->lfast) {
-        int rcf = upload(h, &h->d_fegeom, fe);
+        int rcf = fe.empty() ? 0 : upload(h, &h->d_fegeom, fe);
         rcf |= upload(h, &h->d_fdsharp, fds);
         if (rcf) return bad(FLOU_HIP_ENOMEM, "device upload failed: " + h->err);
     }

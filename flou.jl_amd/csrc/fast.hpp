@@ -21,7 +21,8 @@ struct FastArgs {
     const int* fid;     // ne*2nd mesh face index
     const float* egeom; // per element: Ja_dd/J (d < nd, slots 0..2), 1/J (slot 3), 1/Ja_dd (slots 4..6)
     int egeom_stride;   // 0 when every element is alike, else 8
-    const float* fgeom; // per entry: frame (n, t, b) then surface J (3nd + 1 floats)
+    const float* fgeom; // per entry: frame (n, t, b) then surface J (3nd + 1 floats); curved: per face node
+    const float* ngeom; // curved: per node, component-major [nd*nd + 1][ndofs] (metric, J)
     const float* dsharp;// D♯, row-major n x n
     float lift_m, lift_p; // ∂g_minus[0], ∂g_plus[n-1]
     const float* bcstate;
@@ -34,7 +35,8 @@ struct FastArgs {
 
 typedef hipError_t (*FastLauncher)(const FastArgs&, hipStream_t);
 
-// 2-D and 3-D Euler split form, n = 4..8 nodes per direction, tp = two-point flux.
-FastLauncher fast_launcher(int nd, int n, int tp);
+// 2-D and 3-D Euler split form, n = 4..8 nodes per direction, tp = two-point flux, on
+// axis-aligned affine elements or (curved) with per-node metric and J.
+FastLauncher fast_launcher(int nd, int n, int tp, bool curved);
 
 }  // namespace flouhip

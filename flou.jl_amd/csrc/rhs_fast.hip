@@ -652,6 +652,490 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     if (err) atomicOr(a.errflag, err);
 }
 
+// ======================================================================================
+// Curved elements (per-node metric and J): the same block organisation, with the metric
+// inside the two-point flux, F♯(i, j) = f♯(Q_i, Q_j)·(Ja^d_i + Ja^d_j)/2
+// (Equations/Euler.jl:385-536, PhysicalRegions.jl:1001-1033), the numerical flux along
+// the face-node normal n with the surface Jacobian (interface_fluxes!, Interfaces.jl:111-136),
+// and 1/J per node (apply_massmatrix!).  Rows are kept in physical variable order.
+
+// Elements per block: the line tasks fill most of the block's waves.
+template <int ND, int N>
+constexpr int fastc_epb() {
+    return ND == 3 ? (N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? 2 : 1)))
+                   : (N == 4 ? 16 : (N == 5 ? 32 : (N == 6 ? 16 : (N == 7 ? 9 : 8))));
+}
+
+template <int ND, int N>
+struct CCfg {
+    static constexpr int NV = ND + 2, NFC = 2 * ND;
+    static constexpr int NPE = ipow(N, ND), NPF = ipow(N, ND - 1), NL = ND * NPF;
+    static constexpr int EPB = fastc_epb<ND, N>();
+    static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;
+    static constexpr int NN = EPB * NPE;
+    static constexpr int NPT = (NN + TPB - 1) / TPB;
+    // LDS slot arrays [slot][NN]: Q (NV), P (density, ND velocities, 3 scalars), M (the
+    // ND contravariant metric vectors Ja^d/2, ND*ND), and 1/J.  The row sums of the ND
+    // directions (ND*NV slots) go over Q, P and M once the lines are done; 1/J stays.
+    static constexpr int PIN = ND + 4;
+    static constexpr int SLOTS = NV + PIN + ND * ND + 1;
+    static_assert(ND * NV <= SLOTS - 1, "row sums must fit below the 1/J slot");
+    static constexpr int OFF_Q = 0;
+    static constexpr int OFF_P = NV * NN;
+    static constexpr int OFF_M = OFF_P + PIN * NN;
+    static constexpr int OFF_IJ = OFF_M + ND * ND * NN;
+    static constexpr int LDS = r4(SLOTS * NN);
+};
+
+// Two-point flux along n̄ = Mi + Mj (M = Ja^d/2) in physical variables.  Chandrasekhar
+// returns (m, F_m.., h/2) with m = ρ̂ ū·n̄ and the energy factored, F_E = m·h (the
+// energy row accumulated halved); Kennedy-Gruber and StdAverage return the flux.
+template <int ND, int TP, int PATH>
+__device__ __forceinline__ void twopoint_c(const float* Pi, const float* Pj, const float* Mi, const float* Mj,
+                                           float inv_gm1, float* F) {
+    constexpr int NV = ND + 2, SB = ND + 1;
+    float n[ND], ub[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) { n[k] = Mi[k] + Mj[k]; ub[k] = Pi[1 + k] + Pj[1 + k]; }
+    if constexpr (TP == TP_CHANDRASEKHAR) {
+        float un = ub[0] * n[0];
+#pragma unroll
+        for (int k = 1; k < ND; ++k) un = fmaf(ub[k], n[k], un);
+        float dot = Pi[ND] * Pj[ND];
+#pragma unroll
+        for (int k = ND - 1; k >= 1; --k) dot = fmaf(Pi[k], Pj[k], dot);
+        const float hs = Pi[0] + Pj[0], dA = Pi[0] - Pj[0];
+        const float db = Pi[SB] - Pj[SB];
+        float rho, X, rb;
+        if constexpr (PATH == 2) {
+            // first order in u on both log-means, no reciprocal per pair (see twopoint)
+            const float sh = Pi[SB + 2] + Pj[SB + 2];
+            const float fh = db * sh;
+            const float uh = fh * fh;
+            rb = fmaf(-sh, uh, sh);
+            const float fr = dA * (Pi[SB + 1] + Pj[SB + 1]);
+            X = fmaf(uh, inv_gm1 * (1.0f / 6.0f), fmaf(fr * fr, 1.5f, 0.5f * (inv_gm1 + 1.0f)));
+            rho = fmaf(dA, fr, hs);
+        } else {
+            rb = rcp(Pi[SB] + Pj[SB]);
+            const float fb = db * rb, ub2 = fb * fb;
+            const float fr = dA * rcp(hs);
+            const float ur = fr * fr;
+            float Fr, Fb;
+            if constexpr (PATH == 3) {
+                Fr = fmaf(ur, fmaf(ur, fmaf(ur, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
+                Fb = fmaf(ub2, fmaf(ub2, fmaf(ub2, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
+            } else {
+                if (ur < 0.01f) Fr = fmaf(ur, fmaf(ur, fmaf(ur, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
+                else Fr = 0.5f * (lg(Pi[0]) - lg(Pj[0])) * rcp(fr);
+                if (ub2 < 0.01f) Fb = fmaf(ub2, fmaf(ub2, fmaf(ub2, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
+                else Fb = 0.5f * (lg(Pi[SB]) - lg(Pj[SB])) * rcp(fb);
+            }
+            X = fmaf(Fb, 0.5f * inv_gm1, 0.5f * Fr);
+            rho = hs * rcp(Fr);
+        }
+        const float p = hs * rb;
+        const float m = rho * un;
+        F[0] = m;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) F[1 + k] = fmaf(m, ub[k], p * n[k]);
+        F[NV - 1] = fmaf(rb, X, dot);
+    } else if constexpr (TP == TP_KENNEDY_GRUBER) {
+        float un = ub[0] * n[0];
+#pragma unroll
+        for (int k = 1; k < ND; ++k) un = fmaf(ub[k], n[k], un);
+        const float rho = Pi[0] + Pj[0];
+        const float p = Pi[SB] + Pj[SB];
+        const float e = Pi[SB + 1] + Pj[SB + 1];
+        const float m = rho * un;
+        F[0] = m;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) F[1 + k] = fmaf(m, ub[k], p * n[k]);
+        F[NV - 1] = fmaf(m, e, p * un);
+    } else {
+        // ½(f(Q_i) + f(Q_j))·n̄ with full velocities; Pl = (ρ, u.., p, ρe)
+        float uni = Pi[1] * n[0], unj = Pj[1] * n[0];
+#pragma unroll
+        for (int k = 1; k < ND; ++k) { uni = fmaf(Pi[1 + k], n[k], uni); unj = fmaf(Pj[1 + k], n[k], unj); }
+        const float mi = Pi[0] * uni, mj = Pj[0] * unj;
+        F[0] = 0.5f * (mi + mj);
+#pragma unroll
+        for (int k = 0; k < ND; ++k)
+            F[1 + k] = 0.5f * (fmaf(mi, Pi[1 + k], Pi[SB] * n[k]) + fmaf(mj, Pj[1 + k], Pj[SB] * n[k]));
+        F[NV - 1] = 0.5f * ((Pi[SB + 1] + Pi[SB]) * uni + (Pj[SB + 1] + Pj[SB]) * unj);
+    }
+}
+
+template <int ND, int N, int TP, int PATH>
+__device__ __forceinline__ void line_pairs_c(const float (&P)[N][ND + 4], const float (&M)[N][ND],
+                                             float (&acc)[N][ND + 2], const float* __restrict__ Dsh, float inv_gm1) {
+    constexpr int NV = ND + 2;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) {
+            float F[NV];
+            twopoint_c<ND, TP, PATH>(P[i], P[j], M[i], M[j], inv_gm1, F);
+            const float cij = Dsh[i * N + j], cji = Dsh[j * N + i];
+            const bool setj = i == 0 && j < N - 1;  // first contribution to row j
+            if constexpr (TP == TP_CHANDRASEKHAR) {
+                const float mi = cij * F[0], mj = cji * F[0];
+                acc[i][0] -= mi;
+                acc[j][0] = setj ? -mj : acc[j][0] - mj;
+#pragma unroll
+                for (int v = 1; v <= ND; ++v) {
+                    acc[i][v] = fmaf(-cij, F[v], acc[i][v]);
+                    acc[j][v] = setj ? -cji * F[v] : fmaf(-cji, F[v], acc[j][v]);
+                }
+                acc[i][NV - 1] = fmaf(-mi, F[NV - 1], acc[i][NV - 1]);
+                acc[j][NV - 1] = setj ? -mj * F[NV - 1] : fmaf(-mj, F[NV - 1], acc[j][NV - 1]);
+            } else {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    acc[i][v] = fmaf(-cij, F[v], acc[i][v]);
+                    acc[j][v] = setj ? -cji * F[v] : fmaf(-cji, F[v], acc[j][v]);
+                }
+            }
+        }
+    }
+}
+
+// Numerical flux at a line end in physical variables, as the own element sees it:
+// acc[R] = -coef·jac·[(f(Q_own) + f(Q_ext))·n_own + λ·diss·(Q_own - Q_ext)], with n_own the
+// own outward unit normal (the master side's normal, negated on the slave side) and
+// λ = max(|u·n| + c).  Both elements of a face evaluate the same sums with opposite n, so
+// their values are exactly opposite.  The energy entry is scaled by ehalf.
+template <int ND, int N, int R>
+__device__ __forceinline__ void face_init_c(const float (&o)[ND + 2], const float (&x)[ND + 2], const float* n,
+                                            float coef, float gm1, float gamma, float diss, float ehalf,
+                                            float (&acc)[N][ND + 2], bool& bad) {
+    constexpr int NV = ND + 2;
+    float fs[NV], a2[2];
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+        const float* Q = side ? x : o;
+        const float ir = rcp(Q[0]);
+        float m2 = Q[ND] * Q[ND], mn = Q[ND] * n[ND - 1];
+#pragma unroll
+        for (int k = ND - 1; k >= 1; --k) { m2 = fmaf(Q[k], Q[k], m2); mn = fmaf(Q[k], n[k - 1], mn); }
+        const float p = gm1 * fmaf(-0.5f * m2, ir, Q[NV - 1]);
+        const float un = mn * ir;
+        const float c2 = gamma * p * ir;
+        bad |= !(c2 >= 0.0f);
+        a2[side] = fabsf(un) + sq(c2);
+        if (side == 0) {
+            fs[0] = mn;
+#pragma unroll
+            for (int k = 1; k <= ND; ++k) fs[k] = fmaf(Q[k], un, p * n[k - 1]);
+            fs[NV - 1] = (Q[NV - 1] + p) * un;
+        } else {
+            fs[0] += mn;
+#pragma unroll
+            for (int k = 1; k <= ND; ++k) fs[k] += fmaf(Q[k], un, p * n[k - 1]);
+            fs[NV - 1] += (Q[NV - 1] + p) * un;
+        }
+    }
+    const float cl = coef * fmaxf(a2[0], a2[1]) * diss;
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const float t = fmaf(-coef, fs[v], -cl * (o[v] - x[v]));
+        acc[R][v] = v == NV - 1 ? ehalf * t : t;
+    }
+}
+
+template <int ND, int N, int TP, int MODE, bool FIRST>
+__global__ void __launch_bounds__((CCfg<ND, N>::TPB)) k_fastc(const FastArgs a) {
+    using C = CCfg<ND, N>;
+    constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
+    constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
+    constexpr int FG = 3 * ND + 1;
+    __shared__ float4 smem[(C::LDS + 3) / 4];
+    float* const sm = reinterpret_cast<float*>(smem);
+    float* const sQ = sm + C::OFF_Q;
+    float* const sP = sm + C::OFF_P;
+    float* const sM = sm + C::OFF_M;
+    float* const sIJ = sm + C::OFF_IJ;
+
+    const int tid = threadIdx.x;
+    const int e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int64_t Nd = a.ndofs;
+    const float gamma = a.gamma, gm1 = a.gamma - 1.0f, inv_gm1 = a.inv_gm1;
+    const int nel = a.ne - e0 < EPB ? a.ne - e0 : EPB;
+    const uint32_t n0 = (uint32_t)e0 * NPE;
+    int err = 0;
+
+    // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
+    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
+    const int lle = tl / NL, lt = tl - (tl / NL) * NL;
+    const int ldir = lt / NPF, lf = lt - (lt / NPF) * NPF;
+    const bool line_in = lle < nel;
+    const bool line_ok = tid < EPB * NL && line_in;
+    int xnb[2], xinf[2], xfg[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const int slot = (e0 + lle) * NFC + 2 * ldir + sd;
+        xnb[sd] = line_ok ? a.nb[slot] : -2;
+        xinf[sd] = line_ok ? a.finfo[slot] : 0;
+        xfg[sd] = line_ok ? a.fgi[slot] : 0;
+    }
+
+    // ---- staging: node states, per-node metric and J -----------------------------------
+    const uint32_t blk_bytes = (uint32_t)(nel * NPE) * 4u;
+    float q[NPT][NV], g[NPT][ND * ND + 1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t rq = make_rsrc(a.Q + v * Nd + n0, blk_bytes);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
+    }
+#pragma unroll
+    for (int c = 0; c < ND * ND + 1; ++c) {
+        const rsrc_t rg = make_rsrc(a.ngeom + c * Nd + n0, blk_bytes);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) g[r][c] = bload(rg, (uint32_t)(tid + r * TPB) * 4u);
+    }
+    // face normals and surface Jacobians of both line ends (per face node, master side)
+    float fn[2][ND], fj[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const float* fg = a.fgeom + ((int64_t)xfg[sd] * NPF + lf) * FG;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) fn[sd][k] = line_ok ? fg[k] : 0.0f;
+        fj[sd] = line_ok ? fg[3 * ND] : 0.0f;
+    }
+    // exterior traces
+    uint32_t xo[2];
+    bool inblk[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const int kp = finfo_nbpos(xinf[sd]);
+        inblk[sd] = xnb[sd] >= e0 && xnb[sd] < e0 + nel;
+        xo[sd] = xnb[sd] >= 0 && !inblk[sd]
+                     ? ((uint32_t)xnb[sd] * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
+                     : 0xffffffffu;
+    }
+    float xr[2][NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) xr[sd][v] = bload(rv, xo[sd]);
+    }
+    // flux inputs, metric and 1/J to LDS
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int j = tid + r * TPB;
+        if (j >= NN) continue;
+        const float rho = q[r][0];
+        const float ir = rcp(rho);
+        float u[ND];
+        const float uscale = TP == TP_STDAVG ? ir : 0.5f * ir;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * uscale;
+        float uu = u[ND - 1] * u[ND - 1];
+#pragma unroll
+        for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
+        const float p = gm1 * fmaf((TP == TP_STDAVG ? -0.5f : -2.0f) * rho, uu, q[r][NV - 1]);
+        if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f) || !(g[r][ND * ND] > 0.0f))) err |= ERR_LOG_NEG;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sQ[c * NN + j] = q[r][c];
+        sP[j] = TP == TP_STDAVG ? rho : 0.5f * rho;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sP[(1 + k) * NN + j] = u[k];
+        if constexpr (TP == TP_CHANDRASEKHAR) {
+            sP[(ND + 1) * NN + j] = 0.5f * rho * rcp(p);
+            sP[(ND + 2) * NN + j] = (-1.0f / 6.0f) * ir;
+            sP[(ND + 3) * NN + j] = p * (0.5f * ir);
+        } else if constexpr (TP == TP_KENNEDY_GRUBER) {
+            sP[(ND + 1) * NN + j] = 0.5f * p;
+            sP[(ND + 2) * NN + j] = 0.5f * q[r][NV - 1] * ir;
+        } else {
+            sP[(ND + 1) * NN + j] = p;
+            sP[(ND + 2) * NN + j] = q[r][NV - 1];
+        }
+        // metric: column d of the node's matrix is Ja^d; stored halved per direction
+#pragma unroll
+        for (int c = 0; c < ND * ND; ++c) sM[c * NN + j] = 0.5f * g[r][c];
+        sIJ[j] = rcp(g[r][ND * ND]);
+    }
+    // boundary faces: Dirichlet or inflow states, or the own node (outflow, slip)
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        if (xnb[sd] == -1) {
+            const int k = 2 * ldir + sd;
+            const int bd = finfo_bd(xinf[sd]);
+            const int type = a.bctype[bd];
+            const float* src;
+            int64_t vs = 1;
+            if (type == BC_DIRICHLET) {
+                src = a.bcstate + ((int64_t)a.fid[(e0 + lle) * NFC + k] * NPF + lf) * NV;
+            } else if (type == BC_INFLOW) {
+                src = a.bcconst + bd * NV;
+            } else {
+                src = a.Q + n0 + lle * NPE + line_node<ND, N>(ldir, lf, sd ? N - 1 : 0);
+                vs = Nd;
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xr[sd][v] = src[v * vs];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        if (inblk[sd]) {
+            const int kp = finfo_nbpos(xinf[sd]);
+            const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xr[sd][v] = sQ[v * NN + node];
+        }
+    }
+    // a slip wall reflects the normal momentum: m - 2(m·n)n
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        if (xnb[sd] == -1 && a.bctype[finfo_bd(xinf[sd])] == BC_SLIP) {
+            float mn = xr[sd][1] * fn[sd][0];
+#pragma unroll
+            for (int k = 1; k < ND; ++k) mn = fmaf(xr[sd][1 + k], fn[sd][k], mn);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) xr[sd][1 + k] = fmaf(-2.0f * mn, fn[sd][k], xr[sd][1 + k]);
+        }
+    }
+
+    // ---- line tasks --------------------------------------------------------------------
+    const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
+    const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    float acc[N][NV];
+    {
+        float P[N][ND + 4], M[N][ND];
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const int nd = base + p * stride;
+#pragma unroll
+            for (int c = 0; c < ND + 3; ++c) P[p][c] = sP[c * NN + nd];
+            P[p][ND + 3] = TP == TP_CHANDRASEKHAR ? sP[(ND + 3) * NN + nd] : 0.0f;
+#pragma unroll
+            for (int k = 0; k < ND; ++k) M[p][k] = sM[(k + ND * ldir) * NN + nd];
+        }
+        float o0[NV], o1[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            o0[v] = sQ[v * NN + base];
+            o1[v] = sQ[v * NN + base + (N - 1) * stride];
+        }
+        const float ehalf = TP == TP_CHANDRASEKHAR ? 0.5f : 1.0f;
+        bool bad = false;
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            // own outward normal: the master's normal, negated on the slave side
+            const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
+            float nown[ND];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) nown[k] = s * fn[sd][k];
+            const float coef = 0.5f * fj[sd] * (sd ? a.lift_p : a.lift_m);
+            if (sd == 0) face_init_c<ND, N, 0>(o0, xr[0], nown, coef, gm1, gamma, a.diss, ehalf, acc, bad);
+            else face_init_c<ND, N, N - 1>(o1, xr[1], nown, coef, gm1, gamma, a.diss, ehalf, acc, bad);
+        }
+        if (bad && line_ok) err |= ERR_SQRT_NEG;
+        if constexpr (TP == TP_CHANDRASEKHAR) {
+            constexpr int SB = ND + 1;
+            float amax = P[0][0], amin = P[0][0], bmax = P[0][SB], bmin = P[0][SB];
+#pragma unroll
+            for (int p = 1; p < N; ++p) {
+                amax = fmaxf(amax, P[p][0]); amin = fminf(amin, P[p][0]);
+                bmax = fmaxf(bmax, P[p][SB]); bmin = fminf(bmin, P[p][SB]);
+            }
+            const float da = amax - amin, sa = amax + amin, db = bmax - bmin, sb = bmax + bmin;
+            const float qa = da * da * 1.0001f, pa = sa * sa, qb = db * db * 1.0001f, pb = sb * sb;
+            if (qa < 1e-4f * pa && qb < 1e-4f * pb) line_pairs_c<ND, N, TP, 2>(P, M, acc, a.dsharp, inv_gm1);
+            else if (qa < 1e-2f * pa && qb < 1e-2f * pb) line_pairs_c<ND, N, TP, 3>(P, M, acc, a.dsharp, inv_gm1);
+            else line_pairs_c<ND, N, TP, 0>(P, M, acc, a.dsharp, inv_gm1);
+#pragma unroll
+            for (int p = 0; p < N; ++p) acc[p][NV - 1] += acc[p][NV - 1];  // energy row was halved
+        } else {
+            line_pairs_c<ND, N, TP, 0>(P, M, acc, a.dsharp, inv_gm1);
+        }
+    }
+
+    const float* aux = (MODE == MODE_2N && !FIRST) ? a.reg
+                     : ((MODE == MODE_SSP2 || MODE == MODE_SSP3) ? a.ubase : nullptr);
+    float xv[NPT][NV], uq[NPT][NV];
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int j = tid + r * TPB;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) uq[r][v] = sQ[v * NN + (j < NN ? j : 0)];
+    }
+    __syncthreads();  // inputs are dead from here; 1/J stays
+    if (line_ok) {
+        // row sums of direction d in slots [d*NV, d*NV + NV)
+        float* R = sm + ldir * NV * NN;
+#pragma unroll
+        for (int p = 0; p < N; ++p)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) R[c * NN + base + p * stride] = acc[p][c];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t rx = make_rsrc(aux ? aux + v * Nd + n0 : a.Q, aux ? blk_bytes : 0u);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) xv[r][v] = bload(rx, (uint32_t)(tid + r * TPB) * 4u);
+    }
+    __syncthreads();
+
+    // ---- per node: (Σ_d row sums)/J and the stage epilogue ----------------------------
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t ro = make_rsrc(a.out + v * Nd + n0, blk_bytes);
+        const rsrc_t rr = make_rsrc(a.reg + v * Nd + n0, MODE == MODE_2N ? blk_bytes : 0u);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) {
+            const int j = tid + r * TPB;
+            const int jj = j < NN ? j : 0;
+            float t = sm[v * NN + jj];
+#pragma unroll
+            for (int d = 1; d < ND; ++d) t += sm[(d * NV + v) * NN + jj];
+            const float k = t * sIJ[jj];
+            const uint32_t off = (uint32_t)j * 4u;
+            const float uin = uq[r][v], xx = xv[r][v];
+            float out;
+            if constexpr (MODE == MODE_RHS) {
+                out = k;
+            } else if constexpr (MODE == MODE_2N) {
+                const float tt = FIRST ? a.dt * k : fmaf(a.A, xx, a.dt * k);
+                bstore(rr, off, tt);
+                out = fmaf(a.B, tt, uin);
+            } else if constexpr (MODE == MODE_SSP1) {
+                out = fmaf(a.dt, k, uin);
+            } else if constexpr (MODE == MODE_SSP2) {
+                out = 0.25f * (3.0f * xx + uin + a.dt * k);
+            } else {
+                out = (1.0f / 3.0f) * (xx + 2.0f * uin + 2.0f * a.dt * k);
+            }
+            bstore(ro, off, out);
+        }
+    }
+    if (err) atomicOr(a.errflag, err);
+}
+
+template <int ND, int N, int TP>
+static hipError_t launch_c(const FastArgs& a, hipStream_t s) {
+    using C = CCfg<ND, N>;
+    const int nblocks = (a.ne + C::EPB - 1) / C::EPB;
+    const dim3 g(nblocks), b(C::TPB);
+    switch (a.mode) {
+        case MODE_RHS: hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_RHS, false>), g, b, 0, s, a); break;
+        case MODE_2N:
+            if (a.first) hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_2N, true>), g, b, 0, s, a);
+            else hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_2N, false>), g, b, 0, s, a);
+            break;
+        case MODE_SSP1: hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_SSP1, false>), g, b, 0, s, a); break;
+        case MODE_SSP2: hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_SSP2, false>), g, b, 0, s, a); break;
+        case MODE_SSP3: hipLaunchKernelGGL((k_fastc<ND, N, TP, MODE_SSP3, false>), g, b, 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 template <int ND, int N, int TP>
 static hipError_t launch(const FastArgs& a, hipStream_t s) {
     using C = FCfg<ND, N>;
@@ -681,23 +1165,37 @@ static FastLauncher pick_tp(int tp) {
     }
 }
 
-template <int ND>
+template <int ND, int N, bool CURV>
+static FastLauncher pick_tp_any(int tp) {
+    if constexpr (!CURV) {
+        return pick_tp<ND, N>(tp);
+    } else {
+        switch (tp) {
+            case TP_CHANDRASEKHAR: return &launch_c<ND, N, TP_CHANDRASEKHAR>;
+            case TP_KENNEDY_GRUBER: return &launch_c<ND, N, TP_KENNEDY_GRUBER>;
+            case TP_STDAVG: return &launch_c<ND, N, TP_STDAVG>;
+            default: return nullptr;
+        }
+    }
+}
+
+template <int ND, bool CURV>
 static FastLauncher pick_n(int n, int tp) {
     switch (n) {
-        case 4: return pick_tp<ND, 4>(tp);
-        case 5: return pick_tp<ND, 5>(tp);
-        case 6: return pick_tp<ND, 6>(tp);
-        case 7: return pick_tp<ND, 7>(tp);
-        case 8: return pick_tp<ND, 8>(tp);
+        case 4: return pick_tp_any<ND, 4, CURV>(tp);
+        case 5: return pick_tp_any<ND, 5, CURV>(tp);
+        case 6: return pick_tp_any<ND, 6, CURV>(tp);
+        case 7: return pick_tp_any<ND, 7, CURV>(tp);
+        case 8: return pick_tp_any<ND, 8, CURV>(tp);
         default: return nullptr;
     }
 }
 
 }  // namespace fast
 
-FastLauncher fast_launcher(int nd, int n, int tp) {
-    if (nd == 2) return fast::pick_n<2>(n, tp);
-    if (nd == 3) return fast::pick_n<3>(n, tp);
+FastLauncher fast_launcher(int nd, int n, int tp, bool curved) {
+    if (nd == 2) return curved ? fast::pick_n<2, true>(n, tp) : fast::pick_n<2, false>(n, tp);
+    if (nd == 3) return curved ? fast::pick_n<3, true>(n, tp) : fast::pick_n<3, false>(n, tp);
     return nullptr;
 }
 
