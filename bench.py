@@ -132,8 +132,10 @@ def adjoint_ratio(nel=128, n=6, nsteps=200, checkpoint_every=20, seed=(101, 202)
     split Chandrasekhar + LxF(StdAverage), SSPRK33 at a fixed dt (CFL 0.15 of state A; 0.3
     is beyond the SSPRK33 stability limit of this N=5 split form).
     Loss ||ρ_A(T) - ρ_B(T)||²_{L2,Jω} at T = 200 steps; gradient w.r.t. the initial state
-    by the checkpointed reverse sweep (checkpoints every 20 steps, recompute in between).
-    Returns reverse-sweep time (recompute + VJP) / forward time over the same steps."""
+    by the checkpointed reverse sweep (checkpoints every 20 steps, the last segment's stage
+    inputs kept from the forward run, recompute of the others), and with every stage input
+    kept (no recompute).  Returns gradient time (checkpointed forward + reverse sweep) /
+    forward time over the same steps, both ways."""
     import torch
     import flouhip as F
     from flouhip import adjoint as A
@@ -181,26 +183,42 @@ def adjoint_ratio(nel=128, n=6, nsteps=200, checkpoint_every=20, seed=(101, 202)
     F.timeintegrate(Q, disc, eq, F.SSPRK33, nsteps * dt, dt=dt)
     ev[1].record(stream)
     torch.cuda.synchronize()
-    # the gradient, end to end: checkpointed forward, loss cotangent, reverse sweep
-    ev[2].record(stream)
-    uT, cps = A.forward(disc, F.SSPRK33, QA, dt, nsteps, checkpoint_every)
-    ubar = torch.zeros_like(uT)
-    ubar[0] = 2.0 * wnode * (uT[0] - rhoB)
-    ev[3].record(stream)
-    A.reverse(disc, F.SSPRK33, cps, dt, nsteps, ubar, checkpoint_every)
-    ev[4].record(stream)
-    torch.cuda.synchronize()
-    disc.check()
+    nseg = -(-nsteps // checkpoint_every)
+
+    def grad(keep):
+        """The gradient, end to end: checkpointed forward, loss cotangent, reverse sweep;
+        returns (forward ms, reverse ms, gradient)"""
+        ev[2].record(stream)
+        uT, cps = A.forward(disc, F.SSPRK33, QA, dt, nsteps, checkpoint_every, keep_segments=keep)
+        ubar = torch.zeros_like(uT)
+        ubar[0] = 2.0 * wnode * (uT[0] - rhoB)
+        ev[3].record(stream)
+        A.reverse(disc, F.SSPRK33, cps, dt, nsteps, ubar, checkpoint_every)
+        ev[4].record(stream)
+        torch.cuda.synchronize()
+        disc.check()
+        return ev[2].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4]), ubar
+
+    grad(1)  # warm-up: the allocator's cache then holds the segment buffers
+    cfwd_ms, rev_ms, ubar = grad(1)
+    grad(nseg)
+    afwd_ms, arev_ms, ubar_all = grad(nseg)
     fwd_ms = ev[0].elapsed_time(ev[1])
-    cfwd_ms, rev_ms = ev[2].elapsed_time(ev[3]), ev[3].elapsed_time(ev[4])
     gnorm = float(torch.linalg.norm(ubar))
+    if not torch.equal(ubar, ubar_all):
+        raise RuntimeError("gradient from kept stage inputs differs from the recomputed one")
     return {"value": (cfwd_ms + rev_ms) / fwd_ms, "forward_ms": fwd_ms,
             "checkpointed_forward_ms": cfwd_ms, "reverse_ms": rev_ms,
             "reverse_only_ratio": rev_ms / fwd_ms, "gradient_norm": gnorm,
+            "all_stages_kept": {"ratio": (afwd_ms + arev_ms) / fwd_ms, "forward_ms": afwd_ms,
+                                "reverse_ms": arev_ms,
+                                "device_bytes": nsteps * 3 * 4 * disc.ndofs * 8},
             "workload": "2-D Euler periodic 128^2 quads on [0,2]^2, N=5, FP64, split Chandrasekhar + "
                         "LxF(StdAverage), SSPRK33 x 200 fixed-dt steps (CFL 0.15), L2 density misfit, "
-                        "checkpoints every 20 steps with recompute; value = one gradient (checkpointed "
-                        "forward + reverse sweep with recompute and VJP) / one forward run"}
+                        "checkpoints every 20 steps, the last segment's stage inputs kept from the "
+                        "forward run, the other segments recomputed; value = one gradient "
+                        "(checkpointed forward + reverse sweep with recompute and VJP) / one forward "
+                        "run; all_stages_kept: every step's stage inputs kept in HBM (no recompute)"}
 
 
 def _ssprk33_steps(disc, Q, dt, nsteps):

@@ -20,6 +20,25 @@
 namespace flouhip {
 namespace adj {
 
+// 1/x for normal x whose reciprocal is normal: on the device the hardware reciprocal refined
+// by Newton steps (two in FP64, one in FP32; within about an ulp of the division and far
+// cheaper than its IEEE sequence), on the host the division.
+template <typename T>
+FLOU_HD inline T rcp(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 8) {
+        T r = __builtin_amdgcn_rcp(x);
+        r = fma(r, fma(-x, r, T(1)), r);
+        return fma(r, fma(-x, r, T(1)), r);
+    } else {
+        const T r = __builtin_amdgcn_rcpf(x);
+        return fmaf(r, fmaf(-x, r, T(1)), r);
+    }
+#else
+    return T(1) / x;
+#endif
+}
+
 // Cotangents of the primitives (ρ, u_1..u_d, p) of a state Q, pulled back to Q:
 //   u = m/ρ,  p = (γ-1)(E - |m|²/(2ρ))   (FlouCommon/Euler.jl:142-155, 237-253)
 template <typename T, int ND>
@@ -196,7 +215,7 @@ FLOU_HD inline void twopoint_vjp1(const T* Q1, const T* Q2, const T* n, T gamma,
 // reciprocals (two divisions on the series branch).
 template <typename T>
 FLOU_HD inline void logmean_dab(T a, T b, T* L, T* da, T* db) {
-    const T s = a + b, is = T(1) / s;
+    const T s = a + b, is = rcp(s);
     const T f = (a - b) * is;
     const T u = f * f;
     T F, dF;
@@ -207,11 +226,150 @@ FLOU_HD inline void logmean_dab(T a, T b, T* L, T* da, T* db) {
         F = log(a / b) / (T(2) * f);
         dF = (T(1) / (T(1) - u) - F) / (T(2) * u);
     }
-    const T iF = T(1) / F, h = T(0.5) * iF;
+    const T iF = rcp(F), h = T(0.5) * iF;
     *L = s * h;
     const T g = T(2) * f * dF * iF * iF * is;
     *da = h - g * b;
     *db = h + g * a;
+}
+
+// logmean_dab that also returns 1/(a+b) and F, for callers that reuse them
+template <typename T>
+FLOU_HD inline void logmean_dab2(T a, T b, T* L, T* da, T* db, T* is_out, T* F_out) {
+    const T s = a + b, is = rcp(s);
+    const T f = (a - b) * is;
+    const T u = f * f;
+    T F, dF;
+    if (u < T(0.01)) {
+        F = T(1) + u * (T(1) / T(3) + u * (T(1) / T(5) + u * (T(1) / T(7))));
+        dF = T(1) / T(3) + u * (T(2) / T(5) + u * (T(3) / T(7)));
+    } else {
+        F = log(a / b) / (T(2) * f);
+        dF = (T(1) / (T(1) - u) - F) / (T(2) * u);
+    }
+    const T iF = rcp(F), h = T(0.5) * iF;
+    *L = s * h;
+    const T g = T(2) * f * dF * iF * iF * is;
+    *da = h - g * b;
+    *db = h + g * a;
+    *is_out = is;
+    *F_out = F;
+}
+
+// prim_pullback with 1/ρ and the velocity u given
+template <typename T, int ND>
+FLOU_HD inline void prim_pullback_u(T ir, const T* u, T gamma, T rb, const T* ub, T pb, T* Qb) {
+    T uu = T(0), ubu = T(0);
+    for (int k = 0; k < ND; ++k) {
+        uu += u[k] * u[k];
+        ubu += ub[k] * u[k];
+    }
+    const T g1 = gamma - T(1);
+    Qb[0] += rb - ubu * ir + g1 * pb * uu * T(0.5);
+    for (int k = 0; k < ND; ++k) Qb[1 + k] += ub[k] * ir - g1 * pb * u[k];
+    Qb[ND + 1] += g1 * pb;
+}
+
+// Per-node inputs of twopoint_vjp2p: R = (u_1 .. u_d, p, 1/ρ, 1/p, β = ρ/(2p)).
+template <typename T, int ND>
+FLOU_HD inline void node_prims(const T* Q, T gamma, T* R) {
+    const T ir = rcp(Q[0]);
+    T q = T(0);
+    for (int k = 0; k < ND; ++k) {
+        R[k] = Q[1 + k] * ir;
+        q += Q[1 + k] * R[k];
+    }
+    const T p = (gamma - T(1)) * (Q[ND + 1] - T(0.5) * q);
+    const T ip = rcp(p);
+    R[ND] = p;
+    R[ND + 1] = ir;
+    R[ND + 2] = ip;
+    R[ND + 3] = T(0.5) * Q[0] * ip;
+}
+
+// twopoint_vjp2 with the per-node primitives and reciprocals given (node_prims) and the
+// pair's reciprocals shared: 1/(β1+β2) from the β log-mean, 1/ρ̂ = 2F_ρ/(ρ1+ρ2) and
+// 1/β̂ = 2F_β/(β1+β2) from the log-mean factors (four divisions per pair on the series
+// branch).  Equal to twopoint_vjp2 up to rounding.
+template <typename T, int ND, int TP>
+FLOU_HD inline void twopoint_vjp2p(const T* Q1, const T* R1, const T* Q2, const T* R2, const T* n, T gamma,
+                                   T ig1, const T* Gb, T* Qb1, T* Qb2) {
+    if constexpr (TP == TP_STDAVG) {
+        T h[ND + 2];
+        for (int v = 0; v < ND + 2; ++v) h[v] = T(0.5) * Gb[v];
+        flux_vjp_ir<T, ND>(Q1, R1[ND + 1], n, gamma, h, Qb1);
+        flux_vjp_ir<T, ND>(Q2, R2[ND + 1], n, gamma, h, Qb2);
+        return;
+    } else {
+        const T* u1 = R1;
+        const T* u2 = R2;
+        const T ir1 = R1[ND + 1], ir2 = R2[ND + 1];
+        const T p1 = R1[ND], p2 = R2[ND];
+        T um[ND];
+        T umn = T(0), Gmn = T(0), Gmu = T(0);
+        for (int k = 0; k < ND; ++k) {
+            um[k] = (u1[k] + u2[k]) * T(0.5);
+            umn += um[k] * n[k];
+            Gmn += Gb[1 + k] * n[k];
+            Gmu += Gb[1 + k] * um[k];
+        }
+        T ub1[ND], ub2[ND];
+        T rb1, rb2, pb1, pb2;
+        if constexpr (TP == TP_KENNEDY_GRUBER) {
+            const T rho = (Q1[0] + Q2[0]) * T(0.5);
+            const T p = (p1 + p2) * T(0.5);
+            const T e = (Q1[ND + 1] * ir1 + Q2[ND + 1] * ir2) * T(0.5);
+            const T mn = rho * umn;
+            const T mnb = Gb[0] + Gmu + Gb[ND + 1] * e;
+            const T pbar = Gmn + Gb[ND + 1] * umn;
+            const T eb = Gb[ND + 1] * mn;
+            const T rhob = mnb * umn;
+            for (int k = 0; k < ND; ++k) {
+                ub1[k] = T(0.5) * (mn * Gb[1 + k] + (Gb[ND + 1] * p + mnb * rho) * n[k]);
+                ub2[k] = ub1[k];
+            }
+            const T esb = T(0.5) * eb;
+            rb1 = T(0.5) * rhob - esb * Q1[ND + 1] * ir1 * ir1;
+            rb2 = T(0.5) * rhob - esb * Q2[ND + 1] * ir2 * ir2;
+            pb1 = pb2 = T(0.5) * pbar;
+            Qb1[ND + 1] += esb * ir1;
+            Qb2[ND + 1] += esb * ir2;
+        } else {  // TP_CHANDRASEKHAR
+            const T ip1 = R1[ND + 2], ip2 = R2[ND + 2];
+            const T b1 = R1[ND + 3], b2 = R2[ND + 3];
+            T rho, beta, drho1, drho2, dbeta1, dbeta2, isr, Fr, isb, Fb;
+            logmean_dab2(Q1[0], Q2[0], &rho, &drho1, &drho2, &isr, &Fr);
+            logmean_dab2(b1, b2, &beta, &dbeta1, &dbeta2, &isb, &Fb);
+            const T irho = T(2) * Fr * isr, ibeta = T(2) * Fb * isb;
+            const T p = T(0.5) * (Q1[0] + Q2[0]) * isb;
+            T uu = T(0), sq = T(0);
+            for (int k = 0; k < ND; ++k) {
+                uu += um[k] * um[k];
+                sq += u1[k] * u1[k] + u2[k] * u2[k];
+            }
+            const T h = T(0.5) * ibeta * ig1 - T(0.25) * sq + p * irho + uu;
+            const T mn = rho * umn;
+            const T hb = Gb[ND + 1] * mn;
+            const T mnb = Gb[0] + Gmu + Gb[ND + 1] * h;
+            const T pbar = Gmn + hb * irho;
+            const T rhob = mnb * umn - hb * p * irho * irho;
+            const T betab = T(-0.5) * hb * ibeta * ibeta * ig1;
+            const T sqb2 = T(-0.5) * hb;
+            for (int k = 0; k < ND; ++k) {
+                const T c = T(0.5) * (mn * Gb[1 + k] + mnb * rho * n[k] + T(2) * hb * um[k]);
+                ub1[k] = c + sqb2 * u1[k];
+                ub2[k] = c + sqb2 * u2[k];
+            }
+            const T rp = T(0.5) * pbar * isb, bp = pbar * p * isb;
+            const T bb1 = betab * dbeta1 - bp, bb2 = betab * dbeta2 - bp;
+            rb1 = rhob * drho1 + rp + T(0.5) * bb1 * ip1;
+            rb2 = rhob * drho2 + rp + T(0.5) * bb2 * ip2;
+            pb1 = -bb1 * b1 * ip1;
+            pb2 = -bb2 * b2 * ip2;
+        }
+        prim_pullback_u<T, ND>(ir1, u1, gamma, rb1, ub1, pb1, Qb1);
+        prim_pullback_u<T, ND>(ir2, u2, gamma, rb2, ub2, pb2, Qb2);
+    }
 }
 
 // Both sides of the two-point flux VJP at once: Qb1 += (∂F♯/∂Q1)^T Gb and

@@ -24,6 +24,14 @@
 
 namespace flouhip {
 
+// LDS slot of b's side of the pair (a, b = a + k mod N) on line f, b at node nb: one slot
+// per (k, receiving node), and for even N the k = N/2 pairs (a < N/2) by (f, b - N/2).
+template <int N, int NPE>
+__device__ inline int pair_slot(int k, int nb, int f, int b) {
+    if (N % 2 == 0 && k == N / 2) return (k - 1) * NPE + f * (N / 2) + (b - N / 2);
+    return (k - 1) * NPE + nb;
+}
+
 template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
 __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     using C = Cfg<ND, N>;
@@ -38,20 +46,31 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     __shared__ T sQ[EPB][NV][NPE];
     __shared__ T sW[EPB][NV][NPE];
     __shared__ T sTb[EPB][NF][NV][NPF];
-    // Split form (Euler): both sides of every pair flux's VJP of one direction, each pair
-    // evaluated once per block ([le][line][pair][side][v]), when the table fits in 64 KB
+    // Split form (Euler): every pair flux of a direction is differentiated once, by the
+    // thread of one of its nodes -- node a owns the pairs (a, a+k mod N), k = 1..N/2 (for
+    // even N, k = N/2 only when a < N/2) -- which adds a's side of the VJP to its own sum
+    // and leaves b's side in sPo (one value per pair and variable, at pair_slot); after a
+    // barrier every node adds the sides it received.
     constexpr int NPAIR = N * (N - 1) / 2;
-    constexpr bool PS = VOL == VOL_SPLIT && EQ == EQ_EULER &&
-                        EPB * NPF * NPAIR * 2 * NV * (int)sizeof(T) <= 65536;
-    __shared__ T sPb[PS ? EPB : 1][PS ? NPF * NPAIR : 1][2][PS ? NV : 1];
+    constexpr bool PS = VOL == VOL_SPLIT && EQ == EQ_EULER && EPB * NV * NPF * NPAIR * (int)sizeof(T) <= 65536;
+    __shared__ T sPo[PS ? EPB : 1][PS ? NV : 1][PS ? NPF * NPAIR : 1];
+    // Per-node primitives for the pair phase (adj::node_prims without β, which is rebuilt
+    // from ρ and 1/p), computed once per node instead of once per pair and direction, when
+    // the extra LDS leaves the number of blocks per CU (160 KB of LDS) unchanged
+    constexpr int NR = ND + 4, NRS = ND + 3;
+    constexpr int LDS0 = (int)sizeof(T) * (NOPS + EPB * NG + 2 * EPB * NV * NPE + EPB * NF * NV * NPF +
+                                           (PS ? EPB * NV * NPF * NPAIR : 1));
+    constexpr bool PR = PS && 163840 / (LDS0 + (int)sizeof(T) * EPB * NRS * NPE) == 163840 / LDS0;
+    __shared__ T sR[PR ? EPB : 1][PR ? NRS : 1][PR ? NPE : 1];
 
     const int tid = threadIdx.x;
     const int e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
     const int64_t Nd = a.ndofs;
     const T avel[3] = {a.a0, a.a1, a.a2};
 
-    // FP64 split form, Euler, affine, 6-7 nodes per direction in 2-D and 6 in 3-D (occupancy
-    // is set by LDS there and the registers fit without spills or a lower occupancy):
+    // FP64 split form, Euler, affine, 6 nodes per direction in 3-D (occupancy is set by LDS
+    // there and the registers fit without spills or a lower occupancy; in 2-D the registers
+    // it takes would lower the occupancy):
     // everything a thread reads from HBM is issued
     // up front -- the element's nodes of Q and Y with their J, the face tables of the face
     // node it takes first in the face phase (it = tid), then (GLL: gather) that face node's
@@ -59,7 +78,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     // the epilogue operands of its own node -- so that one barrier ends the staging and the
     // face phase waits for nothing.  The arithmetic and its order are unchanged.
     constexpr bool PF = std::is_same<T, double>::value && EQ == EQ_EULER && VOL == VOL_SPLIT && !CURV &&
-                        ((ND == 2 && N >= 6 && N <= 7) || (ND == 3 && N == 6));
+                        ND == 3 && N == 6;
     constexpr int NFN = EPB * NF * NPF;  // face nodes of the block
     const int pfk = (tid % (NF * NPF)) / NPF, pff = tid % NPF, pfe = e0 + tid / (NF * NPF);
     const bool pf = PF && a.gather && tid < NFN && pfe < a.ne;
@@ -141,6 +160,18 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         }
     }
     __syncthreads();
+    if constexpr (PR) {  // read in the pair phases, after the face phase's barrier
+        for (int i = tid; i < EPB * NPE; i += TPB) {
+            const int le = i / NPE, nd = i % NPE;
+            if (e0 + le >= a.ne) continue;
+            T q[NV], r[NR];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) q[v] = sQ[le][v][nd];
+            adj::node_prims<T, ND>(q, a.gamma, r);
+#pragma unroll
+            for (int c = 0; c < NRS; ++c) sR[le][c][nd] = r[c];
+        }
+    }
 
     // ---- face nodes: cotangent of the own trace ----------------------------------------
     for (int it = tid; it < EPB * NF * NPF; it += TPB) {
@@ -297,72 +328,66 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         }
         M = Mnode;
     }
+    T ri[PS ? NR : 1];
+    if constexpr (PS) {
+        if (valid) adj::node_prims<T, ND>(qi, a.gamma, ri);
+    }
 #pragma unroll
     for (int dir = 0; dir < ND; ++dir) {
-        if constexpr (PS) {
-            // pair phase: Gb = -D♯_ab w_a - D♯_ba w_b of the pair (a < b), pulled back to both
-            const T* Dsh = &sOps[ops_Dsh(N)];
-            __syncthreads();  // the previous direction's table has been read
-            for (int k = tid; k < EPB * NPF * NPAIR; k += TPB) {
-                const int le2 = k / (NPF * NPAIR), r = k - le2 * (NPF * NPAIR);
-                const int f2 = r / NPAIR;
-                int pa = 0, pb = r - f2 * NPAIR;
-                while (pb >= N - 1 - pa) { pb -= N - 1 - pa; ++pa; }
-                pb += pa + 1;
-                if (e0 + le2 >= a.ne) continue;
-                const int na = line_node<ND, N>(dir, f2, pa), nbn = line_node<ND, N>(dir, f2, pb);
-                const T dab = Dsh[pa * N + pb], dba = Dsh[pb * N + pa];
-                T qa[NV], qb[NV], Gb[NV], Qba[NV], Qbb[NV];
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    qa[v] = sQ[le2][v][na];
-                    qb[v] = sQ[le2][v][nbn];
-                    Gb[v] = -dab * sW[le2][v][na] - dba * sW[le2][v][nbn];
-                    Qba[v] = T(0);
-                    Qbb[v] = T(0);
-                }
-                T nbar[ND];  // averaged metric (Ja_a + Ja_b)/2 of the pair
-#pragma unroll
-                for (int c = 0; c < ND; ++c) {
-                    if constexpr (CURV) {
-                        const int64_t ga = (int64_t)(e0 + le2) * NPE;
-                        nbar[c] = (a.ngeom[(c + ND * dir) * Nd + ga + na] + a.ngeom[(c + ND * dir) * Nd + ga + nbn]) /
-                                  T(2);
-                    } else {
-                        nbar[c] = sG[le2][c + ND * dir];
-                    }
-                }
-                adj::twopoint_vjp2<T, ND, TP>(qa, qb, nbar, a.gamma, Gb, Qba, Qbb);
-#pragma unroll
-                for (int v = 0; v < NV; ++v) {
-                    sPb[le2][r][0][v] = Qba[v];
-                    sPb[le2][r][1][v] = Qbb[v];
-                }
-            }
-            __syncthreads();
-            if (!valid) continue;
-        }
         int f, ii;
         node_line<ND, N>(node, dir, f, ii);
         T n[ND];
 #pragma unroll
-        for (int k = 0; k < ND; ++k) n[k] = M[k + ND * dir];
+        for (int k = 0; k < ND; ++k) n[k] = valid ? M[k + ND * dir] : T(0);
         if constexpr (PS) {
+            // pair phase: Gb = -D♯_ab w_a - D♯_ba w_b of the owned pairs, pulled back to both
             const T* Dsh = &sOps[ops_Dsh(N)];
-            for (int j = 0; j < N; ++j) {
-                if (j == ii) {
-                    T Gb[NV];
-                    const T d = Dsh[ii * N + ii];
+            const T ig1 = T(1) / (a.gamma - T(1));
+            __syncthreads();  // the previous direction's received sides have been read
+            if (valid) {
+                for (int k = 1; k <= N / 2; ++k) {
+                    if (N % 2 == 0 && k == N / 2 && ii >= N / 2) break;
+                    const int jj = ii + k < N ? ii + k : ii + k - N;
+                    const int nj = line_node<ND, N>(dir, f, jj);
+                    const T dab = Dsh[ii * N + jj], dba = Dsh[jj * N + ii];
+                    T qj[NV], rj[NR], Gb[NV], Qbb[NV];
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) Gb[v] = -d * wi[v];
-                    adj::flux_vjp<T, ND>(qi, n, a.gamma, Gb, V);
-                } else {
-                    const int lo = j < ii ? j : ii, hi = j < ii ? ii : j;
-                    const int r = f * NPAIR + lo * (2 * N - lo - 1) / 2 + (hi - lo - 1);
-                    const int side = ii < j ? 0 : 1;
+                    for (int v = 0; v < NV; ++v) {
+                        qj[v] = sQ[le][v][nj];
+                        Gb[v] = -dab * wi[v] - dba * sW[le][v][nj];
+                        Qbb[v] = T(0);
+                    }
+                    if constexpr (PR) {
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) V[v] = V[v] + sPb[le][r][side][v];
+                        for (int c = 0; c < NRS; ++c) rj[c] = sR[le][c][nj];
+                        rj[ND + 3] = T(0.5) * qj[0] * rj[ND + 2];
+                    } else {
+                        adj::node_prims<T, ND>(qj, a.gamma, rj);
+                    }
+                    T nbar[ND];  // averaged metric (Ja_a + Ja_b)/2 of the pair
+#pragma unroll
+                    for (int c = 0; c < ND; ++c)
+                        nbar[c] = CURV ? (n[c] + a.ngeom[(c + ND * dir) * Nd + (int64_t)e * NPE + nj]) / T(2) : n[c];
+                    adj::twopoint_vjp2p<T, ND, TP>(qi, ri, qj, rj, nbar, a.gamma, ig1, Gb, V, Qbb);
+                    const int slot = pair_slot<N, NPE>(k, nj, f, jj);
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) sPo[le][v][slot] = Qbb[v];
                 }
+            }
+            __syncthreads();
+            if (!valid) continue;
+            for (int k = 1; k <= N / 2; ++k) {  // the sides received from the owners ii - k
+                if (N % 2 == 0 && k == N / 2 && ii < N / 2) break;
+                const int slot = pair_slot<N, NPE>(k, node, f, ii);
+#pragma unroll
+                for (int v = 0; v < NV; ++v) V[v] = V[v] + sPo[le][v][slot];
+            }
+            const T d = Dsh[ii * N + ii];
+            if (d != T(0)) {  // zero on GLL nodes
+                T Gb[NV];
+#pragma unroll
+                for (int v = 0; v < NV; ++v) Gb[v] = -d * wi[v];
+                adj::flux_vjp<T, ND>(qi, n, a.gamma, Gb, V);
             }
         } else if constexpr (VOL == VOL_STRONG) {
             const T* Ds = &sOps[ops_Ds(N)];

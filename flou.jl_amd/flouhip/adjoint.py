@@ -9,7 +9,8 @@ plain functions over the C-ABI (include/flouhip.h):
 * ``rk_step_vjp``    -- cotangent of u_n from that of u_{n+1}  (flou_hip_rk_step_vjp)
 * ``rk_step_vjp_noalloc`` -- the same without device allocation (flou_hip_rk_step_vjp_noalloc)
 * ``gradient``       -- reverse sweep over nsteps fixed-dt steps: checkpoints every
-  ``checkpoint_every`` steps during the forward run, then per segment one forward
+  ``checkpoint_every`` steps during the forward run, which also keeps the stage inputs
+  of the last ``keep_segments`` segments; then per segment (unless kept) one forward
   recomputation that keeps the stage inputs, followed by the reverse steps.
 
 dt is a fixed schedule (no gradient through the CFL min).  All arithmetic runs in
@@ -83,35 +84,69 @@ def rk_step_vjp_noalloc(disc, solver, dt, stages, ubar, t=0.0):
     return ubar
 
 
-def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtype=None):
+class Checkpoints(list):
+    """States at the segment starts (segment i starts at step i*every), plus in ``stages``
+    the per-step stage inputs of the trailing segments the forward run kept resident:
+    stages[i][k] = [u_n, stage inputs...] of step n = i*every + k."""
+
+    def __init__(self, every):
+        super().__init__()
+        self.every = every
+        self.stages = {}
+
+
+def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtype=None, keep_segments=0):
     """Fixed-dt forward run from Q0 (not modified).  Returns (u_T, checkpoints) where
     checkpoints[i] is the state at step i*checkpoint_every, stored as checkpoint_dtype
-    (default: the working precision)."""
+    (default: the working precision).  The stage inputs of the last ``keep_segments``
+    segments stay in device memory (checkpoints.stages; one allocation of
+    checkpoint_every x stages states per segment), so that the reverse sweep reverses those
+    segments without recomputing them."""
     code = _solver_code(solver)
-    K = checkpoint_every or nsteps or 1
+    K = max(1, int(checkpoint_every or nsteps or 1))
     cdt = checkpoint_dtype or disc.torch_dtype()
     S = _nstages(code)
-    stages = [Q0.clone()] + [disc.new_state() for _ in range(S - 1)]
+    nseg = -(-nsteps // K)
+    first_kept = max(0, nseg - int(keep_segments))
+    cur = [Q0.clone()] + [disc.new_state() for _ in range(S - 1)]
     nxt = disc.new_state()
-    cps = []
-    for n in range(nsteps):
-        if n % K == 0:
-            cps.append(stages[0].to(cdt, copy=True))
-        rk_step_record(disc, solver, dt, stages, nxt)
-        stages[0], nxt = nxt, stages[0]
-    return stages[0], cps
+    cps = Checkpoints(K)
+    for sg in range(nseg):
+        nloc = min(K, nsteps - sg * K)
+        cps.append(cur[0].to(cdt, copy=True))
+        if sg < first_kept:
+            for _ in range(nloc):
+                rk_step_record(disc, solver, dt, cur, nxt)
+                cur[0], nxt = nxt, cur[0]
+            continue
+        buf = torch.empty((nloc, S, disc.nvars, disc.ndofs), dtype=disc.torch_dtype(), device=Q0.device)
+        steps = [[buf[i, s] for s in range(S)] for i in range(nloc)]
+        steps[0][0].copy_(cur[0])
+        for i in range(nloc):
+            rk_step_record(disc, solver, dt, steps[i], steps[i + 1][0] if i + 1 < nloc else nxt)
+        cur[0], nxt = nxt, cur[0]
+        cps.stages[sg] = steps
+    return cur[0], cps
 
 
 def reverse(disc, solver, checkpoints, dt, nsteps, ubar, checkpoint_every):
     """Reverse sweep: ubar (cotangent of u_T, modified in place) -> cotangent of u_0, from
-    the checkpoints of ``forward``.  Per segment one forward recomputation that keeps the
-    stage inputs, then the reverse steps."""
+    the checkpoints of ``forward``.  A segment whose stage inputs the forward run kept is
+    reversed directly; any other gets one forward recomputation that keeps the stage
+    inputs, then its reverse steps."""
     code = _solver_code(solver)
     S = _nstages(code)
     K = max(1, int(checkpoint_every))
-    seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
-    nxt = disc.new_state()
+    kept = getattr(checkpoints, "stages", {})
+    seg_bufs = nxt = None
     for sgi in reversed(range(len(checkpoints))):
+        if sgi in kept:
+            for st in reversed(kept[sgi]):
+                rk_step_vjp(disc, solver, dt, st, ubar)
+            continue
+        if seg_bufs is None:
+            seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
+            nxt = disc.new_state()
         n0 = sgi * K
         nloc = min(K, nsteps - n0)
         seg_bufs[0][0].copy_(checkpoints[sgi])
@@ -124,13 +159,15 @@ def reverse(disc, solver, checkpoints, dt, nsteps, ubar, checkpoint_every):
 
 
 def gradient(disc, solver, Q0, dt, nsteps, seed, checkpoint_every=20, checkpoint_dtype=None,
-             return_state=False):
+             return_state=False, keep_segments=1):
     """Cotangent of the initial state for a loss of the final state.
 
     seed(u_T) returns dL/du_T (a device state tensor).  Memory: nsteps/checkpoint_every
-    checkpoints plus checkpoint_every x (stages) states for the segment being reversed."""
+    checkpoints, the stage inputs of the last keep_segments segments (checkpoint_every x
+    stages states each, kept from the forward run, so those segments are not recomputed),
+    and checkpoint_every x stages states for a segment being recomputed."""
     K = max(1, int(checkpoint_every))
-    uT, cps = forward(disc, solver, Q0, dt, nsteps, K, checkpoint_dtype)
+    uT, cps = forward(disc, solver, Q0, dt, nsteps, K, checkpoint_dtype, keep_segments)
     ubar = seed(uT).to(disc.torch_dtype()).contiguous().clone()
     reverse(disc, solver, cps, dt, nsteps, ubar, K)
     return (ubar, uT) if return_state else ubar
@@ -161,8 +198,8 @@ class TimeIntegrate(torch.autograd.Function):
     sweep (the rrule of timeintegrate)."""
 
     @staticmethod
-    def forward(ctx, Q0, disc, solver, dt, nsteps, checkpoint_every=20):
-        uT, cps = forward(disc, solver, Q0, dt, nsteps, checkpoint_every)
+    def forward(ctx, Q0, disc, solver, dt, nsteps, checkpoint_every=20, keep_segments=1):
+        uT, cps = forward(disc, solver, Q0, dt, nsteps, checkpoint_every, keep_segments=keep_segments)
         ctx.disc, ctx.solver, ctx.dt, ctx.nsteps, ctx.K = disc, solver, dt, nsteps, checkpoint_every
         ctx.cps = cps
         return uT
@@ -172,4 +209,4 @@ class TimeIntegrate(torch.autograd.Function):
         disc = ctx.disc
         ubar = guT.to(disc.torch_dtype()).contiguous().clone()
         reverse(disc, ctx.solver, ctx.cps, ctx.dt, ctx.nsteps, ubar, ctx.K)
-        return ubar, None, None, None, None, None
+        return ubar, None, None, None, None, None, None

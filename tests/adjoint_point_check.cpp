@@ -42,13 +42,23 @@ template <int ND> double run(int tp, double spread, int which) {
         if (tp == 0) flouhip::adj::twopoint_vjp1<double, ND, 0>(Q1, Q2, n, 1.4, G, Qb);
         if (tp == 1) flouhip::adj::twopoint_vjp1<double, ND, 1>(Q1, Q2, n, 1.4, G, Qb);
         if (tp == 2) flouhip::adj::twopoint_vjp1<double, ND, 2>(Q1, Q2, n, 1.4, G, Qb);
-    } else if (which == 4) {  // two-point flux, both states at once (twopoint_vjp2)
+    } else if (which == 4 || which == 5) {  // two-point flux, both states at once
         double V2[NV];
         for (int v = 0; v < NV; ++v) { V2[v] = U(-1, 1); q2[v].d[0] = V2[v]; }
         oracle::twopointflux(d, q1, q2, M + ND * dir, M + ND * dir, F);
-        if (tp == 0) flouhip::adj::twopoint_vjp2<double, ND, 0>(Q1, Q2, n, 1.4, G, Qb, Qb2);
-        if (tp == 1) flouhip::adj::twopoint_vjp2<double, ND, 1>(Q1, Q2, n, 1.4, G, Qb, Qb2);
-        if (tp == 2) flouhip::adj::twopoint_vjp2<double, ND, 2>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+        if (which == 4) {  // twopoint_vjp2
+            if (tp == 0) flouhip::adj::twopoint_vjp2<double, ND, 0>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+            if (tp == 1) flouhip::adj::twopoint_vjp2<double, ND, 1>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+            if (tp == 2) flouhip::adj::twopoint_vjp2<double, ND, 2>(Q1, Q2, n, 1.4, G, Qb, Qb2);
+        } else {  // twopoint_vjp2p on node_prims
+            double R1[ND + 4], R2[ND + 4];
+            flouhip::adj::node_prims<double, ND>(Q1, 1.4, R1);
+            flouhip::adj::node_prims<double, ND>(Q2, 1.4, R2);
+            const double ig1 = 1.0 / 0.4;
+            if (tp == 0) flouhip::adj::twopoint_vjp2p<double, ND, 0>(Q1, R1, Q2, R2, n, 1.4, ig1, G, Qb, Qb2);
+            if (tp == 1) flouhip::adj::twopoint_vjp2p<double, ND, 1>(Q1, R1, Q2, R2, n, 1.4, ig1, G, Qb, Qb2);
+            if (tp == 2) flouhip::adj::twopoint_vjp2p<double, ND, 2>(Q1, R1, Q2, R2, n, 1.4, ig1, G, Qb, Qb2);
+        }
         double jvp = 0, vjp = 0, sc = 0;
         for (int v = 0; v < NV; ++v) {
             jvp += G[v] * F[v].d[0];
@@ -73,10 +83,10 @@ template <int ND> double run(int tp, double spread, int which) {
     return fabs(jvp - vjp) / (sc + 1e-300);
 }
 int main() {
-    const char* nm[] = {"flux", "twopoint", "lxf-left", "lxf-right", "twopoint-both"};
-    for (int which = 0; which < 5; ++which)
+    const char* nm[] = {"flux", "twopoint", "lxf-left", "lxf-right", "twopoint-both", "twopoint-both-prims"};
+    for (int which = 0; which < 6; ++which)
         for (int tp = 0; tp < 3; ++tp) {
-            if (which != 1 && which != 4 && tp > 0) continue;
+            if (which != 1 && which < 4 && tp > 0) continue;
             for (double spread : {1e-4, 0.05, 0.5}) {
                 double e1 = 0, e2 = 0, e3 = 0;
                 for (int t = 0; t < 200; ++t) {

@@ -1,6 +1,6 @@
 """Point-wise adjoint physics (flou.jl_amd/csrc/adjoint.hpp, built for the CPU) against the
 oracle's dual-number JVPs: physical flux, the three two-point fluxes (derivative in the
-first state, and both states at once), LxF in the face frame (both states), 1-D to 3-D,
+first state, and both states at once, from the states or from the per-node primitives), LxF in the face frame (both states), 1-D to 3-D,
 nearly equal and strongly different states (both branches of the log-mean)."""
 import os
 import subprocess
@@ -21,7 +21,9 @@ def test_pointwise_vjps_match_oracle_jvps(tmp_path):
         pytest.fail("build failed:\n" + r.stderr)
     out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout
     rows = [line.split() for line in out.strip().splitlines()]
-    assert len(rows) == 27  # flux, 3 two-point fluxes (one side, both sides), LxF left/right, x 3 spreads
+    # flux, 3 two-point fluxes (one side, both sides, both sides from primitives), LxF left/right,
+    # x 3 spreads
+    assert len(rows) == 36
     for name, tp, spread, e1, e2, e3 in rows:
         worst = max(float(e1), float(e2), float(e3))
         print(name, tp, spread, worst)
