@@ -10,8 +10,9 @@
 // Work unit = one column of one element (n^d contiguous values of Q, m^d of Qe: the columns
 // of the state matrix are element-major, so Q and Qe are flat arrays of units).  A block
 // stages several consecutive units with coalesced loads.  For m = n <= 8 the sizes are
-// compile-time constants: each thread contracts whole lines with A held in registers and
-// the last contraction stores coalesced runs; other sizes take a generic kernel.
+// compile-time constants: each thread contracts whole lines with A held in registers, all
+// contractions stay in LDS, and the block's units move between HBM and LDS as one
+// contiguous run of 16-byte vectors each way; other sizes take a generic kernel.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -91,32 +92,43 @@ struct FixedShape {
 };
 
 // One contraction of the fixed-size kernel: direction P of UPB units, each thread taking
-// whole lines (N inputs -> N outputs) with the N x N matrix in registers.
-template <typename T, int ND, int N, int P>
-__device__ __forceinline__ void fixed_pass(const T (&a)[N][N], T* buf, T* qe, int nu, int tid) {
+// whole lines (N inputs -> N outputs) with the N x N matrix in registers.  The two LDS
+// regions of UPB*NPE values each alternate as source and destination.
+template <typename T, int ND, int N, int P, int UPB>
+__device__ __forceinline__ void fixed_pass(const T (&a)[N][N], T* buf, int nu, int tid) {
     constexpr int NPE = Pow<N, ND>::v;
     constexpr int BEFORE = Pow<N, P>::v, AFTER = Pow<N, ND - 1 - P>::v;
     constexpr int LINES = BEFORE * AFTER;
-    constexpr bool LAST = P == ND - 1;
-    const int src = (P & 1) ? NPE : 0, dst = (P & 1) ? 0 : NPE;
+    const T* src = buf + ((P & 1) ? UPB * NPE : 0);
+    T* dst = buf + ((P & 1) ? 0 : UPB * NPE);
     for (int k = tid; k < nu * LINES; k += kThreads) {
         const int le = k / LINES, l = k - le * LINES;
         const int ib = l % BEFORE, ia = l / BEFORE;
-        const int off = ib + BEFORE * N * ia;
-        const T* in = buf + le * 2 * NPE + src + off;
+        const int off = le * NPE + ib + BEFORE * N * ia;
         T v[N];
 #pragma unroll
-        for (int j = 0; j < N; ++j) v[j] = in[j * BEFORE];
+        for (int j = 0; j < N; ++j) v[j] = src[off + j * BEFORE];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             T acc = T(0);
 #pragma unroll
             for (int j = 0; j < N; ++j) acc += a[i][j] * v[j];
-            if (LAST)
-                qe[(int64_t)le * NPE + off + i * BEFORE] = acc;
-            else
-                buf[le * 2 * NPE + dst + off + i * BEFORE] = acc;
+            dst[off + i * BEFORE] = acc;
         }
+    }
+}
+
+// Copy of `count` values between HBM and LDS (either direction), in 16-byte vectors when
+// both ends are 16-byte aligned and count is a whole number of vectors, else value by value.
+template <typename T>
+__device__ __forceinline__ void block_copy(T* __restrict__ dst, const T* __restrict__ src, int count, int tid) {
+    constexpr int VEC = 16 / (int)sizeof(T);
+    if (((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15) == 0 && count % VEC == 0) {
+        const float4* s4 = reinterpret_cast<const float4*>(src);
+        float4* d4 = reinterpret_cast<float4*>(dst);
+        for (int k = tid; k < count / VEC; k += kThreads) d4[k] = s4[k];
+    } else {
+        for (int k = tid; k < count; k += kThreads) dst[k] = src[k];
     }
 }
 
@@ -124,7 +136,7 @@ template <typename T, int ND, int N>
 __global__ __launch_bounds__(kThreads) void k_equispaced_fixed(const T* __restrict__ Q, T* __restrict__ Qe,
                                                                const double* __restrict__ A, int64_t units) {
     constexpr int NPE = FixedShape<T, ND, N>::NPE, UPB = FixedShape<T, ND, N>::UPB;
-    __shared__ T buf[UPB * 2 * NPE];
+    __shared__ __align__(16) T buf[2 * UPB * NPE];
     const int tid = threadIdx.x;
     T a[N][N];
 #pragma unroll
@@ -133,22 +145,19 @@ __global__ __launch_bounds__(kThreads) void k_equispaced_fixed(const T* __restri
         for (int j = 0; j < N; ++j) a[i][j] = (T)A[i * N + j];
     const int64_t u0 = (int64_t)blockIdx.x * UPB;
     const int nu = (int)((units - u0) < (int64_t)UPB ? (units - u0) : (int64_t)UPB);
-    const T* q = Q + u0 * NPE;
-    for (int k = tid; k < nu * NPE; k += kThreads) {
-        const int le = k / NPE;
-        buf[le * 2 * NPE + (k - le * NPE)] = q[k];
-    }
-    T* qe = Qe + u0 * NPE;
+    block_copy(buf, Q + u0 * NPE, nu * NPE, tid);
     __syncthreads();
-    fixed_pass<T, ND, N, 0>(a, buf, qe, nu, tid);
+    fixed_pass<T, ND, N, 0, UPB>(a, buf, nu, tid);
     if constexpr (ND > 1) {
         __syncthreads();
-        fixed_pass<T, ND, N, 1>(a, buf, qe, nu, tid);
+        fixed_pass<T, ND, N, 1, UPB>(a, buf, nu, tid);
     }
     if constexpr (ND > 2) {
         __syncthreads();
-        fixed_pass<T, ND, N, 2>(a, buf, qe, nu, tid);
+        fixed_pass<T, ND, N, 2, UPB>(a, buf, nu, tid);
     }
+    __syncthreads();
+    block_copy(Qe + u0 * NPE, buf + (ND % 2 == 1 ? UPB * NPE : 0), nu * NPE, tid);
 }
 
 template <typename T, int ND, int N>
