@@ -67,7 +67,13 @@ struct FCfg {
     static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;           // threads per block
     static constexpr int NN = EPB * NPE;                              // nodes per block
     static constexpr int NPT = (NN + TPB - 1) / TPB;                  // nodes per thread
-    // LDS layout in floats, arrays indexed by block node.  Q: [NV][NN] node states.
+    // 2-D: slot arrays hold block node j at pnode(j) = j + j/N, one float of padding after
+    // every row of N nodes (plane stride NNP), so that the lines a wave reads together, N
+    // nodes apart, spread over the LDS banks (2-D N=8: -14 % per stage; neutral in 3-D,
+    // which keeps the unpadded layout).
+    static constexpr bool PAD = ND == 2;
+    static constexpr int NNP = PAD ? NN + NN / N : NN;
+    // LDS layout in floats, arrays indexed by padded block node.  Q: [NV][NNP] node states.
     // P: two-point flux inputs -- slot 0 density, slots 1 .. 2ND-1 the velocities in cyclic
     // order (u, v, w, u, v in 3-D; u, v, u in 2-D), so the ND line-frame velocity slots of a
     // line in any direction are consecutive, then three more scalars (PIN slots).  Once
@@ -77,8 +83,8 @@ struct FCfg {
     static constexpr int PIN = 2 * ND + 3;
     static constexpr int PSL = PIN > (ND - 1) * NV ? PIN : (ND - 1) * NV;
     static constexpr int OFF_Q = 0;
-    static constexpr int OFF_P = r4(NV * NN);
-    static constexpr int OFF_G = r4(OFF_P + PSL * NN);
+    static constexpr int OFF_P = r4(NV * NNP);
+    static constexpr int OFF_G = r4(OFF_P + PSL * NNP);
     static constexpr int OFF_F = OFF_G + 4 * EPB;
     static constexpr int LDS = OFF_F + 2 * NFC * EPB;
 };
@@ -226,14 +232,14 @@ __device__ __forceinline__ void line_pairs(const float (&P)[N][ND + 4], float (&
 // Line addressing.  Node p of a line is block node base + p*stride; the momentum along the
 // line and the others (line-frame slots 1 .. ND) are physical variables m[0 .. ND-1] --
 // per-lane offsets, so one instruction stream serves lines of every direction.  vel is
-// the per-lane offset (1 + direction)*NN of the line's first velocity slot in P.
+// the per-lane offset (1 + direction)*NNP of the line's first velocity slot in P.
 template <int ND>
 struct LineAddr {
     int base, stride, vel;
     int m[ND];
 };
 
-// flux inputs of the N nodes of a line, line-frame order (slot arrays of NN floats)
+// flux inputs of the N nodes of a line, line-frame order (slot arrays of plane stride NN)
 template <int ND, int N, int NN, bool H>
 __device__ __forceinline__ void load_line(const float* __restrict__ sP, const LineAddr<ND>& L,
                                           float (&P)[N][ND + 4]) {
@@ -269,6 +275,12 @@ __device__ __forceinline__ void store_line(float* __restrict__ R, const LineAddr
 #pragma unroll
         for (int c = 0; c < ND + 2; ++c) r[c * NN] = acc[p][c];
     }
+}
+
+// slot-array position of block node j (PAD: rows of N nodes, one float between rows)
+template <int N, bool PAD>
+__device__ __forceinline__ int pnode(int j) {
+    return PAD ? j + j / N : j;
 }
 
 // line-frame slot of physical variable v in a line of direction d (momenta cycle)
@@ -344,7 +356,7 @@ template <int ND, int N, int TP, int MODE, bool FIRST>
 __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     using C = FCfg<ND, N>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
-    constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
+    constexpr int TPB = C::TPB, NN = C::NN, NNP = C::NNP, NPT = C::NPT;
     constexpr int FG = 3 * ND + 1;  // face geometry entry: frame (n, t, b) and surface J
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
@@ -436,6 +448,7 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
         if (j >= NN) continue;
+        const int pj = pnode<N, C::PAD>(j);
         const float rho = q[r][0];
         const float ir = rcp(rho);
         float in[ND + 4], u[ND], hir = 0.0f;
@@ -450,7 +463,7 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
         const float p = gm1 * fmaf((TP == TP_CHANDRASEKHAR ? -2.0f : -0.5f) * rho, uu, q[r][NV - 1]);
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f))) err |= ERR_LOG_NEG;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) sQ[c * NN + j] = q[r][c];
+        for (int c = 0; c < NV; ++c) sQ[c * NNP + pj] = q[r][c];
         if constexpr (TP == TP_CHANDRASEKHAR) {
             in[0] = 0.5f * rho;
             in[ND + 1] = in[0] * rcp(p);
@@ -467,12 +480,12 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
             in[ND + 1] = p;
             in[ND + 2] = q[r][NV - 1];
         }
-        sP[j] = in[0];
+        sP[pj] = in[0];
 #pragma unroll
-        for (int c = 0; c < 2 * ND - 1; ++c) sP[(1 + c) * NN + j] = u[c % ND];
-        sP[(2 * ND) * NN + j] = in[ND + 1];
-        sP[(2 * ND + 1) * NN + j] = in[ND + 2];
-        if constexpr (TP == TP_CHANDRASEKHAR) sP[(2 * ND + 2) * NN + j] = in[ND + 3];
+        for (int c = 0; c < 2 * ND - 1; ++c) sP[(1 + c) * NNP + pj] = u[c % ND];
+        sP[(2 * ND) * NNP + pj] = in[ND + 1];
+        sP[(2 * ND + 1) * NNP + pj] = in[ND + 2];
+        if constexpr (TP == TP_CHANDRASEKHAR) sP[(2 * ND + 2) * NNP + pj] = in[ND + 3];
     }
     if (face_lane) {
         const float lift = (fk & 1) ? a.lift_p : a.lift_m;
@@ -506,16 +519,16 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     for (int sd = 0; sd < 2; ++sd) {
         if (inblk[sd]) {
             const int kp = finfo_nbpos(xinf[sd]);
-            const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+            const int node = pnode<N, C::PAD>((xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0));
 #pragma unroll
-            for (int v = 0; v < NV; ++v) xr[sd][v] = sQ[v * NN + node];
+            for (int v = 0; v < NV; ++v) xr[sd][v] = sQ[v * NNP + node];
         }
     }
     // ---- line tasks ------------------------------------------------------------------
     LineAddr<ND> L;
-    L.base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
-    L.stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
-    L.vel = (1 + ldir) * NN;
+    L.base = pnode<N, C::PAD>(lle * NPE + line_node<ND, N>(ldir, lf, 0));
+    L.stride = ldir == 0 ? 1 : (ldir == 1 ? (C::PAD ? N + 1 : N) : (C::PAD ? N * (N + 1) : NPF));
+    L.vel = (1 + ldir) * NNP;
 #pragma unroll
     for (int k = 0; k < ND; ++k) L.m[k] = 1 + (ldir + k) % ND;
     // exterior traces in the line frame; a slip wall reflects the normal momentum
@@ -539,13 +552,13 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     {
         // face fluxes at the two line ends start rows 0 and N-1
         float o0[NV], o1[NV];
-        load_vars<ND, NN>(sQ + L.base, L, o0);
-        load_vars<ND, NN>(sQ + L.base + (N - 1) * L.stride, L, o1);
+        load_vars<ND, NNP>(sQ + L.base, L, o0);
+        load_vars<ND, NNP>(sQ + L.base + (N - 1) * L.stride, L, o1);
         const float2 g0 = sF[lle * NFC + 2 * ldir], g1 = sF[lle * NFC + 2 * ldir + 1];
         const float ehalf = TP == TP_CHANDRASEKHAR ? 0.5f : 1.0f;  // energy row accumulated halved
         constexpr bool PO = TP == TP_CHANDRASEKHAR;
         float P[N][ND + 4];
-        load_line<ND, N, NN, PO>(sP, L, P);
+        load_line<ND, N, NNP, PO>(sP, L, P);
         bool bad = false;
         face_init<ND, N, 0, PO>(o0, xl[0], P[0], g0.x, g0.y, gm1, gamma, a.diss, ehalf, acc, bad);
         face_init<ND, N, N - 1, PO>(o1, xl[1], P[N - 1], g1.x, g1.y, gm1, gamma, a.diss, ehalf, acc, bad);
@@ -596,10 +609,10 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) uq[r][v] = sQ[v * NN + (j < NN ? j : 0)];
+        for (int v = 0; v < NV; ++v) uq[r][v] = sQ[v * NNP + pnode<N, C::PAD>(j < NN ? j : 0)];
     }
     __syncthreads();  // flux inputs and node states are dead from here
-    if (line_ok) store_line<ND, N, NN>(ldir == ND - 1 ? sQ : sP + ldir * NV * NN, L, acc);
+    if (line_ok) store_line<ND, N, NNP>(ldir == ND - 1 ? sQ : sP + ldir * NV * NNP, L, acc);
     if constexpr (!EARLY_AUX) load_aux();
     __syncthreads();
 
@@ -610,15 +623,15 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
         const int jj = j < NN ? j : 0;
-        const int le = jj / NPE;
+        const int le = jj / NPE, pj = pnode<N, C::PAD>(jj);
         float s[ND];
 #pragma unroll
         for (int d = 0; d < ND; ++d) s[d] = sG[le * 4 + d];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            float t = s[ND - 1] * sQ[lslot<ND>(ND - 1, v) * NN + jj];
+            float t = s[ND - 1] * sQ[lslot<ND>(ND - 1, v) * NNP + pj];
 #pragma unroll
-            for (int d = ND - 2; d >= 0; --d) t = fmaf(s[d], sP[(d * NV + lslot<ND>(d, v)) * NN + jj], t);
+            for (int d = ND - 2; d >= 0; --d) t = fmaf(s[d], sP[(d * NV + lslot<ND>(d, v)) * NNP + pj], t);
             kv[r][v] = t;
         }
     }
