@@ -86,13 +86,24 @@ def rk_step_vjp_noalloc(disc, solver, dt, stages, ubar, t=0.0):
 
 class Checkpoints(list):
     """States at the segment starts (segment i starts at step i*every), plus in ``stages``
-    the per-step stage inputs of the trailing segments the forward run kept resident:
-    stages[i][k] = [u_n, stage inputs...] of step n = i*every + k."""
+    the stage inputs of the trailing segments the forward run kept resident: stages[i] is
+    one (steps, stages, nvars, ndofs) tensor, [k, s] = stage input s of step i*every + k
+    (s = 0: the state u_n)."""
 
     def __init__(self, every):
         super().__init__()
         self.every = every
         self.stages = {}
+
+
+def _step_ptrs(buf, k):
+    """ctypes array of the stage buffers of step k of a kept segment (no tensor views: the
+    loops below launch one step per call, and creating hundreds of view objects per
+    sweep stalls the host long enough to starve the GPU)."""
+    S = buf.shape[1]
+    sb = buf.shape[2] * buf.shape[3] * buf.element_size()
+    base = buf.data_ptr() + k * S * sb
+    return (ctypes.c_void_p * S)(*[base + s * sb for s in range(S)])
 
 
 def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtype=None, keep_segments=0):
@@ -120,12 +131,14 @@ def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtyp
                 cur[0], nxt = nxt, cur[0]
             continue
         buf = torch.empty((nloc, S, disc.nvars, disc.ndofs), dtype=disc.torch_dtype(), device=Q0.device)
-        steps = [[buf[i, s] for s in range(S)] for i in range(nloc)]
-        steps[0][0].copy_(cur[0])
+        buf[0, 0].copy_(cur[0])
+        disc._sync_stream()
         for i in range(nloc):
-            rk_step_record(disc, solver, dt, steps[i], steps[i + 1][0] if i + 1 < nloc else nxt)
+            dst = _step_ptrs(buf, i + 1)[0] if i + 1 < nloc else disc._dptr(nxt).value
+            C.check(C.lib.flou_hip_rk_step_record(disc.h, code, float(dt), 0.0, _step_ptrs(buf, i),
+                                                  ctypes.c_void_p(dst)), disc.h, "rk_step_record")
         cur[0], nxt = nxt, cur[0]
-        cps.stages[sg] = steps
+        cps.stages[sg] = buf
     return cur[0], cps
 
 
@@ -139,10 +152,16 @@ def reverse(disc, solver, checkpoints, dt, nsteps, ubar, checkpoint_every):
     K = max(1, int(checkpoint_every))
     kept = getattr(checkpoints, "stages", {})
     seg_bufs = nxt = None
+    ub = disc._dptr(ubar)
     for sgi in reversed(range(len(checkpoints))):
         if sgi in kept:
-            for st in reversed(kept[sgi]):
-                rk_step_vjp(disc, solver, dt, st, ubar)
+            buf = kept[sgi]
+            if buf.shape[1] != S or buf.shape[2:] != (disc.nvars, disc.ndofs) or buf.dtype != disc.torch_dtype():
+                raise ValueError("kept stage inputs do not match this scheme and discretisation")
+            disc._sync_stream()
+            for i in reversed(range(buf.shape[0])):
+                C.check(C.lib.flou_hip_rk_step_vjp(disc.h, code, float(dt), 0.0, _step_ptrs(buf, i), ub),
+                        disc.h, "rk_step_vjp")
             continue
         if seg_bufs is None:
             seg_bufs = [[disc.new_state() for _ in range(S)] for _ in range(min(K, nsteps))]
