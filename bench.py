@@ -112,7 +112,7 @@ def time_form(precision, Q64, dt, steps, warmup):
     return dict(wall_s=wall, gpu_ms=gpu_ms, ndofs=ndofs, rho=rho, fast=fast)
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fast3d_6_chandrasekhar_64cube.md")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_fast_3_6_chandrasekhar_64cube.md")
 
 
 def measured_traffic():
@@ -451,7 +451,7 @@ def main():
     achieved = bytes_per_launch / (launch_ms / 1e3) / 1e9
     traffic_src = None
     if main_form == "fp32" and r["fast"]:
-        kname = "k_fast3d<6, CHANDRASEKHAR, MODE_2N> (FP32 fused rhs + CK2N54 stage)"
+        kname = "k_fast<3, 6, CHANDRASEKHAR, MODE_2N> (FP32 fused rhs + CK2N54 stage)"
         traffic = measured_traffic()
         if traffic is not None:
             traffic_src = ("rocprofv3 FETCH_SIZE + WRITE_SIZE of this kernel on this workload, "
