@@ -383,10 +383,18 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
 
     // connectivity checks and per element-face tables
     const int64_t ne = h->ne, nf = h->nf;
+    // face orientations (master2slave, StdSegment.jl:161-167, StdQuad.jl:161-181): 0 or 1 on
+    // segment faces, 0..7 on quad faces, 0 on point faces and on boundary faces
+    bool oriented = false;
     if (d->face_orientation)
-        for (int64_t f = 0; f < nf; ++f)
-            if (d->face_orientation[f] != 0)
-                return bad(FLOU_HIP_EUNSUPPORTED, "face orientations other than 0 are not implemented");
+        for (int64_t f = 0; f < nf; ++f) {
+            const int o = d->face_orientation[f];
+            if (o >= (nd == 3 ? 8 : (nd == 2 ? 2 : 1)))
+                return bad(FLOU_HIP_EINVAL, "face orientation out of range for the face's shape");
+            if (o != 0 && d->face_eleminds[f * 2 + 1] < 0)
+                return bad(FLOU_HIP_EINVAL, "boundary faces must have orientation 0");
+            oriented |= o != 0;
+        }
     std::vector<int> nb(ne * NF), finfo(ne * NF), fgi(ne * NF), fid(ne * NF);
     for (int64_t e = 0; e < ne; ++e)
         for (int k = 0; k < NF; ++k) {
@@ -402,10 +410,12 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                 const int kp = d->face_elempos[f * 2 + 1 - s];
                 if (kp < 0 || kp >= NF) return bad(FLOU_HIP_EINVAL, "bad face_elempos");
                 info |= kp << 1;
+                if (d->face_orientation) info |= (int)d->face_orientation[f] << 28;
             } else {
                 if (s != 0) return bad(FLOU_HIP_EINVAL, "boundary faces must have the element on side 1");
                 const int bd = d->face_boundary ? d->face_boundary[f] : -1;
                 if (bd < 0 || bd >= h->nbd) return bad(FLOU_HIP_EINVAL, "boundary face without a valid boundary index");
+                if (bd >= (1 << 24) - 1) return bad(FLOU_HIP_EINVAL, "too many boundaries");
                 info |= (bd + 1) << 4;
             }
             nb[e * NF + k] = (int)other;
@@ -583,13 +593,14 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // axis-aligned affine elements (diagonal metric, frames n = ±e_d) or on curved elements
     // (per-node metric and J, per-face-node frames), GLL traces
     // (l = e_1/e_n and ∂g supported on the end nodes up to 1e-12) and a vanishing D♯
-    // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable).  Anything
-    // else runs the plain kernel in FP32.
+    // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable) and every face
+    // orientation 0 (the fast kernels pair face nodes by index).  Anything else runs the
+    // plain kernel in FP32.
     std::vector<float> fe, fds;
     if (h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
         h->vol == FLOU_HIP_SPLIT && h->savg == FLOU_HIP_AVG_STD &&
         (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
-        h->ndofs < (int64_t(1) << 30)) {
+        h->ndofs < (int64_t(1) << 30) && !oriented) {
         bool ok = true;
         double dmax = 0.0;
         for (int i = 0; i < n * n; ++i) dmax = std::max(dmax, std::fabs(d->Dsharp[i]));

@@ -42,12 +42,14 @@ __host__ __device__ constexpr int ops_w(int n) { return 3 * n * n + 4 * n; }
 __host__ __device__ constexpr int ops_size(int n) { return 3 * n * n + 5 * n; }
 
 // Per element-face table entry "finfo":
-//   bit 0     : side of this element on the face (0 = master / side 1 in Flou, 1 = slave)
-//   bits 1..3 : face position of the neighbour (0-based) when the face is interior
-//   bits 4..  : boundary index + 1 for boundary faces, 0 for interior faces
+//   bit 0       : side of this element on the face (0 = master / side 1 in Flou, 1 = slave)
+//   bits 1..3   : face position of the neighbour (0-based) when the face is interior
+//   bits 4..27  : boundary index + 1 for boundary faces, 0 for interior faces
+//   bits 28..30 : orientation of an interior face (mesh.faces[i].orientation)
 __host__ __device__ inline int finfo_side(int x) { return x & 1; }
 __host__ __device__ inline int finfo_nbpos(int x) { return (x >> 1) & 7; }
-__host__ __device__ inline int finfo_bd(int x) { return (x >> 4) - 1; }
+__host__ __device__ inline int finfo_bd(int x) { return ((x >> 4) & 0xffffff) - 1; }
+__host__ __device__ inline int finfo_orient(int x) { return (x >> 28) & 7; }
 
 template <typename T>
 struct RhsArgs {
@@ -109,6 +111,54 @@ __host__ __device__ inline int line_node(int dir, int f, int p) {
     if (dir == 0) return p + N * (fi + N * fj);
     if (dir == 1) return fi + N * (p + N * fj);
     return fi + N * (fj + N * p);
+}
+
+// Face-node maps between the two sides of a face with orientation o, for face nodes
+// numbered as line_node's lines (f = a + N*b on a quad face, a fastest): master2slave and
+// slave2master of StdSegment.jl:153-167 (o = 1 reverses the segment) and StdQuad.jl:
+// 138-181 (o = 1..3 rotate, 4..7 reflect the square).
+template <int ND, int N>
+__host__ __device__ inline int face_m2s(int f, int o) {
+    if (ND < 2 || o == 0) return f;
+    if (ND == 2) return N - 1 - f;
+    const int a = f % N, b = f / N, r = N - 1;
+    switch (o) {
+        case 1: return b + N * (r - a);
+        case 2: return (r - a) + N * (r - b);
+        case 3: return (r - b) + N * a;
+        case 4: return b + N * a;
+        case 5: return (r - a) + N * b;
+        case 6: return (r - b) + N * (r - a);
+        default: return a + N * (r - b);
+    }
+}
+template <int ND, int N>
+__host__ __device__ inline int face_s2m(int f, int o) {
+    if (ND < 2 || o == 0) return f;
+    if (ND == 2) return N - 1 - f;
+    const int a = f % N, b = f / N, r = N - 1;
+    switch (o) {
+        case 1: return (r - b) + N * a;
+        case 2: return (r - a) + N * (r - b);
+        case 3: return b + N * (r - a);
+        case 4: return b + N * a;
+        case 5: return (r - a) + N * b;
+        case 6: return (r - b) + N * (r - a);
+        default: return a + N * (r - b);
+    }
+}
+// For own face node f of an interior face (finfo `info`): the neighbour's face node that
+// meets it, and the master's index of the pair, which indexes the per-face-node geometry.
+template <int ND, int N>
+__host__ __device__ inline int face_nb_node(int f, int info) {
+    const int o = finfo_orient(info);
+    if (o == 0) return f;
+    return finfo_side(info) ? face_s2m<ND, N>(f, o) : face_m2s<ND, N>(f, o);
+}
+template <int ND, int N>
+__host__ __device__ inline int face_master_node(int f, int info) {
+    const int o = finfo_orient(info);
+    return (o == 0 || !finfo_side(info)) ? f : face_s2m<ND, N>(f, o);
 }
 
 // (line index, position) of `node` in direction `dir`.

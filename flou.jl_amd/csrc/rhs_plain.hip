@@ -192,12 +192,14 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
     if (pf && a.gather && pf_nbe >= 0) {
         const int kp = finfo_nbpos(pf_info);
         const T* Qn = a.Q + (int64_t)pf_nbe * NPE;
-        const int nn = line_node<ND, N>(kp >> 1, pff, (kp & 1) ? N - 1 : 0);
+        const int nn = line_node<ND, N>(kp >> 1, face_nb_node<ND, N>(pff, pf_info), (kp & 1) ? N - 1 : 0);
 #pragma unroll
         for (int v = 0; v < NV; ++v) pf_ext[v] = Qn[v * Nd + nn];
     }
     if (pf) {
-        const T* g = a.fgeom + ((int64_t)pf_fgi * (CURV ? NPF : 1) + (CURV ? pff : 0)) * (3 * ND + 1);
+        const T* g = a.fgeom +
+                     ((int64_t)pf_fgi * (CURV ? NPF : 1) + (CURV ? face_master_node<ND, N>(pff, pf_info) : 0)) *
+                         (3 * ND + 1);
 #pragma unroll
         for (int c = 0; c < 3 * ND + 1; ++c) pf_g[c] = g[c];
     }
@@ -283,7 +285,9 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         const bool first = PF && it == tid;  // prefetched before the staging barrier
         const int info = first ? pf_info : a.finfo[slot];
         const int nbe = first ? pf_nbe : a.nb[slot];
-        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) + (CURV ? f : 0)) *
+        // face geometry per face node (curved) is stored by the master's face-node index
+        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) +
+                                (CURV ? face_master_node<ND, N>(f, info) : 0)) *
                                    (3 * ND + 1);
         T gbuf[PF ? 3 * ND + 1 : 1];
         if constexpr (PF) {
@@ -293,12 +297,13 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
         }
         if (nbe >= 0) {
             const int kp = finfo_nbpos(info), ndir = kp >> 1, nlr = kp & 1;
+            const int fn = face_nb_node<ND, N>(f, info);  // the neighbour's face node meeting f
             const T* Qn = a.Q + (int64_t)nbe * NPE;
             if (a.gather && first) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ext[v] = pf_ext[v];
             } else if (a.gather) {
-                const int nn = line_node<ND, N>(ndir, f, nlr ? N - 1 : 0);
+                const int nn = line_node<ND, N>(ndir, fn, nlr ? N - 1 : 0);
 #pragma unroll
                 for (int v = 0; v < NV; ++v) ext[v] = Qn[v * Nd + nn];
             } else {
@@ -306,7 +311,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_plain(const RhsArgs<T> a)
 #pragma unroll
                 for (int v = 0; v < NV; ++v) {
                     T s = T(0);
-                    for (int p = 0; p < N; ++p) s = s + l[p] * Qn[v * Nd + line_node<ND, N>(ndir, f, p)];
+                    for (int p = 0; p < N; ++p) s = s + l[p] * Qn[v * Nd + line_node<ND, N>(ndir, fn, p)];
                     ext[v] = s;
                 }
             }

@@ -106,13 +106,14 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     T pf_ext[NV], pf_yl[N][NV], pf_g[3 * ND + 1], pf_jn = T(1);
     if (pf && pf_nbe >= 0) {
         const int kp = finfo_nbpos(pf_info), ndir = kp >> 1;
+        const int fn = face_nb_node<ND, N>(pff, pf_info);
         const int64_t base = (int64_t)pf_nbe * NPE;
         pf_jn = a.egeom[(int64_t)pf_nbe * a.egeom_stride + ND * ND];
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
-            pf_ext[v] = a.Q[v * Nd + base + line_node<ND, N>(ndir, pff, (kp & 1) ? N - 1 : 0)];
+            pf_ext[v] = a.Q[v * Nd + base + line_node<ND, N>(ndir, fn, (kp & 1) ? N - 1 : 0)];
 #pragma unroll
-            for (int p = 0; p < N; ++p) pf_yl[p][v] = A.Y[v * Nd + base + line_node<ND, N>(ndir, pff, p)];
+            for (int p = 0; p < N; ++p) pf_yl[p][v] = A.Y[v * Nd + base + line_node<ND, N>(ndir, fn, p)];
         }
     }
     if (pf) {
@@ -198,7 +199,9 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         const int info = first ? pf_info : a.finfo[slot];
         const int nbe = first ? pf_nbe : a.nb[slot];
         const bool slave = finfo_side(info);
-        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) + (CURV ? f : 0)) *
+        // face geometry per face node (curved) is stored by the master's face-node index
+        const T* g = a.fgeom + ((int64_t)(first ? pf_fgi : a.fgi[slot]) * (CURV ? NPF : 1) +
+                                (CURV ? face_master_node<ND, N>(f, info) : 0)) *
                                    (3 * ND + 1);
         T gbuf[PF ? 3 * ND + 1 : 1];
         if constexpr (PF) {
@@ -217,6 +220,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             const T* gq = &sOps[nlr ? ops_gp(N) : ops_gm(N)];
             const T Jn = first ? pf_jn : a.egeom[(int64_t)nbe * a.egeom_stride + ND * ND];
             const int64_t base = (int64_t)nbe * NPE;
+            const int fn = face_nb_node<ND, N>(f, info);  // the neighbour's face node meeting f
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
                 T s = T(0), t = T(0);
@@ -226,12 +230,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
                     ext[v] = pf_ext[v];
                 } else {
                     for (int p = 0; p < N; ++p) {
-                        const int64_t nn = base + line_node<ND, N>(ndir, f, p);
+                        const int64_t nn = base + line_node<ND, N>(ndir, fn, p);
                         const int64_t gi = v * Nd + nn;
                         s = s + lq[p] * a.Q[gi];
                         t = t + gq[p] * (CURV ? A.Y[gi] / a.ngeom[ND * ND * Nd + nn] : A.Y[gi]);
                     }
-                    ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, f, nlr ? N - 1 : 0)] : s;
+                    ext[v] = a.gather ? a.Q[v * Nd + base + line_node<ND, N>(ndir, fn, nlr ? N - 1 : 0)] : s;
                 }
                 const T Ln = CURV ? A.c * t : A.c * t / Jn;
                 Fb[v] = Fb[v] + (slave ? -Ln : Ln);  // -σ_nb L_nb, σ_nb = -σ_own

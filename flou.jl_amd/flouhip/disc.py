@@ -101,7 +101,8 @@ class MultielementDisc:
         d.elem_facepos = _ptr(arr(mesh.elem_side, np.int8), ctypes.c_int8)
         d.face_eleminds = _ptr(arr(mesh.face_elems, np.int64), ctypes.c_int64)
         d.face_elempos = _ptr(arr(mesh.face_pos, np.int8), ctypes.c_int8)
-        d.face_orientation = None
+        forient = getattr(mesh, "face_orientation", None)  # mesh.faces[i].orientation; None: all 0
+        d.face_orientation = None if forient is None else _ptr(arr(forient, np.uint8), ctypes.c_uint8)
         d.face_boundary = _ptr(arr(mesh.face_boundary, np.int32), ctypes.c_int32)
         if getattr(mesh, "mapping", None) is None:
             ejac, emet, evol = mesh.element_geometry(std)

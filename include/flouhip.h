@@ -122,7 +122,10 @@ typedef struct flou_hip_desc {
     const int8_t* elem_facepos;    /* nelements * 2nd: side of the element on that face (0 = master) */
     const int64_t* face_eleminds;  /* nfaces * 2: eleminds - 1 (-1: boundary exterior) */
     const int8_t* face_elempos;    /* nfaces * 2: elempos - 1 (-1: boundary exterior) */
-    const uint8_t* face_orientation; /* nfaces, NULL = all 0 */
+    const uint8_t* face_orientation; /* nfaces, NULL = all 0: mesh.faces[i].orientation, the
+                                        master2slave map of the face's nodes (StdSegment.jl:
+                                        161-167: 0..1; StdQuad.jl:161-181: 0..7); 0 on boundary
+                                        faces.  Non-zero orientations run the plain kernels. */
     const int32_t* face_boundary;  /* nfaces: boundary index of boundary faces, else -1 */
 
     /* Geometry (src/FlouSpatial/PhysicalRegions.jl:306-364, 485-535) */

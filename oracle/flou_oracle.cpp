@@ -742,6 +742,21 @@ void oracle_geometry(oracle_handle* h, double* e_jac, double* e_metric, double* 
     std::memcpy(f_jac, d.f_jac.data(), d.f_jac.size() * sizeof(double));
 }
 
+// Replace the element/face connectivity of the same numbers of elements and faces (a mesh
+// whose elements number their nodes differently: the rhs then reads face orientations).
+void oracle_set_connectivity(oracle_handle* h, const long* e_face, const int* e_side, const long* f_elem,
+                             const int* f_pos, const int* f_orient) {
+    Disc& d = h->d;
+    std::memcpy(d.e_face.data(), e_face, d.e_face.size() * sizeof(long));
+    std::memcpy(d.e_side.data(), e_side, d.e_side.size() * sizeof(int));
+    std::memcpy(d.f_elem.data(), f_elem, d.f_elem.size() * sizeof(long));
+    std::memcpy(d.f_pos.data(), f_pos, d.f_pos.size() * sizeof(int));
+    std::memcpy(d.f_orient.data(), f_orient, d.f_orient.size() * sizeof(int));
+}
+
+int oracle_master2slave(int nd, int n, int i, int o) { return master2slave(nd, n, i, o); }
+int oracle_slave2master(int nd, int n, int i, int o) { return slave2master(nd, n, i, o); }
+
 void oracle_coords(oracle_handle* h, double* coords, double* f_coords) {
     const Disc& d = h->d;
     if (coords) std::memcpy(coords, d.coords.data(), d.coords.size() * sizeof(double));

@@ -214,6 +214,39 @@ inline int line_node(int nd, int n, int dir, int f, int a) {
     return fi + n * (fj + n * a);
 }
 
+// Face-node maps between the sides of a face with orientation `o` (0-based indices; quad
+// faces numbered with the first index fastest, as lineardofs/cartesiandofs do):
+// master2slave and slave2master of StdSegment.jl:153-167 and StdQuad.jl:138-181.
+inline int quad_li(int n, int x, int y) { return (x - 1) + n * (y - 1); }  // 1-based (x, y)
+inline int master2slave(int nd, int n, int i, int o) {
+    if (o == 0 || nd == 1) return i;
+    if (nd == 2) return n - 1 - i;  // StdSegment: ndofs - (i - 1), 1-based
+    const int m1 = i % n + 1, m2 = i / n + 1;
+    switch (o) {
+        case 1: return quad_li(n, m2, n - m1 + 1);
+        case 2: return quad_li(n, n - m1 + 1, n - m2 + 1);
+        case 3: return quad_li(n, n - m2 + 1, m1);
+        case 4: return quad_li(n, m2, m1);
+        case 5: return quad_li(n, n - m1 + 1, m2);
+        case 6: return quad_li(n, n - m2 + 1, n - m1 + 1);
+        default: return quad_li(n, m1, n - m2 + 1);
+    }
+}
+inline int slave2master(int nd, int n, int i, int o) {
+    if (o == 0 || nd == 1) return i;
+    if (nd == 2) return n - 1 - i;
+    const int s1 = i % n + 1, s2 = i / n + 1;
+    switch (o) {
+        case 1: return quad_li(n, n - s2 + 1, s1);
+        case 2: return quad_li(n, n - s1 + 1, n - s2 + 1);
+        case 3: return quad_li(n, s2, n - s1 + 1);
+        case 4: return quad_li(n, s2, s1);
+        case 5: return quad_li(n, n - s1 + 1, s2);
+        case 6: return quad_li(n, n - s2 + 1, n - s1 + 1);
+        default: return quad_li(n, s1, n - s2 + 1);
+    }
+}
+
 // ------------------------------------------------------------------------------------
 // Point-wise physics (FlouCommon/Euler.jl, FlouCommon/LinearAdvection.jl)
 template <class T>
@@ -934,14 +967,17 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
         for (int f = 0; f < npf; ++f) apply_bc(d, QF(face, 0, f), face, f, QF(face, 1, f));
     }
 
-    // interface_fluxes! (Interfaces.jl:111-136); Cartesian orientation is 0 (j == i)
+    // interface_fluxes! (Interfaces.jl:111-136): master node f meets slave node
+    // j = master2slave(f, orientation); the frame and jac are the master's
 #pragma omp parallel for schedule(static)
     for (long face = 0; face < nf; ++face) try {
+        const int orient = d.f_orient[face];
         for (int f = 0; f < npf; ++f) {
+            const int j = master2slave(nd, n, f, orient);
             const double* fr = d.frame(face, f);
             const double jac = d.facejac(face, f);
             const T* Ql = QF(face, 0, f);
-            const T* Qr = QF(face, 1, f);
+            const T* Qr = QF(face, 1, j);
             T F[5];
             if (d.eq == EQ_ADVECTION) {
                 // LinearAdvection.jl:27-41 (rotations are the identity)
@@ -958,7 +994,7 @@ void rhs(const Disc& d, const T* Q, T* dQ, Work<T>& wk) {
                 rotate2phys(nd, Fr, fr, F);
             }
             T* F1 = FN(face, 0, f);
-            T* F2 = FN(face, 1, f);
+            T* F2 = FN(face, 1, j);
             for (int v = 0; v < nv; ++v) {
                 F1[v] = F[v] * jac;
                 F2[v] = -F1[v];

@@ -49,6 +49,9 @@ def lib():
         L.oracle_connectivity.argtypes = [ctypes.c_void_p, _lp, _ip, _lp, _ip, _ip]
         L.oracle_geometry.argtypes = [ctypes.c_void_p, _dp, _dp, _dp, _dp, _dp]
         L.oracle_coords.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_set_connectivity.argtypes = [ctypes.c_void_p, _lp, _ip, _lp, _ip, _ip]
+        L.oracle_master2slave.argtypes = [ctypes.c_int] * 4
+        L.oracle_slave2master.argtypes = [ctypes.c_int] * 4
         L.oracle_set_bc.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.oracle_max_dt.restype = ctypes.c_double
@@ -169,6 +172,18 @@ class OracleDisc:
         return dict(e_face=e_face.reshape(self.ne, 2 * nd), e_side=e_side.reshape(self.ne, 2 * nd),
                     f_elem=f_elem.reshape(self.nf, 2), f_pos=f_pos.reshape(self.nf, 2), f_bd=f_bd)
 
+    def set_connectivity(self, e_face, e_side, f_elem, f_pos, f_orient):
+        """Replace the connectivity (same element and face counts), with face orientations."""
+        self._conn = [np.ascontiguousarray(e_face, dtype=np.int64).reshape(-1),
+                      np.ascontiguousarray(e_side, dtype=np.int32).reshape(-1),
+                      np.ascontiguousarray(f_elem, dtype=np.int64).reshape(-1),
+                      np.ascontiguousarray(f_pos, dtype=np.int32).reshape(-1),
+                      np.ascontiguousarray(f_orient, dtype=np.int32).reshape(-1)]
+        c = self._conn
+        assert c[0].size == self.ne * 2 * self.nd and c[2].size == 2 * self.nf and c[4].size == self.nf
+        lib().oracle_set_connectivity(self.h, _p(c[0], _lp), _p(c[1], _ip), _p(c[2], _lp), _p(c[3], _ip),
+                                      _p(c[4], _ip))
+
     def geometry(self):
         nd = self.nd
         e_jac = np.zeros(self.ne)
@@ -251,6 +266,14 @@ class OracleDisc:
         du = np.array(dQ, dtype=np.float64, order="F", copy=True)
         self._check(lib().oracle_rk_run_jvp(self.h, scheme, _p(u), _p(du), float(dt), int(nsteps)))
         return u, du
+
+
+def master2slave(nd, n, i, o):
+    return lib().oracle_master2slave(nd, n, i, o)
+
+
+def slave2master(nd, n, i, o):
+    return lib().oracle_slave2master(nd, n, i, o)
 
 
 def set_threads(n):

@@ -146,6 +146,20 @@ def test_create_rejects_inconsistent_connectivity():
     assert b"disagree" in C.lib.flou_hip_create_error()
 
 
+@pytest.mark.parametrize("value,text", [(2, b"out of range"), (7, b"out of range")])
+def test_create_rejects_bad_face_orientations(value, text):
+    """Segment faces (2-D) take orientations 0 and 1 (StdSegment.jl:161-167)."""
+    keep = []
+    d = _valid_desc(keep)
+    fo = np.zeros(d.nfaces, dtype=np.uint8)
+    fo[0] = value
+    keep.append(fo)
+    d.face_orientation = fo.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    h = ctypes.c_void_p()
+    assert C.lib.flou_hip_create(ctypes.byref(d), ctypes.byref(h)) == C.EINVAL
+    assert text in C.lib.flou_hip_create_error()
+
+
 def test_null_arguments():
     assert C.lib.flou_hip_rhs(None, None, None, 0.0) == C.EINVAL
     assert C.lib.flou_hip_destroy(None) == C.EINVAL
