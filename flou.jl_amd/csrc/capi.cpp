@@ -569,23 +569,24 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                                     : plain_f32_3d(n, h->eq, kvol, h->tp, h->curved));
         if (!h->l32) return bad(FLOU_HIP_EUNSUPPORTED, "no kernel for this combination");
     }
-    // adjoint kernels: StdAverage surface average with or without LxF dissipation
+    // adjoint kernels: the StdAverage surface average with LxF or no dissipation has a
+    // hand-written face-flux VJP; every other Euler surface flux (Chandrasekhar average,
+    // Scalar and Matrix dissipation) differentiates the plain flux in dual numbers (gen)
+    const int gen = h->eq == FLOU_HIP_EULER &&
+                    (h->savg != FLOU_HIP_AVG_STD || h->sdiss == FLOU_HIP_DISS_MATRIX ||
+                     h->sdiss == FLOU_HIP_DISS_SCALAR);
     if (h->vol == FLOU_HIP_HYBRID) {
         h->vjp_why = "adjoint: the HybridDivOperator is not differentiated";
     } else if (kvol == VOL_SPLIT_NB) {
         h->vjp_why = "adjoint: the split form on Gauss nodes is not differentiated";
-    } else if (h->eq == FLOU_HIP_EULER && h->savg != FLOU_HIP_AVG_STD) {
-        h->vjp_why = "adjoint: only the StdAverage surface average is differentiated";
-    } else if (h->sdiss == FLOU_HIP_DISS_MATRIX || h->sdiss == FLOU_HIP_DISS_SCALAR) {
-        h->vjp_why = "adjoint: MatrixDissipation and ScalarDissipation are not differentiated";
     } else if (h->prec == FLOU_HIP_FP64) {
-        h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp, h->curved)
-                         : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp, h->curved)
-                                    : vjp_f64_3d(n, h->eq, h->vol, h->tp, h->curved));
+        h->v64 = nd == 1 ? vjp_f64_1d(n, h->eq, h->vol, h->tp, h->curved, gen)
+                         : (nd == 2 ? vjp_f64_2d(n, h->eq, h->vol, h->tp, h->curved, gen)
+                                    : vjp_f64_3d(n, h->eq, h->vol, h->tp, h->curved, gen));
     } else {
-        h->v32 = nd == 1 ? vjp_f32_1d(n, h->eq, h->vol, h->tp, h->curved)
-                         : (nd == 2 ? vjp_f32_2d(n, h->eq, h->vol, h->tp, h->curved)
-                                    : vjp_f32_3d(n, h->eq, h->vol, h->tp, h->curved));
+        h->v32 = nd == 1 ? vjp_f32_1d(n, h->eq, h->vol, h->tp, h->curved, gen)
+                         : (nd == 2 ? vjp_f32_2d(n, h->eq, h->vol, h->tp, h->curved, gen)
+                                    : vjp_f32_3d(n, h->eq, h->vol, h->tp, h->curved, gen));
     }
     if (!h->v64 && !h->v32 && h->vjp_why.empty()) h->vjp_why = "adjoint: no kernel for this combination";
 

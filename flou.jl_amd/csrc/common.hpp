@@ -205,11 +205,11 @@ RhsLauncherF32 plain_f32_3d(int n, int eq, int vol, int tp, int curved);
 
 typedef hipError_t (*VjpLauncherF64)(const VjpArgs<double>&, hipStream_t);
 typedef hipError_t (*VjpLauncherF32)(const VjpArgs<float>&, hipStream_t);
-VjpLauncherF64 vjp_f64_1d(int n, int eq, int vol, int tp, int curved);
-VjpLauncherF64 vjp_f64_2d(int n, int eq, int vol, int tp, int curved);
-VjpLauncherF64 vjp_f64_3d(int n, int eq, int vol, int tp, int curved);
-VjpLauncherF32 vjp_f32_1d(int n, int eq, int vol, int tp, int curved);
-VjpLauncherF32 vjp_f32_2d(int n, int eq, int vol, int tp, int curved);
-VjpLauncherF32 vjp_f32_3d(int n, int eq, int vol, int tp, int curved);
+VjpLauncherF64 vjp_f64_1d(int n, int eq, int vol, int tp, int curved, int gen);
+VjpLauncherF64 vjp_f64_2d(int n, int eq, int vol, int tp, int curved, int gen);
+VjpLauncherF64 vjp_f64_3d(int n, int eq, int vol, int tp, int curved, int gen);
+VjpLauncherF32 vjp_f32_1d(int n, int eq, int vol, int tp, int curved, int gen);
+VjpLauncherF32 vjp_f32_2d(int n, int eq, int vol, int tp, int curved, int gen);
+VjpLauncherF32 vjp_f32_3d(int n, int eq, int vol, int tp, int curved, int gen);
 
 }  // namespace flouhip

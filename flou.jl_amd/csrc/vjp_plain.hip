@@ -20,6 +20,7 @@
 #include "common.hpp"
 #include "physics_plain.hpp"
 #include "adjoint.hpp"
+#include "dual.hpp"
 #include <type_traits>
 
 namespace flouhip {
@@ -32,7 +33,36 @@ __device__ inline int pair_slot(int k, int nb, int f, int b) {
     return (k - 1) * NPE + nb;
 }
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
+// Reverse product of any Euler numerical flux in the face frame (Equations/Euler.jl:
+// 99-380: the StdAverage or Chandrasekhar average with LxF, Scalar or Matrix dissipation):
+// one forward evaluation of the plain flux in dual numbers per input direction (2 NV),
+// Rlb += (∂Φ/∂Rl)^T Fb, Rrb += (∂Φ/∂Rr)^T Fb.  Used where the derivative is not written out
+// by hand (adj::numflux_vjp covers StdAverage with LxF or no dissipation).
+template <typename T, int ND>
+__device__ inline void numflux_vjp_fwd(const T* Rl, const T* Rr, const RhsArgs<T>& a, const T* Fb, T* Rlb,
+                                       T* Rrb) {
+    using D = adj::Dual1<T>;
+    constexpr int NV = ND + 2;
+#pragma nounroll
+    for (int s = 0; s < 2 * NV; ++s) {
+        D L[NV], R[NV], F[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            L[v] = D(Rl[v], s == v ? T(1) : T(0));
+            R[v] = D(Rr[v], s == NV + v ? T(1) : T(0));
+        }
+        plain::euler_numericalflux<D, ND>(L, R, a.savg, a.sdiss, a.typo, D(a.intensity, T(0)), D(a.gamma, T(0)),
+                                          a.errflag, F);
+        T acc = Fb[0] * F[0].d;
+#pragma unroll
+        for (int v = 1; v < NV; ++v) acc = acc + Fb[v] * F[v].d;
+        if (s < NV) Rlb[s] = Rlb[s] + acc;
+        else Rrb[s - NV] = Rrb[s - NV] + acc;
+    }
+}
+
+// GEN: the face-flux VJP through numflux_vjp_fwd (any Euler surface flux)
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV, bool GEN>
 __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     using C = Cfg<ND, N>;
     constexpr int NPE = C::NPE, NPF = C::NPF, NF = C::NF, EPB = C::EPB, TPB = C::TPB;
@@ -282,7 +312,8 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             plain::rotate2face<T, ND>(Fb, g, Rb);
 #pragma unroll
             for (int v = 0; v < NV; ++v) { Rlb[v] = T(0); Rrb[v] = T(0); }
-            adj::numflux_vjp<T, ND>(Rl, Rr, a.sdiss == DISS_LXF, a.intensity, a.gamma, Rb, Rlb, Rrb);
+            if constexpr (GEN) numflux_vjp_fwd<T, ND>(Rl, Rr, a, Rb, Rlb, Rrb);
+            else adj::numflux_vjp<T, ND>(Rl, Rr, a.sdiss == DISS_LXF, a.intensity, a.gamma, Rb, Rlb, Rrb);
             plain::rotate2phys<T, ND>(Rlb, g, Qlb);
             plain::rotate2phys<T, ND>(Rrb, g, Qrb);
         }
@@ -478,19 +509,28 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     }
 }
 
-template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV>
+template <typename T, int ND, int N, int EQ, int VOL, int TP, bool CURV, bool GEN = false>
 static hipError_t launch_vjp(const VjpArgs<T>& a, hipStream_t s) {
     using C = Cfg<ND, N>;
     const int nblocks = (a.r.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_vjp<T, ND, N, EQ, VOL, TP, CURV>), dim3(nblocks), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_vjp<T, ND, N, EQ, VOL, TP, CURV, GEN>), dim3(nblocks), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
 template <typename T, int ND, int N, bool CURV>
-static hipError_t (*pick_vjp(int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
+static hipError_t (*pick_vjp(int eq, int vol, int tp, int gen))(const VjpArgs<T>&, hipStream_t) {
     if (eq == EQ_ADVECTION)
         return vol == VOL_STRONG ? &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_STRONG, 0, CURV>
                                  : &launch_vjp<T, ND, N, EQ_ADVECTION, VOL_SPLIT, 0, CURV>;
+    if (gen) {
+        if (vol == VOL_STRONG) return &launch_vjp<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV, true>;
+        switch (tp) {
+            case TP_STDAVG: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG, CURV, true>;
+            case TP_CHANDRASEKHAR: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_CHANDRASEKHAR, CURV, true>;
+            case TP_KENNEDY_GRUBER: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_KENNEDY_GRUBER, CURV, true>;
+            default: return nullptr;
+        }
+    }
     if (vol == VOL_STRONG) return &launch_vjp<T, ND, N, EQ_EULER, VOL_STRONG, 0, CURV>;
     switch (tp) {
         case TP_STDAVG: return &launch_vjp<T, ND, N, EQ_EULER, VOL_SPLIT, TP_STDAVG, CURV>;
@@ -501,15 +541,15 @@ static hipError_t (*pick_vjp(int eq, int vol, int tp))(const VjpArgs<T>&, hipStr
 }
 
 template <typename T, int ND, bool CURV>
-static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp))(const VjpArgs<T>&, hipStream_t) {
+static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp, int gen))(const VjpArgs<T>&, hipStream_t) {
     switch (n) {
-        case 2: return pick_vjp<T, ND, 2, CURV>(eq, vol, tp);
-        case 3: return pick_vjp<T, ND, 3, CURV>(eq, vol, tp);
-        case 4: return pick_vjp<T, ND, 4, CURV>(eq, vol, tp);
-        case 5: return pick_vjp<T, ND, 5, CURV>(eq, vol, tp);
-        case 6: return pick_vjp<T, ND, 6, CURV>(eq, vol, tp);
-        case 7: return pick_vjp<T, ND, 7, CURV>(eq, vol, tp);
-        case 8: return pick_vjp<T, ND, 8, CURV>(eq, vol, tp);
+        case 2: return pick_vjp<T, ND, 2, CURV>(eq, vol, tp, gen);
+        case 3: return pick_vjp<T, ND, 3, CURV>(eq, vol, tp, gen);
+        case 4: return pick_vjp<T, ND, 4, CURV>(eq, vol, tp, gen);
+        case 5: return pick_vjp<T, ND, 5, CURV>(eq, vol, tp, gen);
+        case 6: return pick_vjp<T, ND, 6, CURV>(eq, vol, tp, gen);
+        case 7: return pick_vjp<T, ND, 7, CURV>(eq, vol, tp, gen);
+        case 8: return pick_vjp<T, ND, 8, CURV>(eq, vol, tp, gen);
         default: return nullptr;
     }
 }
@@ -537,14 +577,14 @@ static hipError_t (*pick_vjp_n(int n, int eq, int vol, int tp))(const VjpArgs<T>
 #error "define FLOUHIP_ND"
 #endif
 
-hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp, int curved))(
+hipError_t (*FLOUHIP_NAME(FLOUHIP_PTAG, FLOUHIP_DTAG)(int n, int eq, int vol, int tp, int curved, int gen))(
     const VjpArgs<FLOUHIP_SCALAR>&, hipStream_t) {
 #if FLOUHIP_ND > 1
-    if (curved) return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, true>(n, eq, vol, tp);
+    if (curved) return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, true>(n, eq, vol, tp, gen);
 #else
     if (curved) return nullptr;
 #endif
-    return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, false>(n, eq, vol, tp);
+    return pick_vjp_n<FLOUHIP_SCALAR, FLOUHIP_ND, false>(n, eq, vol, tp, gen);
 }
 
 }  // namespace flouhip

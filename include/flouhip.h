@@ -260,9 +260,10 @@ int flou_hip_check(flou_hip_handle* h);
 
 /* Adjoint of rhs! (Equations/Hyperbolic.jl:31-69): Qbar += (d rhs / dQ)^T dQbar, evaluated
  * at Q.  Flou has no AD rule for rhs!; this backs the ChainRulesCore rrule of the binding.
- * Supported: the StdAverage surface average with or without LxF dissipation (not
- * MatrixDissipation or the Chandrasekhar surface average: FLOU_HIP_EUNSUPPORTED).
- * Qbar must not overlap Q or dQbar. */
+ * Supported: the strong and split forms with every surface flux (StdAverage or
+ * Chandrasekhar average, no dissipation, LxF, Scalar or Matrix dissipation); the
+ * HybridDivOperator, the split form on Gauss nodes and a handle with a stage limiter set
+ * return FLOU_HIP_EUNSUPPORTED.  Qbar must not overlap Q or dQbar. */
 int flou_hip_rhs_vjp(flou_hip_handle* h, const void* Q, const void* dQbar, void* Qbar);
 
 /* One RK step of FlouTime.timeintegrate (FlouTime.jl:34-56; the OrdinaryDiffEq

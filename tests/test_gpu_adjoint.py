@@ -9,6 +9,9 @@ Pins:
   * the full gradient of that misfit on a 4x4, N=3 mesh, column by column from dual-number
     JVPs of the oracle;
   * FP32 adjoint against the FP64 adjoint.
+Every Euler surface flux is covered: StdAverage with LxF or none by the hand-written face
+VJP, the Chandrasekhar average, LxF on it, Scalar and Matrix dissipation by the dual-number
+face VJP.
 """
 import numpy as np
 import pytest
@@ -42,12 +45,21 @@ VJP_CASES = [
     (2, 5, [4, 4], "strong", "std", "lxf", "adv", "GLL", None),
     (2, 5, [4, 4], "split", "std", "lxf", "adv", "GLL", None),
     (2, 4, [4, 3], "strong", "std", "lxf", "adv", "GL", None),
+    # surface fluxes whose face VJP is formed from dual-number evaluations of the flux
+    (1, 4, [6], "split", "ch", "matrix", "euler", "GLL", None),
+    (2, 4, [3, 3], "split", "ch", "scalar", "euler", "GLL", None),
+    (2, 4, [3, 3], "split", "ch", "matrix", "euler", "GLL", "walls"),
+    (2, 5, [3, 3], "split", "kg", "chavg", "euler", "GLL", "walls"),
+    (3, 4, [2, 2, 2], "split", "ch", "matrix", "euler", "GLL", None),
+    (3, 4, [2, 1, 2], "split", "ch", "lxf-ch", "euler", "GLL", None),
+    (3, 3, [2, 2, 2], "strong", "std", "scalar-std", "euler", "GLL", None),
+    (3, 4, [2, 2, 2], "split", "ch", "scalar", "euler", "GLL", None),
 ]
 
 
-def _find(nd, n, vol, tp, bcs, eq="euler"):
+def _find(nd, n, vol, tp, bcs, eq="euler", surf="lxf"):
     """The VJP case with these values (picked by value, so inserting a case moves nothing)."""
-    hits = [c for c in VJP_CASES if (c[0], c[1], c[3], c[4], c[6], c[8]) == (nd, n, vol, tp, eq, bcs)]
+    hits = [c for c in VJP_CASES if (c[0], c[1], c[3], c[4], c[5], c[6], c[8]) == (nd, n, vol, tp, surf, eq, bcs)]
     assert len(hits) == 1, hits
     return hits[0]
 
@@ -129,6 +141,8 @@ RK_VJP_CASES = {
     "3d-ch-n6": _find(3, 6, "split", "ch", None),    # FP64 3-D N=6: epilogue operands loaded early
     "2d-walls": _find(2, 4, "split", "ch", "walls"),
     "2d-walls-n6": _find(2, 6, "split", "ch", "walls"),
+    "2d-matrix-walls": _find(2, 4, "split", "ch", "walls", surf="matrix"),  # dual-number face VJP
+    "3d-scalar": _find(3, 4, "split", "ch", None, surf="scalar"),
 }
 
 
@@ -312,10 +326,3 @@ def test_fp32_adjoint_against_fp64_adjoint():
     print(f"FP32 adjoint vs FP64 adjoint: relative L2 error {err:.2e}")
     assert err <= 1e-4
 
-
-def test_vjp_rejects_matrix_dissipation():
-    mesh, std, equation, op, orc = build(2, 4, [2, 2], "split", "ch", "matrix")
-    disc = F.MultielementDisc(mesh, std, equation, op)
-    Q = disc.to_device(_state(disc, orc, 2, "euler", seed=1))
-    with pytest.raises(NotImplementedError, match="adjoint"):
-        A.rhs_vjp(torch.zeros_like(Q), torch.ones_like(Q), Q, disc)

@@ -21,6 +21,8 @@ SURF = {
     "matrix": (lambda: F.MatrixDissipation(F.ChandrasekharAverage(), 1.0), O.AVG_CHANDRASEKHAR, O.DISS_MATRIX),
     "scalar": (lambda: F.ScalarDissipation(F.ChandrasekharAverage(), 1.0), O.AVG_CHANDRASEKHAR, O.DISS_SCALAR),
     "scalar-std": (lambda: F.ScalarDissipation(F.StdAverage(), 0.5), O.AVG_STD, O.DISS_SCALAR),
+    "chavg": (lambda: F.ChandrasekharAverage(), O.AVG_CHANDRASEKHAR, O.DISS_NONE),
+    "lxf-ch": (lambda: F.LxF(F.ChandrasekharAverage(), 0.7), O.AVG_CHANDRASEKHAR, O.DISS_LXF),
 }
 TP = {"std": (F.StdAverage, O.TP_STDAVG), "ch": (F.ChandrasekharAverage, O.TP_CHANDRASEKHAR),
       "kg": (F.KennedyGruber, O.TP_KENNEDY_GRUBER)}
