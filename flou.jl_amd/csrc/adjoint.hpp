@@ -475,7 +475,7 @@ FLOU_HD inline void numflux_vjp(const T* Rl, const T* Rr, bool lxf, T intensity,
     for (int k = 0; k < ND; ++k) e1[k] = k == 0 ? T(1) : T(0);
     T h[NV];
     for (int v = 0; v < NV; ++v) h[v] = T(0.5) * Fb[v];
-    const T ir[2] = {T(1) / Rl[0], T(1) / Rr[0]};
+    const T ir[2] = {rcp(Rl[0]), rcp(Rr[0])};
     flux_vjp_ir<T, ND>(Rl, ir[0], e1, gamma, h, Rlb);
     flux_vjp_ir<T, ND>(Rr, ir[1], e1, gamma, h, Rrb);
     if (!lxf) return;
@@ -506,7 +506,7 @@ FLOU_HD inline void numflux_vjp(const T* Rl, const T* Rr, bool lxf, T intensity,
     for (int k = 0; k < ND; ++k) ub[k] = T(0);
     ub[0] = (un[w] > T(0) ? T(1) : (un[w] < T(0) ? T(-1) : T(0))) * lb;
     const T ab = lb;
-    const T pb = T(0.5) * ab * gamma * ir[w] / a[w];
+    const T pb = T(0.5) * ab * gamma * ir[w] * rcp(a[w]);
     const T rb = T(-0.5) * ab * a[w] * ir[w];
     prim_pullback_ir<T, ND>(R, ir[w], gamma, rb, ub, pb, Rb);
 }

@@ -26,6 +26,7 @@ struct flou_hip_handle {
     int eq = 0, vol = 0, tp = 0, savg = 0, sdiss = 0, typo = 0;
     double gamma = 1.4, intensity = 1.0, a[3] = {0, 0, 0}, anorm = 0.0;
     int gather = 0;
+    int lift_end = 0;
     int egeom_stride = 0;
     int curved = 0;   // per-node geometry (GEOM_CURVED)
     double blend = 1.0, sjac[3] = {1.0, 1.0, 1.0};  // HybridDivOperator
@@ -187,6 +188,7 @@ RhsArgs<T> make_args(flou_hip_handle* h) {
     a.bcconst = (const T*)h->d_bcconst;
     a.bctype = h->d_bctype;
     a.gather = h->gather;
+    a.lift_end = h->lift_end;
     a.gamma = (T)h->gamma;
     a.intensity = (T)h->intensity;
     a.a0 = (T)h->a[0];
@@ -434,9 +436,11 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     std::memcpy(&ops[ops_gp(n)], d->dg_plus, n * sizeof(double));
     std::memcpy(&ops[ops_w(n)], d->omega, n * sizeof(double));
     h->gather = 1;
+    h->lift_end = 1;
     for (int j = 0; j < n; ++j) {
         if (d->l_minus[j] != (j == 0 ? 1.0 : 0.0)) h->gather = 0;
         if (d->l_plus[j] != (j == n - 1 ? 1.0 : 0.0)) h->gather = 0;
+        if ((j > 0 && d->dg_minus[j] != 0.0) || (j < n - 1 && d->dg_plus[j] != 0.0)) h->lift_end = 0;
     }
 
     // geometry

@@ -75,6 +75,7 @@ struct RhsArgs {
     const T* bcconst;  // inflow states, nboundaries * nv
     const int* bctype; // nboundaries
     int gather;        // 1 when l = (e_1, e_n) exactly: traces are node gathers
+    int lift_end;      // 1 when ∂g± vanish exactly off the end nodes (GLL DGSEM lift)
     T gamma, intensity;
     T a0, a1, a2;      // advection velocity
     int savg, sdiss, typo;

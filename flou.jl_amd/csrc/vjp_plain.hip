@@ -213,16 +213,25 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         const T* lk = &sOps[lr ? ops_lp(N) : ops_lm(N)];
         const T* gk = &sOps[lr ? ops_gp(N) : ops_gm(N)];
         T own[NV], ext[NV], Lo[NV];
+        const int pend = lr ? N - 1 : 0, nend = line_node<ND, N>(dir, f, pend);
+        if (a.gather && a.lift_end) {  // GLL: the trace and the lift live on the end node
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            T s = T(0), t = T(0);
-            for (int p = 0; p < N; ++p) {
-                const int nn = line_node<ND, N>(dir, f, p);
-                s = s + lk[p] * sQ[le][v][nn];
-                t = t + gk[p] * sW[le][v][nn];
+            for (int v = 0; v < NV; ++v) {
+                own[v] = sQ[le][v][nend];
+                Lo[v] = gk[pend] * sW[le][v][nend];
             }
-            own[v] = a.gather ? sQ[le][v][line_node<ND, N>(dir, f, lr ? N - 1 : 0)] : s;
-            Lo[v] = t;
+        } else {
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                T s = T(0), t = T(0);
+                for (int p = 0; p < N; ++p) {
+                    const int nn = line_node<ND, N>(dir, f, p);
+                    s = s + lk[p] * sQ[le][v][nn];
+                    t = t + gk[p] * sW[le][v][nn];
+                }
+                own[v] = a.gather ? sQ[le][v][nend] : s;
+                Lo[v] = t;
+            }
         }
         const int slot = e * NF + k;
         const bool first = pf && it == tid;  // issued before the staging barrier
@@ -258,6 +267,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
 #pragma unroll
                     for (int p = 0; p < N; ++p) t = t + gq[p] * pf_yl[p][v];
                     ext[v] = pf_ext[v];
+                } else if (a.gather && a.lift_end) {  // one node of the neighbour's line
+                    const int pn = nlr ? N - 1 : 0;
+                    const int64_t nn = base + line_node<ND, N>(ndir, fn, pn);
+                    const int64_t gi = v * Nd + nn;
+                    t = gq[pn] * (CURV ? A.Y[gi] / a.ngeom[ND * ND * Nd + nn] : A.Y[gi]);
+                    ext[v] = a.Q[gi];
                 } else {
                     for (int p = 0; p < N; ++p) {
                         const int64_t nn = base + line_node<ND, N>(ndir, fn, p);
