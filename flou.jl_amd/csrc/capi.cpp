@@ -75,6 +75,7 @@ struct flou_hip_handle {
     void* d_fegeom = nullptr;
     int fegeom_stride = 0;
     void* d_fdsharp = nullptr;
+    void* d_fnj = nullptr;      // curved fast kernel: per face node (normal, surface J), float4
     float lift_m = 0.0f, lift_p = 0.0f, fdiss = 0.0f;
     std::string err;
 };
@@ -136,7 +137,7 @@ void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
                     h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
-                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol, h->d_eqA};
+                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol, h->d_eqA, h->d_fnj};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -265,6 +266,7 @@ int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, d
         a.egeom = (const float*)h->d_fegeom;
         a.egeom_stride = h->fegeom_stride;
         a.fgeom = (const float*)h->d_fgeom;
+        a.fnj = (const float4*)h->d_fnj;
         a.ngeom = (const float*)h->d_ngeom;
         a.dsharp = (const float*)h->d_fdsharp;
         a.lift_m = h->lift_m;
@@ -601,7 +603,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable) and every face
     // orientation 0 (the fast kernels pair face nodes by index).  Anything else runs the
     // plain kernel in FP32.
-    std::vector<float> fe, fds;
+    std::vector<float> fe, fds, fnj;
     if (h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
         h->vol == FLOU_HIP_SPLIT && h->savg == FLOU_HIP_AVG_STD &&
         (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
@@ -645,6 +647,16 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             }
             h->fegeom_stride = uniform ? 0 : 8;
         }
+        if (h->lfast && h->curved) {
+            // the curved kernel's face data: normal and surface J of each face node, the 4 of
+            // the 3nd+1 values per node it uses, as one 16-byte vector
+            const int64_t nq = nf * h->npf;
+            fnj.assign(nq * 4, 0.0f);
+            for (int64_t q = 0; q < nq; ++q) {
+                for (int c = 0; c < nd; ++c) fnj[q * 4 + c] = (float)fg[q * FG + c];
+                fnj[q * 4 + 3] = (float)fg[q * FG + 3 * nd];
+            }
+        }
         if (h->lfast) {
             fds.resize(n * n);
             for (int i = 0; i < n * n; ++i) fds[i] = (float)d->Dsharp[i];
@@ -668,6 +680,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     if (h->lfast) {
         int rcf = fe.empty() ? 0 : upload(h, &h->d_fegeom, fe);
         rcf |= upload(h, &h->d_fdsharp, fds);
+        if (!fnj.empty()) rcf |= upload(h, &h->d_fnj, fnj);
         if (rcf) return bad(FLOU_HIP_ENOMEM, "device upload failed: " + h->err);
     }
     int rc = 0;

@@ -23,6 +23,7 @@ struct FastArgs {
     int egeom_stride;   // 0 when every element is alike, else 8
     const float* fgeom; // per entry: frame (n, t, b) then surface J (3nd + 1 floats); curved: per face node
     const float* ngeom; // curved: per node, component-major [nd*nd + 1][ndofs] (metric, J)
+    const float4* fnj;  // curved: per face node of each face entry, (normal, surface J)
     const float* dsharp;// D♯, row-major n x n
     float lift_m, lift_p; // ∂g_minus[0], ∂g_plus[n-1]
     const float* bcstate;

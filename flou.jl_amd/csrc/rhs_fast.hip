@@ -861,7 +861,6 @@ __global__ void __launch_bounds__((CCfg<ND, N>::TPB)) k_fastc(const FastArgs a) 
     using C = CCfg<ND, N>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
     constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
-    constexpr int FG = 3 * ND + 1;
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
     float* const sQ = sm + C::OFF_Q;
@@ -907,14 +906,16 @@ __global__ void __launch_bounds__((CCfg<ND, N>::TPB)) k_fastc(const FastArgs a) 
 #pragma unroll
         for (int r = 0; r < NPT; ++r) g[r][c] = bload(rg, (uint32_t)(tid + r * TPB) * 4u);
     }
-    // face normals and surface Jacobians of both line ends (per face node, master side)
+    // face normals and surface Jacobians of both line ends (per face node, master side),
+    // one 16-byte load each
     float fn[2][ND], fj[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
-        const float* fg = a.fgeom + ((int64_t)xfg[sd] * NPF + lf) * FG;
-#pragma unroll
-        for (int k = 0; k < ND; ++k) fn[sd][k] = line_ok ? fg[k] : 0.0f;
-        fj[sd] = line_ok ? fg[3 * ND] : 0.0f;
+        const float4 q = line_ok ? a.fnj[(int64_t)xfg[sd] * NPF + lf] : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        fn[sd][0] = q.x;
+        if (ND > 1) fn[sd][1] = q.y;
+        if (ND > 2) fn[sd][ND - 1] = q.z;
+        fj[sd] = q.w;
     }
     // exterior traces
     uint32_t xo[2];
