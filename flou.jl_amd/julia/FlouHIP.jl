@@ -660,51 +660,72 @@ function step_vjp!(hd, code, dt, t, stages::Vector{<:HIPState}, ubar::HIPState)
 end
 
 """
-    integrate_steps(Q0::HIPState, disc, equation, solver, dt, nsteps; checkpoint_every=20)
+    integrate_steps(Q0::HIPState, disc, equation, solver, dt, nsteps; checkpoint_every=20,
+                    keep_segments=1)
 
 `nsteps` fixed-dt RK steps from `Q0` (not modified); the differentiable form of
 `timeintegrate`.  Its `rrule` keeps a checkpoint every `checkpoint_every` steps in the
-forward run; the pullback recomputes each segment once with `flou_hip_rk_step_record`
-(keeping the stage inputs) and walks it backwards with `flou_hip_rk_step_vjp`, so memory
-is (nsteps / K + K·stages) states.
+forward run and, for the last `keep_segments` segments, every step's stage inputs; the
+pullback walks a kept segment backwards with `flou_hip_rk_step_vjp` directly and recomputes
+any other segment once with `flou_hip_rk_step_record` (keeping the stage inputs) first, so
+memory is (nsteps / K + (keep_segments + 1)·K·stages) states.
 """
 function integrate_steps(Q0::HIPState, disc::MultielementDisc, equation, solver, dt, nsteps;
-                         checkpoint_every=20)
-    Q, _ = forward_checkpointed(Q0, disc, solver, dt, nsteps, nsteps + 1)
+                         checkpoint_every=20, keep_segments=1)
+    Q, _, _ = forward_checkpointed(Q0, disc, solver, dt, nsteps, nsteps + 1, 0)
     return Q
 end
 
-function forward_checkpointed(Q0::HIPState, disc, solver, dt, nsteps, K)
+function forward_checkpointed(Q0::HIPState, disc, solver, dt, nsteps, K, keep_segments)
     hd = owner_of(disc, Q0)
     code = scheme_code(solver)
     stage_limiter(solver) === nothing || throw(ArgumentError("adjoint: the stage limiter is not differentiated"))
     S = nstages(code)
+    nseg = cld(nsteps, K)
+    first_kept = max(0, nseg - keep_segments)      # 0-based index of the first kept segment
     stages = HIPState[copy(Q0)]
     append!(stages, [similar(Q0) for _ in 2:S])
     nxt = similar(Q0)
     cps = HIPState[]
+    kept = Dict{Int,Vector{Vector{HIPState}}}()    # segment (1-based) => stage inputs per step
     for n in 0:nsteps - 1
+        sg = n ÷ K
         n % K == 0 && push!(cps, copy(stages[1]))
-        step_record!(hd, code, dt, n * dt, stages, nxt)
+        if sg >= first_kept
+            steps = get!(kept, sg + 1, Vector{Vector{HIPState}}())
+            st = HIPState[copy(stages[1])]
+            append!(st, [similar(Q0) for _ in 2:S])
+            step_record!(hd, code, dt, n * dt, st, nxt)
+            push!(steps, st)
+        else
+            step_record!(hd, code, dt, n * dt, stages, nxt)
+        end
         stages[1], nxt = nxt, stages[1]
     end
-    return stages[1], cps
+    return stages[1], cps, kept
 end
 
 function ChainRulesCore.rrule(::typeof(integrate_steps), Q0::HIPState, disc::MultielementDisc, equation,
-                              solver, dt, nsteps; checkpoint_every=20)
+                              solver, dt, nsteps; checkpoint_every=20, keep_segments=1)
     K = max(1, checkpoint_every)
-    QT, cps = forward_checkpointed(Q0, disc, solver, dt, nsteps, K)
+    QT, cps, kept = forward_checkpointed(Q0, disc, solver, dt, nsteps, K, keep_segments)
     hd = owner_of(disc, Q0)
     code = scheme_code(solver)
     S = nstages(code)
     function integrate_steps_pullback(QTbar)
         ubar = copy(unthunk(QTbar))
-        seg = [[similar(Q0) for _ in 1:S] for _ in 1:min(K, nsteps)]
+        seg = nothing
         nxt = similar(Q0)
         for sgi in length(cps):-1:1
             n0 = (sgi - 1) * K
             nloc = min(K, nsteps - n0)
+            if haskey(kept, sgi)  # stage inputs kept by the forward run
+                for j in nloc:-1:1
+                    step_vjp!(hd, code, dt, (n0 + j - 1) * dt, kept[sgi][j], ubar)
+                end
+                continue
+            end
+            seg === nothing && (seg = [[similar(Q0) for _ in 1:S] for _ in 1:min(K, nsteps)])
             copyto!(seg[1][1], cps[sgi])
             for j in 1:nloc  # recompute the segment, keeping every stage input
                 step_record!(hd, code, dt, (n0 + j - 1) * dt, seg[j], j < nloc ? seg[j + 1][1] : nxt)
