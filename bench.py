@@ -85,9 +85,13 @@ def run_steps(disc, Q, dt, nsteps, bufs):
     return cur
 
 
-def time_form(precision, Q64, dt, steps, warmup):
+def time_form(precision, Q64, dt, steps, warmup, plain=False):
+    """CK2N54 steps of the C3 workload in one precision; plain: the reference-order kernel
+    even where the fast kernel applies (option 1 of flou_hip_set_option)."""
     import torch
     disc, eq, std = build_disc(precision)
+    if plain:
+        disc.set_option(1, 1)
     Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()
     bufs = [disc.new_state(), disc.new_state()]
     cur = run_steps(disc, Q, dt, warmup, bufs)
@@ -488,6 +492,13 @@ def main():
         d = res["fp32"]["rho"] - r64["rho"]
         out["density_rel_l2_err_fp32_vs_fp64"] = float(_t.linalg.norm(d) / _t.linalg.norm(r64["rho"]))
         out["density_err_after_steps"] = args.steps + args.warmup
+        # the speedup split into its two parts: precision (FP64 -> FP32, both in the
+        # reference-order plain kernel) and algorithm (FP32 plain -> FP32 fast kernel)
+        rp = time_form("fp32", Q64, dt, args.steps, args.warmup, plain=True)
+        print(f"# fp32 plain: {rp['gpu_ms'] / (5 * args.steps):.3f} ms/stage", file=sys.stderr, flush=True)
+        out["fp32_plain"] = {"value": nn * stages / (rp["gpu_ms"] / 1e3), "ms_per_step": rp["gpu_ms"] / args.steps,
+                             "speedup_fp64_plain_to_fp32_plain": r64["gpu_ms"] / rp["gpu_ms"],
+                             "speedup_fp32_plain_to_fp32_fast": rp["gpu_ms"] / r["gpu_ms"]}
     if not args.no_secondary and not args.only:
         for name in ("c2", "c5"):
             out[name] = secondary_workload(name)
