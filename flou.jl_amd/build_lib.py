@@ -34,7 +34,8 @@ EXACT = ["-ffp-contract=off"]
 def units():
     u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT), ("limiter.o", "limiter.hip", EXACT),
          ("output.o", "output.hip", []),
-         ("rhs_fast.o", "rhs_fast.hip", ["-ffp-contract=fast"])]
+         ("rhs_fast.o", "rhs_fast.hip", ["-ffp-contract=fast"]),
+         ("vjp_fast.o", "vjp_fast.hip", ["-ffp-contract=fast"])]
     for prec in ("F64", "F32"):
         for nd in (1, 2, 3):
             u.append((f"rhs_plain_{prec.lower()}_{nd}d.o", "rhs_plain.hip",

@@ -76,6 +76,10 @@ struct flou_hip_handle {
     int fegeom_stride = 0;
     void* d_fdsharp = nullptr;
     void* d_fnj = nullptr;      // curved fast kernel: per face node (normal, surface J), float4
+    // FP32 adjoint on the curved fast path: its kernel, S_ij = ω_i D♯_ij and 1/ω
+    FastVjpLauncher vfast = nullptr;
+    void* d_fwsh = nullptr;
+    void* d_fiw = nullptr;
     float lift_m = 0.0f, lift_p = 0.0f, fdiss = 0.0f;
     std::string err;
 };
@@ -137,7 +141,8 @@ void free_all(flou_hip_handle* h) {
     void* ptrs[] = {h->d_ops, h->d_nb, h->d_finfo, h->d_fgi, h->d_fid, h->d_egeom, h->d_fgeom,
                     h->d_dxe, h->d_jac, h->d_w, h->d_bcstate, h->d_bcconst, h->d_bctype,
                     h->d_err, h->d_partial, h->d_out, h->d_work[0], h->d_work[1], h->d_work[2],
-                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol, h->d_eqA, h->d_fnj};
+                    h->d_fegeom, h->d_fdsharp, h->d_ngeom, h->d_evol, h->d_eqA, h->d_fnj,
+                    h->d_fwsh, h->d_fiw};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -204,6 +209,35 @@ RhsArgs<T> make_args(flou_hip_handle* h) {
     return a;
 }
 
+// The fast kernels' arguments for the state Q (the stage fields left zero).
+FastArgs fast_args(const flou_hip_handle* h, const void* Q) {
+    FastArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.Q = (const float*)Q;
+    a.ndofs = h->ndofs;
+    a.ne = (int)h->ne;
+    a.nb = h->d_nb;
+    a.finfo = h->d_finfo;
+    a.fgi = h->d_fgi;
+    a.fid = h->d_fid;
+    a.egeom = (const float*)h->d_fegeom;
+    a.egeom_stride = h->fegeom_stride;
+    a.fgeom = (const float*)h->d_fgeom;
+    a.fnj = (const float4*)h->d_fnj;
+    a.ngeom = (const float*)h->d_ngeom;
+    a.dsharp = (const float*)h->d_fdsharp;
+    a.lift_m = h->lift_m;
+    a.lift_p = h->lift_p;
+    a.bcstate = (const float*)h->d_bcstate;
+    a.bcconst = (const float*)h->d_bcconst;
+    a.bctype = h->d_bctype;
+    a.gamma = (float)h->gamma;
+    a.inv_gm1 = (float)(1.0 / (h->gamma - 1.0));
+    a.diss = h->fdiss;
+    a.errflag = h->d_err;
+    return a;
+}
+
 // One adjoint pass (see VjpArgs): o0 = a0·P0 + a1·P1 + (d rhs/dQ)^T(c·Y) at Q,
 // o1 = b0·o0 + b1·P1 + b2·P2.  o0/o1 may alias P0/P1/P2 (point-wise) but not Y or Q.
 int launch_vjp(flou_hip_handle* h, const void* Q, const void* Y, double c, const void* P0, double a0,
@@ -221,6 +255,17 @@ int launch_vjp(flou_hip_handle* h, const void* Q, const void* Y, double c, const
         a.a0 = a0; a.a1 = a1; a.b0 = b0; a.b1 = b1; a.b2 = b2;
         a.o0 = (double*)o0; a.o1 = (double*)o1;
         e = h->v64(a, h->stream);
+    } else if (h->vfast && !h->force_plain) {
+        FastVjpArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.f = fast_args(h, Q);
+        a.Y = (const float*)Y; a.c = (float)c;
+        a.P0 = (const float*)P0; a.P1 = (const float*)P1; a.P2 = (const float*)P2;
+        a.a0 = (float)a0; a.a1 = (float)a1; a.b0 = (float)b0; a.b1 = (float)b1; a.b2 = (float)b2;
+        a.o0 = (float*)o0; a.o1 = (float*)o1;
+        a.wsh = (const float*)h->d_fwsh;
+        a.iw = (const float*)h->d_fiw;
+        e = h->vfast(a, h->stream);
     } else {
         VjpArgs<float> a;
         std::memset(&a, 0, sizeof(a));
@@ -249,35 +294,12 @@ int launch_stage(flou_hip_handle* h, int mode, int first, double dt, double A, d
         a.mode = mode; a.first = first;
         e = h->l64(a, h->stream);
     } else if (h->lfast && !h->force_plain) {
-        FastArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.Q = (const float*)u_in;
+        FastArgs a = fast_args(h, u_in);
         a.out = (float*)out;
         a.ubase = (const float*)ubase;
         a.reg = (float*)reg;
         a.dt = (float)dt; a.A = (float)A; a.B = (float)B;
         a.mode = mode; a.first = first;
-        a.ndofs = h->ndofs;
-        a.ne = (int)h->ne;
-        a.nb = h->d_nb;
-        a.finfo = h->d_finfo;
-        a.fgi = h->d_fgi;
-        a.fid = h->d_fid;
-        a.egeom = (const float*)h->d_fegeom;
-        a.egeom_stride = h->fegeom_stride;
-        a.fgeom = (const float*)h->d_fgeom;
-        a.fnj = (const float4*)h->d_fnj;
-        a.ngeom = (const float*)h->d_ngeom;
-        a.dsharp = (const float*)h->d_fdsharp;
-        a.lift_m = h->lift_m;
-        a.lift_p = h->lift_p;
-        a.bcstate = (const float*)h->d_bcstate;
-        a.bcconst = (const float*)h->d_bcconst;
-        a.bctype = h->d_bctype;
-        a.gamma = (float)h->gamma;
-        a.inv_gm1 = (float)(1.0 / (h->gamma - 1.0));
-        a.diss = h->fdiss;
-        a.errflag = h->d_err;
         e = h->lfast(a, h->stream);
     } else {
         RhsArgs<float> a = make_args<float>(h);
@@ -603,7 +625,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
     // diagonal, with fewer than 2^30 nodes (32-bit byte offsets per variable) and every face
     // orientation 0 (the fast kernels pair face nodes by index).  Anything else runs the
     // plain kernel in FP32.
-    std::vector<float> fe, fds, fnj;
+    std::vector<float> fe, fds, fnj, fws, fiw;
     if (h->prec == FLOU_HIP_FP32 && (nd == 2 || nd == 3) && h->eq == FLOU_HIP_EULER &&
         h->vol == FLOU_HIP_SPLIT && h->savg == FLOU_HIP_AVG_STD &&
         (h->sdiss == FLOU_HIP_DISS_NONE || h->sdiss == FLOU_HIP_DISS_LXF) && n >= 4 &&
@@ -657,6 +679,23 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                 fnj[q * 4 + 3] = (float)fg[q * FG + 3 * nd];
             }
         }
+        if (h->lfast && h->curved && h->v32 && std::find(nb.begin(), nb.end(), -1) == nb.end()) {
+            // the FP32 adjoint on this path (Chandrasekhar volume flux, every face interior):
+            // S_ij = ω_i D♯_ij, antisymmetric off the zero diagonal (SBP on GLL nodes)
+            double smax = 0.0, asym = 0.0;
+            fws.resize(n * n);
+            fiw.resize(n);
+            for (int i = 0; i < n; ++i) {
+                fiw[i] = (float)(1.0 / d->omega[i]);
+                for (int j = 0; j < n; ++j) {
+                    const double sij = d->omega[i] * d->Dsharp[i * n + j], sji = d->omega[j] * d->Dsharp[j * n + i];
+                    smax = std::max(smax, std::fabs(sij));
+                    asym = std::max(asym, std::fabs(sij + sji));
+                    fws[i * n + j] = (float)(0.5 * (sij - sji));
+                }
+            }
+            if (asym <= 1e-10 * smax) h->vfast = fast_vjp_launcher(nd, n, h->tp);
+        }
         if (h->lfast) {
             fds.resize(n * n);
             for (int i = 0; i < n * n; ++i) fds[i] = (float)d->Dsharp[i];
@@ -681,6 +720,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
         int rcf = fe.empty() ? 0 : upload(h, &h->d_fegeom, fe);
         rcf |= upload(h, &h->d_fdsharp, fds);
         if (!fnj.empty()) rcf |= upload(h, &h->d_fnj, fnj);
+        if (h->vfast) rcf |= upload(h, &h->d_fwsh, fws) | upload(h, &h->d_fiw, fiw);
         if (rcf) return bad(FLOU_HIP_ENOMEM, "device upload failed: " + h->err);
     }
     int rc = 0;
@@ -732,6 +772,7 @@ int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value) {
     if (which == 0) { h->typo = value ? 1 : 0; return FLOU_HIP_OK; }
     if (which == 1) { h->force_plain = value ? 1 : 0; return FLOU_HIP_OK; }
     if (which == 2) return h->lfast ? 1 : 0;  // query: does FP32 run the fast kernel?
+    if (which == 3) return h->vfast ? 1 : 0;  // query: does the FP32 adjoint run the fast kernel?
     return fail(h, FLOU_HIP_EINVAL, "unknown option");
 }
 

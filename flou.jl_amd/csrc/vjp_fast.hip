@@ -1,0 +1,561 @@
+// FP32 adjoint of the curved fast path: the vector-Jacobian product of the rhs of the
+// split-form Euler operator with the Chandrasekhar two-point flux and the LxF(StdAverage)
+// or StdAverage surface flux on GLL nodes (SplitDivOperator, interface_fluxes!, the FR
+// lift and 1/J: OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:
+// 99-165, 474-536, MultielementDiscontinuous.jl:132-137), fused with the reverse RK stage
+// epilogue (FastVjpArgs).  The same mathematics as k_vjp (vjp_plain.hip), organised like
+// the forward fast kernel k_fastc (rhs_fast.hip):
+//   * EPB elements per block, one thread per (element, direction, line);
+//   * staging: per node Q, Y, the metric and J from HBM; to LDS the state, the halved
+//     primitives a = ρ/2, v = u/2 and β = ρ/(2p), the metric vectors Ja^d/2 and
+//     w = c·Y/J; each line thread also loads the neighbour's trace and Y/J at its two
+//     line ends;
+//   * line phase: the reverse product of the face flux at both line ends with respect to
+//     the own trace (conservative variables, into a face array in LDS), then every pair
+//     of the line once.  Off the (zero) diagonal of D♯ on GLL nodes S_ij = ω_i D♯_ij is
+//     antisymmetric, so the cotangent of the pair flux F♯(i, j), -D♯_ij w_i - D♯_ji w_j,
+//     is S_ij (w_j/ω_j - w_i/ω_i); its reverse product is accumulated for both nodes in
+//     registers, as cotangents of (a, v, β);
+//   * per node: Σ_d of the line rows, one pullback (a, v, β) -> (ρ, m, E), the face terms
+//     of the node's faces, then the epilogue
+//       o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2  (o1 optional).
+// Faces must all be interior (periodic or between elements) and of orientation 0.
+#include "common.hpp"
+#include "fast.hpp"
+#include "fast_util.hpp"
+
+namespace flouhip {
+namespace fast {
+
+template <int ND, int N>
+constexpr int fvjp_epb() {
+    return ND == 3 ? (N == 4 ? 4 : (N == 5 ? 3 : (N == 6 ? 2 : 1)))
+                   : (N == 4 ? 16 : (N == 5 ? 32 : (N == 6 ? 16 : (N == 7 ? 9 : 8))));
+}
+
+template <int ND, int N>
+struct VCfg {
+    static constexpr int NV = ND + 2, NFC = 2 * ND;
+    static constexpr int NPE = ipow(N, ND), NPF = ipow(N, ND - 1), NL = ND * NPF;
+    static constexpr int EPB = fvjp_epb<ND, N>();
+    static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;
+    static constexpr int NN = EPB * NPE;
+    static constexpr int NPT = (NN + TPB - 1) / TPB;
+    // LDS slot arrays [slot][NN]: Q (NV), the primitives a, v.., β (NV), the metric vectors
+    // Ja^d/2 (ND*ND), w = c·Y/J (NV).  The row sums of the ND directions (ND*NV slots) go
+    // over them once the lines are done.  Then the face cotangents [EPB][2ND][NV][NPF].
+    static constexpr int OFF_Q = 0;
+    static constexpr int OFF_A = NV * NN;
+    static constexpr int OFF_M = 2 * NV * NN;
+    static constexpr int OFF_W = (2 * NV + ND * ND) * NN;
+    static constexpr int OFF_F = (3 * NV + ND * ND) * NN;
+    static constexpr int LDS = r4(OFF_F + EPB * NFC * NV * NPF);
+    static_assert(ND * NV <= 3 * NV + ND * ND, "row sums must fit in the staging slots");
+    static_assert(EPB * NL >= 64, "lanes past the line tasks repeat a line of the block");
+};
+
+// The log-mean factor F(u) = atanh(f)/f at u = f², f = (x_i - x_j)/(x_i + x_j), and
+// dF/du: the reference's series 1 + u/3 + u²/5 + u³/7 for u < 0.01, else
+// ln(x_i/x_j)/(2f) (Utilities.jl:34-44) with dF/du = (1/(1-u) - F)/(2u).  SER: the caller
+// knows that u < 0.01.  lxi, lxj: ln x_i, ln x_j (unused on the series branch).
+template <bool SER>
+__device__ __forceinline__ void lm_factor(float u, float f, float lxi, float lxj, float& F, float& dF) {
+    const float Fs = fmaf(u, fmaf(u, fmaf(u, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
+    const float dFs = fmaf(u, fmaf(u, 3.0f / 7.0f, 2.0f / 5.0f), 1.0f / 3.0f);
+    if (SER) {
+        F = Fs;
+        dF = dFs;
+    } else {
+        // both branches evaluated and selected (no divergent control flow in the unrolled
+        // pair block); the log form is not used where u is small or zero
+        const float Fl = 0.5f * (lxi - lxj) * rcp(f);
+        const float dFl = 0.5f * (rcp(1.0f - u) - Fl) * rcp(u);
+        const bool ser = u < 0.01f;
+        F = ser ? Fs : Fl;
+        dF = ser ? dFs : dFl;
+    }
+}
+
+// Reverse product of the Chandrasekhar two-point flux along n̄ = M_i + M_j (M = Ja^d/2;
+// Equations/Euler.jl:474-536) for the flux cotangent G, as cotangents of the halved node
+// primitives (a, v.., β), scaled by s and added to gi and gj.  Forward:
+//   ρ̂ = (a_i + a_j)/F_ρ,  p̄ = (a_i + a_j)/(β_i + β_j),  ū = v_i + v_j,  m = ρ̂ ū·n̄,
+//   h = (F_β/(γ-1) + F_ρ)/(β_i + β_j) + 2 v_i·v_j
+//     (= 1/(2(γ-1)β̂) - (|u_i|² + |u_j|²)/4 + p̄/ρ̂ + |ū|²),
+//   F♯ = (m, m ū + p̄ n̄, m h),
+// F_ρ, F_β the log-mean factors of ρ and β.
+template <int ND, bool SER>
+__device__ __forceinline__ void pair_vjp(float ai, float aj, const float* vi, const float* vj, float bi, float bj,
+                                         float lai, float laj, float lbi, float lbj, const float* Mi,
+                                         const float* Mj, const float* G, float s, float ig1, float* gi, float* gj) {
+    constexpr int NV = ND + 2;
+    float n[ND], ub[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        n[k] = Mi[k] + Mj[k];
+        ub[k] = vi[k] + vj[k];
+    }
+    float un = ub[0] * n[0], dot = vi[0] * vj[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        un = fmaf(ub[k], n[k], un);
+        dot = fmaf(vi[k], vj[k], dot);
+    }
+    const float hs = ai + aj, dA = ai - aj, sb = bi + bj, db = bi - bj;
+    const float rb = rcp(sb), ihs = rcp(hs);
+    const float fr = dA * ihs, ur = fr * fr, fb = db * rb, u2 = fb * fb;
+    float Fr, dFr, Fb, dFb;
+    lm_factor<SER>(ur, fr, lai, laj, Fr, dFr);
+    lm_factor<SER>(u2, fb, lbi, lbj, Fb, dFb);
+    const float iFr = rcp(Fr);
+    const float rho = hs * iFr;
+    const float m = rho * un;
+    const float Xf = fmaf(Fb, ig1, Fr);
+    const float h = fmaf(rb, Xf, 2.0f * dot);
+    // reverse
+    float gu = G[1] * ub[0], gn = G[1] * n[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        gu = fmaf(G[1 + k], ub[k], gu);
+        gn = fmaf(G[1 + k], n[k], gn);
+    }
+    const float hb = G[NV - 1] * m;                // h̄
+    const float mb = fmaf(G[NV - 1], h, G[0] + gu);  // m̄
+    const float mbr = mb * rho;                      // cotangent of ū·n̄
+    const float rhob = mb * un;                      // cotangent of ρ̂
+    const float t = hb * rb;
+    float rbb = fmaf(hb, Xf, gn * hs);               // of 1/(β_i + β_j), through h and p̄
+    const float Frb = fmaf(-rhob * rho, iFr, t);     // of F_ρ, through ρ̂ and h
+    float hsb = fmaf(gn, rb, rhob * iFr);            // of a_i + a_j, through p̄ and ρ̂
+    const float frb = 2.0f * fr * Frb * dFr;
+    const float dAb = frb * ihs;
+    hsb = fmaf(-dAb, fr, hsb);
+    const float fbb = 2.0f * fb * (t * ig1) * dFb;
+    const float dbb = fbb * rb;
+    rbb = fmaf(fbb, db, rbb);
+    const float sbb = -rbb * rb * rb;
+    const float dotb = 2.0f * hb;
+    gi[0] = fmaf(s, hsb + dAb, gi[0]);
+    gj[0] = fmaf(s, hsb - dAb, gj[0]);
+    gi[NV - 1] = fmaf(s, sbb + dbb, gi[NV - 1]);
+    gj[NV - 1] = fmaf(s, sbb - dbb, gj[NV - 1]);
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        const float ubb = fmaf(m, G[1 + k], mbr * n[k]);
+        gi[1 + k] = fmaf(s, fmaf(dotb, vj[k], ubb), gi[1 + k]);
+        gj[1 + k] = fmaf(s, fmaf(dotb, vi[k], ubb), gj[1 + k]);
+    }
+}
+
+// All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
+// j's read from the LDS slot arrays (a, v.., β at sA; the metric vector of the line's
+// direction at sM; w at sW), w scaled by 1/ω of the node's position on the line.
+template <int ND, int N, int NN, bool SER>
+__device__ __forceinline__ void line_vjp(const float* __restrict__ sA, const float* __restrict__ sM,
+                                         const float* __restrict__ sW, int base, int stride,
+                                         const float* __restrict__ iw, const float* __restrict__ wsh, float ig1,
+                                         float (&acc)[N][ND + 2]) {
+    constexpr int NV = ND + 2;
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        const int ni = base + i * stride;
+        float ai, vi[ND], bi, Mi[ND], wi[NV];
+        ai = sA[ni];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            vi[k] = sA[(1 + k) * NN + ni];
+            Mi[k] = sM[k * NN + ni];
+        }
+        bi = sA[(ND + 1) * NN + ni];
+        const float iwi = iw[i];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) wi[v] = sW[v * NN + ni] * iwi;
+        const float lai = SER ? 0.0f : lg(ai), lbi = SER ? 0.0f : lg(bi);
+#pragma unroll
+        for (int j = i + 1; j < N; ++j) {
+            const int nj = base + j * stride;
+            float aj, vj[ND], bj, Mj[ND], G[NV];
+            aj = sA[nj];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) {
+                vj[k] = sA[(1 + k) * NN + nj];
+                Mj[k] = sM[k * NN + nj];
+            }
+            bj = sA[(ND + 1) * NN + nj];
+            const float iwj = iw[j];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) G[v] = fmaf(sW[v * NN + nj], iwj, -wi[v]);
+            const float laj = SER ? 0.0f : lg(aj), lbj = SER ? 0.0f : lg(bj);
+            pair_vjp<ND, SER>(ai, aj, vi, vj, bi, bj, lai, laj, lbi, lbj, Mi, Mj, G, wsh[i * N + j], ig1, acc[i],
+                              acc[j]);
+            // keep each pair's loads and arithmetic together: the registers then hold the
+            // line's rows, node i and one pair (the scheduler would otherwise hoist the
+            // loads of every pair and spill)
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
+// Reverse product, with respect to the own trace o, of the face flux as the own element
+// sees it at a line end (Interfaces.jl:18-23, Equations/Euler.jl:99-165 in the own frame):
+//   S = (f(o) + f(x))·n + λ·diss·(o - x),  λ = max(|u_o·n| + c_o, |u_x·n| + c_x),
+// with n the own outward normal and x the neighbour's trace; adds (∂S/∂o)^T G to ob.
+// Returns false if a sound speed is not real.
+template <int ND>
+__device__ __forceinline__ bool face_vjp(const float* o, const float* x, const float* n, const float* G,
+                                         float gamma, float gm1, float diss, float* ob) {
+    constexpr int NV = ND + 2;
+    const float ir = rcp(o[0]);
+    float u[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) u[k] = o[1 + k] * ir;
+    float uu = u[0] * u[0], un = u[0] * n[0], gm = G[1] * o[1], gn = G[1] * n[0];
+    float mx2 = x[1] * x[1], mxn = x[1] * n[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        uu = fmaf(u[k], u[k], uu);
+        un = fmaf(u[k], n[k], un);
+        gm = fmaf(G[1 + k], o[1 + k], gm);
+        gn = fmaf(G[1 + k], n[k], gn);
+        mx2 = fmaf(x[1 + k], x[1 + k], mx2);
+        mxn = fmaf(x[1 + k], n[k], mxn);
+    }
+    const float p = gm1 * fmaf(-0.5f * o[0], uu, o[NV - 1]);
+    const float c2 = gamma * p * ir;
+    const float c = sq(c2);
+    const float ao = fabsf(un) + c;
+    const float irx = rcp(x[0]);
+    const float px = gm1 * fmaf(-0.5f * mx2, irx, x[NV - 1]);
+    const float c2x = gamma * px * irx;
+    const float ax = fabsf(mxn * irx) + sq(c2x);
+    const float lam = fmaxf(ao, ax) * diss;
+    float unb = fmaf(G[NV - 1], o[NV - 1] + p, gm);  // of u_o·n
+    float pb = fmaf(G[NV - 1], un, gn);               // of p_o
+    // λ's dependence on o when the own wave speed is the larger one
+    float dl = (o[0] - x[0]) * G[0];
+#pragma unroll
+    for (int v = 1; v < NV; ++v) dl = fmaf(o[v] - x[v], G[v], dl);
+    const float cb = ao >= ax ? dl * diss : 0.0f;  // of c_o (and of |u_o·n|)
+    unb = fmaf(cb, un >= 0.0f ? 1.0f : -1.0f, unb);
+    const float c2b = 0.5f * cb * rcp(c);
+    pb = fmaf(c2b, gamma * ir, pb);
+    const float pg = gm1 * pb;
+    const float uir = unb * ir;
+    ob[0] += fmaf(-c2b * c2, ir, fmaf(-uir, un, fmaf(0.5f * pg, uu, lam * G[0])));
+#pragma unroll
+    for (int k = 0; k < ND; ++k)
+        ob[1 + k] += fmaf(G[1 + k], un, fmaf(G[0] + uir, n[k], fmaf(-pg, u[k], lam * G[1 + k])));
+    ob[NV - 1] += fmaf(G[NV - 1], un, pg + lam * G[NV - 1]);
+    return c2 >= 0.0f && c2x >= 0.0f;
+}
+
+template <int ND, int N>
+__global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A) {
+    using C = VCfg<ND, N>;
+    constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
+    constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
+    const FastArgs& a = A.f;
+    __shared__ float4 smem[(C::LDS + 3) / 4];
+    float* const sm = reinterpret_cast<float*>(smem);
+    float* const sQ = sm + C::OFF_Q;
+    float* const sA = sm + C::OFF_A;
+    float* const sM = sm + C::OFF_M;
+    float* const sW = sm + C::OFF_W;
+    float* const sF = sm + C::OFF_F;
+
+    const int tid = threadIdx.x;
+    const int e0 = xcd_remap(blockIdx.x, gridDim.x) * EPB;
+    const int64_t Nd = a.ndofs;
+    const float gamma = a.gamma, gm1 = a.gamma - 1.0f, ig1 = a.inv_gm1;
+    const int nel = a.ne - e0 < EPB ? a.ne - e0 : EPB;
+    const uint32_t n0 = (uint32_t)e0 * NPE;
+    int err = 0;
+
+    // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
+    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
+    const int lle = tl / NL, lt = tl - (tl / NL) * NL;
+    const int ldir = lt / NPF, lf = lt - (lt / NPF) * NPF;
+    const bool line_ok = tid < EPB * NL && lle < nel;
+    int xnb[2], xinf[2], xfg[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const int slot = (e0 + lle) * NFC + 2 * ldir + sd;
+        xnb[sd] = line_ok ? a.nb[slot] : -2;
+        xinf[sd] = line_ok ? a.finfo[slot] : 0;
+        xfg[sd] = line_ok ? a.fgi[slot] : 0;
+    }
+
+    // ---- staging: node states, cotangents, per-node metric and J -----------------------
+    const uint32_t blk_bytes = (uint32_t)(nel * NPE) * 4u;
+    float q[NPT][NV], y[NPT][NV], g[NPT][ND * ND + 1];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t rq = make_rsrc(a.Q + v * Nd + n0, blk_bytes);
+        const rsrc_t ry = make_rsrc(A.Y + v * Nd + n0, blk_bytes);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) {
+            q[r][v] = bload(rq, (uint32_t)(tid + r * TPB) * 4u);
+            y[r][v] = bload(ry, (uint32_t)(tid + r * TPB) * 4u);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < ND * ND + 1; ++c) {
+        const rsrc_t rg = make_rsrc(a.ngeom + c * Nd + n0, blk_bytes);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) g[r][c] = bload(rg, (uint32_t)(tid + r * TPB) * 4u);
+    }
+    // face normal and surface Jacobian of both line ends (per face node, master side)
+    float fn[2][ND], fj[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const float4 t = line_ok ? a.fnj[(int64_t)xfg[sd] * NPF + lf] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+        fn[sd][0] = t.x;
+        if (ND > 1) fn[sd][1] = t.y;
+        if (ND > 2) fn[sd][ND - 1] = t.z;
+        fj[sd] = t.w;
+    }
+    // the neighbours' traces, Y and J at both line ends (faces are interior)
+    uint32_t xo[2];
+    bool inblk[2];
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        const int kp = finfo_nbpos(xinf[sd]);
+        inblk[sd] = xnb[sd] >= e0 && xnb[sd] < e0 + nel;
+        xo[sd] = xnb[sd] >= 0 && !inblk[sd]
+                     ? ((uint32_t)xnb[sd] * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
+                     : 0xffffffffu;
+    }
+    float xq[2][NV], xw[2][NV], xj[2];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
+        const rsrc_t ryv = make_rsrc(A.Y + v * Nd, (uint32_t)Nd * 4u);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            xq[sd][v] = bload(rv, xo[sd]);
+            xw[sd][v] = bload(ryv, xo[sd]);
+        }
+    }
+    {
+        const rsrc_t rj = make_rsrc(a.ngeom + ND * ND * Nd, (uint32_t)Nd * 4u);
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) xj[sd] = bload(rj, xo[sd]);
+    }
+    // state, primitives, metric and w to LDS
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int j = tid + r * TPB;
+        if (j >= NN) continue;
+        const float rho = q[r][0];
+        const float ir = rcp(rho);
+        float u[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * ir;
+        float uu = u[ND - 1] * u[ND - 1];
+#pragma unroll
+        for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
+        const float p = gm1 * fmaf(-0.5f * rho, uu, q[r][NV - 1]);
+        const float J = g[r][ND * ND];
+        if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f) || !(J > 0.0f))) err |= ERR_LOG_NEG;
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sQ[c * NN + j] = q[r][c];
+        sA[j] = 0.5f * rho;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) sA[(1 + k) * NN + j] = 0.5f * u[k];
+        sA[(ND + 1) * NN + j] = 0.5f * rho * rcp(p);
+#pragma unroll
+        for (int c = 0; c < ND * ND; ++c) sM[c * NN + j] = 0.5f * g[r][c];
+        const float cj = A.c * rcp(J);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) sW[c * NN + j] = cj * y[r][c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sd = 0; sd < 2; ++sd) {
+        if (inblk[sd]) {
+            const int kp = finfo_nbpos(xinf[sd]);
+            const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                xq[sd][v] = sQ[v * NN + node];
+                xw[sd][v] = sW[v * NN + node];
+            }
+        } else {
+            const float cj = A.c * rcp(xj[sd]);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) xw[sd][v] *= cj;
+        }
+    }
+
+    // ---- line tasks --------------------------------------------------------------------
+    const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
+    const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    float acc[N][NV];
+    {
+        // face terms of both line ends: S̄ = -co·w_own + cn·w_nb with co, cn = ½·jac·∂g of
+        // the own and the neighbour's end
+        bool bad = false;
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd) {
+            const int nd = base + (sd ? (N - 1) * stride : 0);
+            float o[NV], G[NV], ob[NV];
+            const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
+            float nown[ND];
+#pragma unroll
+            for (int k = 0; k < ND; ++k) nown[k] = s * fn[sd][k];
+            const int kp = finfo_nbpos(xinf[sd]);
+            const float co = 0.5f * fj[sd] * (sd ? a.lift_p : a.lift_m);
+            const float cn = 0.5f * fj[sd] * ((kp & 1) ? a.lift_p : a.lift_m);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                o[v] = sQ[v * NN + nd];
+                G[v] = fmaf(cn, xw[sd][v], -co * sW[v * NN + nd]);
+                ob[v] = 0.0f;
+            }
+            bad |= !face_vjp<ND>(o, xq[sd], nown, G, gamma, gm1, a.diss, ob);
+            if (line_ok) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) sF[((lle * NFC + 2 * ldir + sd) * NV + v) * NPF + lf] = ob[v];
+            }
+        }
+        if (bad && line_ok) err |= ERR_SQRT_NEG;
+
+        float amax = 0.0f, amin = 0.0f, bmax = 0.0f, bmin = 0.0f;
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const int nd = base + p * stride;
+            const float ap = sA[nd], bp = sA[(ND + 1) * NN + nd];
+            amax = p ? fmaxf(amax, ap) : ap; amin = p ? fminf(amin, ap) : ap;
+            bmax = p ? fmaxf(bmax, bp) : bp; bmin = p ? fminf(bmin, bp) : bp;
+#pragma unroll
+            for (int v = 0; v < NV; ++v) acc[p][v] = 0.0f;
+        }
+        // the series branch for every pair of the line when the spreads of ρ and β allow
+        // it (u < 0.01 on the line's extreme pair, with a margin), as in the forward
+        const float da = amax - amin, sa = amax + amin, dbm = bmax - bmin, sbm = bmax + bmin;
+        const float* const sMd = sM + ND * ldir * NN;
+        if (da * da * 1.0001f < 1e-2f * sa * sa && dbm * dbm * 1.0001f < 1e-2f * sbm * sbm)
+            line_vjp<ND, N, NN, true>(sA, sMd, sW, base, stride, A.iw, A.wsh, ig1, acc);
+        else
+            line_vjp<ND, N, NN, false>(sA, sMd, sW, base, stride, A.iw, A.wsh, ig1, acc);
+    }
+
+    // epilogue operands (null pointers read as zero)
+    float p0[NPT][NV], p1[NPT][NV], p2[NPT][NV];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        const rsrc_t r0 = make_rsrc(A.P0 ? A.P0 + v * Nd + n0 : a.Q, A.P0 ? blk_bytes : 0u);
+        const rsrc_t r1 = make_rsrc(A.P1 ? A.P1 + v * Nd + n0 : a.Q, A.P1 ? blk_bytes : 0u);
+        const rsrc_t r2 = make_rsrc(A.P2 ? A.P2 + v * Nd + n0 : a.Q, A.P2 ? blk_bytes : 0u);
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) {
+            const uint32_t off = (uint32_t)(tid + r * TPB) * 4u;
+            p0[r][v] = bload(r0, off);
+            p1[r][v] = bload(r1, off);
+            p2[r][v] = bload(r2, off);
+        }
+    }
+    __syncthreads();  // the lines are done: the staging slots are dead
+    if (line_ok) {
+        // rows of direction d in slots [d*NV, d*NV + NV)
+        float* R = sm + ldir * NV * NN;
+#pragma unroll
+        for (int p = 0; p < N; ++p)
+#pragma unroll
+            for (int c = 0; c < NV; ++c) R[c * NN + base + p * stride] = acc[p][c];
+    }
+    __syncthreads();
+
+    // ---- per node: Σ_d rows, pullback to (ρ, m, E), face terms, epilogue ----------------
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+        const int j = tid + r * TPB;
+        const int jj = j < NN ? j : 0;
+        float t[NV];
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            t[v] = sm[v * NN + jj];
+#pragma unroll
+            for (int d = 1; d < ND; ++d) t[v] += sm[(d * NV + v) * NN + jj];
+        }
+        // (a, v, β) = (ρ/2, u/2, ρ/(2p)); u = m/ρ, p = (γ-1)(E - ρ|u|²/2)
+        const float rho = q[r][0], ir = rcp(rho);
+        float u[ND];
+#pragma unroll
+        for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * ir;
+        float uu = u[ND - 1] * u[ND - 1];
+#pragma unroll
+        for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
+        const float p = gm1 * fmaf(-0.5f * rho, uu, q[r][NV - 1]);
+        const float ip = rcp(p);
+        const float rb = 0.5f * fmaf(t[NV - 1], ip, t[0]);       // of ρ at fixed (u, p)
+        const float pg = -gm1 * t[NV - 1] * (0.5f * rho * ip) * ip;  // (γ-1) × cotangent of p
+        float V[NV], ubu = 0.0f;
+#pragma unroll
+        for (int k = 0; k < ND; ++k) {
+            const float ubk = 0.5f * t[1 + k];
+            ubu = fmaf(ubk, u[k], ubu);
+            V[1 + k] = fmaf(ubk, ir, -pg * u[k]);
+        }
+        V[0] = fmaf(-ubu, ir, fmaf(0.5f * pg, uu, rb));
+        V[NV - 1] = pg;
+        // faces of the node's element that it lies on
+        const int le = jj / NPE, node = jj - (jj / NPE) * NPE;
+        int ix[3];
+        ix[0] = node % N;
+        ix[1] = ND > 1 ? (node / N) % N : 0;
+        ix[2] = ND > 2 ? node / (N * N) : 0;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const int pos = ix[d];
+            const int f = ND == 2 ? ix[1 - d] : (d == 0 ? ix[1] + N * ix[2] : (d == 1 ? ix[0] + N * ix[2]
+                                                                                       : ix[0] + N * ix[1]));
+            if (pos == 0 || pos == N - 1) {
+                const float* F = sF + ((le * NFC + 2 * d + (pos == 0 ? 0 : 1)) * NV) * NPF + f;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) V[v] += F[v * NPF];
+            }
+        }
+        const uint32_t off = (uint32_t)j * 4u;
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const float o = fmaf(A.a1, p1[r][v], fmaf(A.a0, p0[r][v], V[v]));
+            const rsrc_t ro1 = make_rsrc(A.o1 ? A.o1 + v * Nd + n0 : a.Q, A.o1 ? blk_bytes : 0u);
+            bstore(ro1, off, fmaf(A.b2, p2[r][v], fmaf(A.b1, p1[r][v], A.b0 * o)));
+            const rsrc_t ro0 = make_rsrc(A.o0 + v * Nd + n0, blk_bytes);
+            bstore(ro0, off, o);
+        }
+    }
+    if (err) atomicOr(a.errflag, err);
+}
+
+template <int ND, int N>
+hipError_t launch_fvjp(const FastVjpArgs& a, hipStream_t s) {
+    using C = VCfg<ND, N>;
+    const int nb = (a.f.ne + C::EPB - 1) / C::EPB;
+    hipLaunchKernelGGL((k_fvjp<ND, N>), dim3(nb), dim3(C::TPB), 0, s, a);
+    return hipGetLastError();
+}
+
+template <int ND>
+FastVjpLauncher fvjp_n(int n) {
+    switch (n) {
+        case 4: return &launch_fvjp<ND, 4>;
+        case 5: return &launch_fvjp<ND, 5>;
+        case 6: return &launch_fvjp<ND, 6>;
+        case 7: return &launch_fvjp<ND, 7>;
+        case 8: return &launch_fvjp<ND, 8>;
+        default: return nullptr;
+    }
+}
+
+}  // namespace fast
+
+FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp) {
+    if (tp != TP_CHANDRASEKHAR) return nullptr;
+    if (nd == 2) return fast::fvjp_n<2>(n);
+    if (nd == 3) return fast::fvjp_n<3>(n);
+    return nullptr;
+}
+
+}  // namespace flouhip

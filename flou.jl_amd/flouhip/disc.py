@@ -143,6 +143,10 @@ class MultielementDisc:
         """True when this (FP32) discretisation runs the fast fused kernel."""
         return C.lib.flou_hip_set_option(self.h, 2, 0) == 1
 
+    def uses_fast_vjp(self):
+        """True when this (FP32) discretisation's adjoint runs the fast fused kernel."""
+        return C.lib.flou_hip_set_option(self.h, 3, 0) == 1
+
     def _set_bcs(self, bcs):
         mesh = self.mesh
         for name, bc in bcs.items():
