@@ -6,10 +6,12 @@
 // epilogue (FastVjpArgs).  The same mathematics as k_vjp (vjp_plain.hip), organised like
 // the forward fast kernel k_fastc (rhs_fast.hip):
 //   * EPB elements per block, one thread per (element, direction, line);
-//   * staging: per node Q, Y, the metric and J from HBM; to LDS the state, the halved
-//     primitives a = ρ/2, v = u/2 and β = ρ/(2p), the metric vectors Ja^d/2 and
-//     w = c·Y/J; each line thread also loads the neighbour's trace and Y/J at its two
-//     line ends;
+//   * staging: per node Q, Y, the metric and J from HBM; to LDS one record of 16-byte
+//     vectors per node with the halved primitives a = ρ/2, v = u/2, β = ρ/(2p) and
+//     w = c·Y/J, and the metric vector Ja^d/2 of each direction (a partner node costs
+//     four 128-bit reads, and eight consecutive lanes fall on distinct banks for lines of
+//     every direction); each line thread also loads the neighbour's trace and Y/J at its
+//     two line ends;
 //   * line phase: the reverse product of the face flux at both line ends with respect to
 //     the own trace (conservative variables, into a face array in LDS), then every pair
 //     of the line once.  Off the (zero) diagonal of D♯ on GLL nodes S_ij = ω_i D♯_ij is
@@ -41,16 +43,16 @@ struct VCfg {
     static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;
     static constexpr int NN = EPB * NPE;
     static constexpr int NPT = (NN + TPB - 1) / TPB;
-    // LDS slot arrays [slot][NN]: Q (NV), the primitives a, v.., β (NV), the metric vectors
-    // Ja^d/2 (ND*ND), w = c·Y/J (NV).  The row sums of the ND directions (ND*NV slots) go
-    // over them once the lines are done.  Then the face cotangents [EPB][2ND][NV][NPF].
-    static constexpr int OFF_Q = 0;
-    static constexpr int OFF_A = NV * NN;
-    static constexpr int OFF_M = 2 * NV * NN;
-    static constexpr int OFF_W = (2 * NV + ND * ND) * NN;
-    static constexpr int OFF_F = (3 * NV + ND * ND) * NN;
+    // LDS: per node NR 16-byte records of (a, v.., β, w..) ([NR][NN] float4), then per
+    // direction the metric vector Ja^d/2 ([ND][NN] of MW floats).  The row sums of the ND
+    // directions ([ND*NV][NN] floats) go over them once the lines are done.  Then the face
+    // cotangents [EPB][2ND][NV][NPF].
+    static constexpr int PW = 2 * NV, NR = (PW + 3) / 4, MW = ND == 3 ? 4 : 2;
+    static constexpr int OFF_R = 0;
+    static constexpr int OFF_M = 4 * NR * NN;
+    static constexpr int OFF_F = OFF_M + ND * MW * NN;
     static constexpr int LDS = r4(OFF_F + EPB * NFC * NV * NPF);
-    static_assert(ND * NV <= 3 * NV + ND * ND, "row sums must fit in the staging slots");
+    static_assert(ND * NV * NN <= OFF_F, "row sums must fit in the staging area");
     static_assert(EPB * NL >= 64, "lanes past the line tasks repeat a line of the block");
 };
 
@@ -147,47 +149,80 @@ __device__ __forceinline__ void pair_vjp(float ai, float aj, const float* vi, co
     }
 }
 
+// Node j's record (a, v.., β, w..) from the NR float4 arrays at sR.
+template <int ND, int NN>
+__device__ __forceinline__ void load_rec(const float4* __restrict__ sR, int j, float (&pw)[VCfg<ND, 4>::NR * 4]) {
+    constexpr int NR = VCfg<ND, 4>::NR;
+#pragma unroll
+    for (int r = 0; r < NR; ++r) {
+        const float4 t = sR[r * NN + j];
+        pw[4 * r] = t.x;
+        pw[4 * r + 1] = t.y;
+        pw[4 * r + 2] = t.z;
+        pw[4 * r + 3] = t.w;
+    }
+}
+
+// Node j's metric vector of one direction (the array of that direction at sMd).
+template <int ND>
+__device__ __forceinline__ void load_metric(const float* __restrict__ sMd, int j, float* M) {
+    if constexpr (ND == 3) {
+        const float4 t = reinterpret_cast<const float4*>(sMd)[j];
+        M[0] = t.x;
+        M[1] = t.y;
+        M[2] = t.z;
+    } else {
+        const float2 t = reinterpret_cast<const float2*>(sMd)[j];
+        M[0] = t.x;
+        M[1] = t.y;
+    }
+}
+
+// The conservative state of a record: ρ = 2a, m = ρ·2v, p = a/β, E = p/(γ-1) + ρ|u|²/2.
+template <int ND>
+__device__ __forceinline__ void rec_state(const float* pw, float ig1, float* Q) {
+    const float rho = 2.0f * pw[0];
+    float uu = 0.0f;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        const float u = 2.0f * pw[1 + k];
+        Q[1 + k] = rho * u;
+        uu = fmaf(u, u, uu);
+    }
+    Q[0] = rho;
+    Q[ND + 1] = fmaf(0.5f * rho, uu, pw[0] * rcp(pw[ND + 1]) * ig1);
+}
+
 // All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
-// j's read from the LDS slot arrays (a, v.., β at sA; the metric vector of the line's
-// direction at sM; w at sW), w scaled by 1/ω of the node's position on the line.
+// j's read from LDS (records at sR, the metric vector of the line's direction at sMd), w
+// scaled by 1/ω of the node's position on the line.
 template <int ND, int N, int NN, bool SER>
-__device__ __forceinline__ void line_vjp(const float* __restrict__ sA, const float* __restrict__ sM,
-                                         const float* __restrict__ sW, int base, int stride,
-                                         const float* __restrict__ iw, const float* __restrict__ wsh, float ig1,
-                                         float (&acc)[N][ND + 2]) {
-    constexpr int NV = ND + 2;
+__device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const float* __restrict__ sMd, int base,
+                                         int stride, const float* __restrict__ iw, const float* __restrict__ wsh,
+                                         float ig1, float (&acc)[N][ND + 2]) {
+    constexpr int NV = ND + 2, NP = VCfg<ND, 4>::NR * 4;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const int ni = base + i * stride;
-        float ai, vi[ND], bi, Mi[ND], wi[NV];
-        ai = sA[ni];
-#pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            vi[k] = sA[(1 + k) * NN + ni];
-            Mi[k] = sM[k * NN + ni];
-        }
-        bi = sA[(ND + 1) * NN + ni];
+        float pi[NP], Mi[ND], wi[NV];
+        load_rec<ND, NN>(sR, ni, pi);
+        load_metric<ND>(sMd, ni, Mi);
         const float iwi = iw[i];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) wi[v] = sW[v * NN + ni] * iwi;
-        const float lai = SER ? 0.0f : lg(ai), lbi = SER ? 0.0f : lg(bi);
+        for (int v = 0; v < NV; ++v) wi[v] = pi[NV + v] * iwi;
+        const float lai = SER ? 0.0f : lg(pi[0]), lbi = SER ? 0.0f : lg(pi[ND + 1]);
 #pragma unroll
         for (int j = i + 1; j < N; ++j) {
             const int nj = base + j * stride;
-            float aj, vj[ND], bj, Mj[ND], G[NV];
-            aj = sA[nj];
-#pragma unroll
-            for (int k = 0; k < ND; ++k) {
-                vj[k] = sA[(1 + k) * NN + nj];
-                Mj[k] = sM[k * NN + nj];
-            }
-            bj = sA[(ND + 1) * NN + nj];
+            float pj[NP], Mj[ND], G[NV];
+            load_rec<ND, NN>(sR, nj, pj);
+            load_metric<ND>(sMd, nj, Mj);
             const float iwj = iw[j];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) G[v] = fmaf(sW[v * NN + nj], iwj, -wi[v]);
-            const float laj = SER ? 0.0f : lg(aj), lbj = SER ? 0.0f : lg(bj);
-            pair_vjp<ND, SER>(ai, aj, vi, vj, bi, bj, lai, laj, lbi, lbj, Mi, Mj, G, wsh[i * N + j], ig1, acc[i],
-                              acc[j]);
+            for (int v = 0; v < NV; ++v) G[v] = fmaf(pj[NV + v], iwj, -wi[v]);
+            const float laj = SER ? 0.0f : lg(pj[0]), lbj = SER ? 0.0f : lg(pj[ND + 1]);
+            pair_vjp<ND, SER>(pi[0], pj[0], pi + 1, pj + 1, pi[ND + 1], pj[ND + 1], lai, laj, lbi, lbj, Mi, Mj, G,
+                              wsh[i * N + j], ig1, acc[i], acc[j]);
             // keep each pair's loads and arithmetic together: the registers then hold the
             // line's rows, node i and one pair (the scheduler would otherwise hoist the
             // loads of every pair and spill)
@@ -257,10 +292,9 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
     const FastArgs& a = A.f;
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
-    float* const sQ = sm + C::OFF_Q;
-    float* const sA = sm + C::OFF_A;
+    constexpr int NR = C::NR, NP = 4 * NR, MW = C::MW;
+    float4* const sR = reinterpret_cast<float4*>(sm + C::OFF_R);
     float* const sM = sm + C::OFF_M;
-    float* const sW = sm + C::OFF_W;
     float* const sF = sm + C::OFF_F;
 
     const int tid = threadIdx.x;
@@ -357,17 +391,28 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
         const float p = gm1 * fmaf(-0.5f * rho, uu, q[r][NV - 1]);
         const float J = g[r][ND * ND];
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f) || !(J > 0.0f))) err |= ERR_LOG_NEG;
-#pragma unroll
-        for (int c = 0; c < NV; ++c) sQ[c * NN + j] = q[r][c];
-        sA[j] = 0.5f * rho;
-#pragma unroll
-        for (int k = 0; k < ND; ++k) sA[(1 + k) * NN + j] = 0.5f * u[k];
-        sA[(ND + 1) * NN + j] = 0.5f * rho * rcp(p);
-#pragma unroll
-        for (int c = 0; c < ND * ND; ++c) sM[c * NN + j] = 0.5f * g[r][c];
         const float cj = A.c * rcp(J);
+        float pw[NP];
+        pw[0] = 0.5f * rho;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) sW[c * NN + j] = cj * y[r][c];
+        for (int k = 0; k < ND; ++k) pw[1 + k] = 0.5f * u[k];
+        pw[ND + 1] = 0.5f * rho * rcp(p);
+#pragma unroll
+        for (int c = 0; c < NV; ++c) pw[NV + c] = cj * y[r][c];
+#pragma unroll
+        for (int c = 2 * NV; c < NP; ++c) pw[c] = 0.0f;
+#pragma unroll
+        for (int rr = 0; rr < NR; ++rr)
+            sR[rr * NN + j] = make_float4(pw[4 * rr], pw[4 * rr + 1], pw[4 * rr + 2], pw[4 * rr + 3]);
+        // metric: column d of the node's matrix is Ja^d; stored halved per direction
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            if constexpr (ND == 3)
+                reinterpret_cast<float4*>(sM + d * MW * NN)[j] =
+                    make_float4(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1], 0.5f * g[r][ND * d + 2], 0.0f);
+            else
+                reinterpret_cast<float2*>(sM + d * MW * NN)[j] = make_float2(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1]);
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -375,11 +420,11 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
         if (inblk[sd]) {
             const int kp = finfo_nbpos(xinf[sd]);
             const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
+            float pw[NP];
+            load_rec<ND, NN>(sR, node, pw);
+            rec_state<ND>(pw, ig1, xq[sd]);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) {
-                xq[sd][v] = sQ[v * NN + node];
-                xw[sd][v] = sW[v * NN + node];
-            }
+            for (int v = 0; v < NV; ++v) xw[sd][v] = pw[NV + v];
         } else {
             const float cj = A.c * rcp(xj[sd]);
 #pragma unroll
@@ -398,7 +443,9 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) {
             const int nd = base + (sd ? (N - 1) * stride : 0);
-            float o[NV], G[NV], ob[NV];
+            float o[NV], G[NV], ob[NV], pw[NP];
+            load_rec<ND, NN>(sR, nd, pw);
+            rec_state<ND>(pw, ig1, o);
             const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
             float nown[ND];
 #pragma unroll
@@ -408,8 +455,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
             const float cn = 0.5f * fj[sd] * ((kp & 1) ? a.lift_p : a.lift_m);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                o[v] = sQ[v * NN + nd];
-                G[v] = fmaf(cn, xw[sd][v], -co * sW[v * NN + nd]);
+                G[v] = fmaf(cn, xw[sd][v], -co * pw[NV + v]);
                 ob[v] = 0.0f;
             }
             bad |= !face_vjp<ND>(o, xq[sd], nown, G, gamma, gm1, a.diss, ob);
@@ -424,7 +470,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
 #pragma unroll
         for (int p = 0; p < N; ++p) {
             const int nd = base + p * stride;
-            const float ap = sA[nd], bp = sA[(ND + 1) * NN + nd];
+            const float ap = sR[nd].x, bp = sm[C::OFF_R + 4 * (((ND + 1) / 4) * NN + nd) + (ND + 1) % 4];
             amax = p ? fmaxf(amax, ap) : ap; amin = p ? fminf(amin, ap) : ap;
             bmax = p ? fmaxf(bmax, bp) : bp; bmin = p ? fminf(bmin, bp) : bp;
 #pragma unroll
@@ -433,11 +479,11 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
         // the series branch for every pair of the line when the spreads of ρ and β allow
         // it (u < 0.01 on the line's extreme pair, with a margin), as in the forward
         const float da = amax - amin, sa = amax + amin, dbm = bmax - bmin, sbm = bmax + bmin;
-        const float* const sMd = sM + ND * ldir * NN;
+        const float* const sMd = sM + ldir * MW * NN;
         if (da * da * 1.0001f < 1e-2f * sa * sa && dbm * dbm * 1.0001f < 1e-2f * sbm * sbm)
-            line_vjp<ND, N, NN, true>(sA, sMd, sW, base, stride, A.iw, A.wsh, ig1, acc);
+            line_vjp<ND, N, NN, true>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
         else
-            line_vjp<ND, N, NN, false>(sA, sMd, sW, base, stride, A.iw, A.wsh, ig1, acc);
+            line_vjp<ND, N, NN, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
     }
 
     // epilogue operands (null pointers read as zero)
