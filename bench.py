@@ -344,6 +344,102 @@ def secondary_workload(name, steps=None, warmup=2):
     return out
 
 
+def c5_adjoint(nsteps=100, every=10):
+    """C5 adjoint: the curved 32^3, N=6 mesh and state of the C5 leg, loss Σ Jω (ρ(T) - 1)²
+    after 100 SSPRK33 steps (CFL 0.1 of the FP64 state), gradient w.r.t. the initial state
+    with FP64 checkpoints every 10 steps (the last segment's stage inputs kept; FP32 also
+    with every stage input kept).  FP32 (fast forward and adjoint kernels) and FP64; per
+    precision the forward and gradient times (HIP events, after one untimed gradient) and
+    their ratio, and the FP32 gradient's relative L2 error against the FP64 one."""
+    import numpy as np
+    import torch
+    import flouhip as F
+    from flouhip import adjoint as A
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    b = F.LagrangeBasis("GLL", 7)
+    std = F.StdHex(b, F.DGSEMrec(b), 5)
+    eq = F.EulerEquation(3, GAMMA)
+
+    def mapping(X):
+        sn = np.ones((X.shape[0], 1))
+        for k in range(3):
+            sn = sn * np.sin(2 * np.pi * X[:, [k]])
+        return X + 0.05 * sn
+    out, grads, dt, Q64 = {}, {}, None, None
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for form in ("fp64", "fp32"):
+        mesh = F.CartesianMesh(3, (0.0,) * 3, (1.0,) * 3, (32,) * 3)
+        mesh.apply_periodicBCs(("1", "2"), ("3", "4"), ("5", "6"))
+        mesh.apply_mapping(mapping)
+        disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
+        if Q64 is None:
+            X = torch.tensor(mesh.node_coords(std), device="cuda").T
+            g = torch.Generator(device="cpu")
+            g.manual_seed(7)
+            ph = (torch.rand(5, 3, generator=g) * 6.283).tolist()
+            modes = [torch.sin(2 * math.pi * X[0] + ph[v][0]) * torch.sin(2 * math.pi * X[1] + ph[v][1])
+                     * torch.sin(2 * math.pi * X[2] + ph[v][2]) for v in range(5)]
+            prim = torch.stack([1 + 0.1 * modes[0], 0.17 * modes[1], 0.17 * modes[2], 0.17 * modes[3],
+                                1 + 0.1 * modes[4]])
+            Q64 = torch.empty_like(prim)
+            Q64[0] = prim[0]
+            Q64[1:4] = prim[0] * prim[1:4]
+            Q64[4] = prim[4] / (GAMMA - 1) + prim[0] * (prim[1:4] ** 2).sum(0) / 2
+            del X, prim, modes
+            dt = F.get_max_dt(Q64, disc, eq, 0.1)
+        Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()
+        wq = (torch.tensor(disc.elem_jac, device="cuda")
+              * torch.tensor(np.tile(std.ω, disc.nelements), device="cuda")).to(disc.torch_dtype())
+
+        def seed(u):
+            gr = torch.zeros_like(u)
+            gr[0] = 2.0 * wq * (u[0] - 1.0)
+            return gr
+
+        def grad(keep=1):
+            return A.gradient(disc, F.SSPRK33, Q, dt, nsteps, seed, checkpoint_every=every,
+                              checkpoint_dtype=torch.float64, keep_segments=keep)
+        grad()  # warm-up (allocator cache, first launches)
+        torch.cuda.synchronize()
+        ev[0].record(stream)
+        A.forward(disc, F.SSPRK33, Q, dt, nsteps)
+        ev[1].record(stream)
+        gq = grad()
+        ev[2].record(stream)
+        torch.cuda.synchronize()
+        disc.check()
+        fwd, gms = ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2])
+        out[form] = {"forward_ms": fwd, "gradient_ms": gms, "gradient_over_forward": gms / fwd,
+                     "fast_adjoint_kernel": disc.uses_fast_vjp()}
+        if form == "fp32":
+            # every stage input kept in HBM (3 x 100 states of 225 MB): no recomputation
+            nseg = -(-nsteps // every)
+            grad(nseg)
+            torch.cuda.synchronize()
+            ev[2].record(stream)
+            gk = grad(nseg)
+            ev[3].record(stream)
+            torch.cuda.synchronize()
+            disc.check()
+            gka = ev[2].elapsed_time(ev[3])
+            out[form]["all_stages_kept"] = {"gradient_ms": gka, "gradient_over_forward": gka / fwd,
+                                            "max_abs_diff_vs_recomputed": float((gk - gq).abs().max())}
+            del gk
+            torch.cuda.empty_cache()
+        grads[form] = gq.to(torch.float64)
+        del disc, Q, wq, gq
+        torch.cuda.empty_cache()
+    out["fp32_gradient_rel_l2_err_vs_fp64"] = float(torch.linalg.norm(grads["fp32"] - grads["fp64"])
+                                                  / torch.linalg.norm(grads["fp64"]))
+    out["fp32_gradient_speedup_vs_fp64"] = out["fp64"]["gradient_ms"] / out["fp32"]["gradient_ms"]
+    out["workload"] = ("C5 curved 32^3 N=6 (the c5 leg's mesh and state), loss sum Jw (rho(T) - 1)^2 after "
+                       "100 SSPRK33 steps, gradient with FP64 checkpoints every 10 steps (last segment kept; "
+                       "FP32 also with every stage input kept); FP32: fast forward, recomputation and adjoint "
+                       "kernels, FP64: plain kernels")
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -393,7 +489,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj"], default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
     ap.add_argument("--no-secondary", action="store_true", help="skip the C2 and C5 workloads")
@@ -412,6 +508,9 @@ def main():
     torch.cuda.set_device(0)
     if args.only == "adjoint":  # profiling run of the adjoint workload alone
         print(json.dumps({"adjoint": adjoint_ratio()}), flush=True)
+        return
+    if args.only == "c5adj":  # the C5 adjoint alone
+        print(json.dumps({"c5_adjoint": c5_adjoint()}), flush=True)
         return
     if args.only in ("c2", "c5"):  # profiling run of one secondary workload alone
         print(json.dumps({args.only: secondary_workload(args.only, args.steps, args.warmup)}), flush=True)
@@ -509,6 +608,12 @@ def main():
         adj = adjoint_ratio()
         out["adjoint_forward_ratio"] = adj["value"]
         out["adjoint"] = adj
+        if not args.no_secondary:
+            out["c5_adjoint"] = c5a = c5_adjoint()
+            print(f"# c5 adjoint: fp32 gradient/forward {c5a['fp32']['gradient_over_forward']:.2f} "
+                  f"(all stages kept {c5a['fp32']['all_stages_kept']['gradient_over_forward']:.2f}), "
+                  f"fp64 {c5a['fp64']['gradient_over_forward']:.2f}, fp32 error "
+                  f"{c5a['fp32_gradient_rel_l2_err_vs_fp64']:.2e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -10,8 +10,8 @@
 //     vectors per node with the halved primitives a = ρ/2, v = u/2, β = ρ/(2p) and
 //     w = c·Y/J, and the metric vector Ja^d/2 of each direction (a partner node costs
 //     four 128-bit reads, and eight consecutive lanes fall on distinct banks for lines of
-//     every direction); each line thread also loads the neighbour's trace and Y/J at its
-//     two line ends;
+//     every direction); each line thread also loads the own and the neighbour's trace
+//     and the neighbour's Y/J at its two line ends;
 //   * line phase: the reverse product of the face flux at both line ends with respect to
 //     the own trace (conservative variables, into a face array in LDS), then every pair
 //     of the line once.  Off the (zero) diagonal of D♯ on GLL nodes S_ij = ω_i D♯_ij is
@@ -58,8 +58,12 @@ struct VCfg {
 
 // The log-mean factor F(u) = atanh(f)/f at u = f², f = (x_i - x_j)/(x_i + x_j), and
 // dF/du: the reference's series 1 + u/3 + u²/5 + u³/7 for u < 0.01, else
-// ln(x_i/x_j)/(2f) (Utilities.jl:34-44) with dF/du = (1/(1-u) - F)/(2u).  SER: the caller
-// knows that u < 0.01.  lxi, lxj: ln x_i, ln x_j (unused on the series branch).
+// ln(x_i/x_j)/(2f) (Utilities.jl:34-44).  On the log branch dF/du = (1/(1-u) - F)/(2u)
+// cancels for small u (at u = 0.01 the bracket is 0.0067, so F's error, about 1e-6 from
+// the difference of two FP32 logarithms, grows 150-fold), so for 0.01 <= u < 0.25 both
+// come from the series Σ u^k/(2k+1) to k = 11 and its derivative (FP32 errors 6e-8 and
+// 1e-7), above that from the log form.  SER: the caller knows that u < 0.01.  lxi, lxj:
+// ln x_i, ln x_j (unused on the series branch).
 template <bool SER>
 __device__ __forceinline__ void lm_factor(float u, float f, float lxi, float lxj, float& F, float& dF) {
     const float Fs = fmaf(u, fmaf(u, fmaf(u, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
@@ -68,13 +72,18 @@ __device__ __forceinline__ void lm_factor(float u, float f, float lxi, float lxj
         F = Fs;
         dF = dFs;
     } else {
-        // both branches evaluated and selected (no divergent control flow in the unrolled
-        // pair block); the log form is not used where u is small or zero
+        // every form evaluated and selected (no divergent control flow in the unrolled pair
+        // block); the log form is not used where u is small or zero
+        constexpr int K = 12;
+        float Fm = 1.0f / (2 * K + 1), dFm = (float)K / (2 * K + 1);
+#pragma unroll
+        for (int k = K - 1; k >= 0; --k) Fm = fmaf(Fm, u, 1.0f / (2 * k + 1));
+#pragma unroll
+        for (int k = K - 1; k >= 1; --k) dFm = fmaf(dFm, u, (float)k / (2 * k + 1));
         const float Fl = 0.5f * (lxi - lxj) * rcp(f);
         const float dFl = 0.5f * (rcp(1.0f - u) - Fl) * rcp(u);
-        const bool ser = u < 0.01f;
-        F = ser ? Fs : Fl;
-        dF = ser ? dFs : dFl;
+        F = u < 0.01f ? Fs : (u < 0.25f ? Fm : Fl);
+        dF = u < 0.01f ? dFs : (u < 0.25f ? dFm : dFl);
     }
 }
 
@@ -176,21 +185,6 @@ __device__ __forceinline__ void load_metric(const float* __restrict__ sMd, int j
         M[0] = t.x;
         M[1] = t.y;
     }
-}
-
-// The conservative state of a record: ρ = 2a, m = ρ·2v, p = a/β, E = p/(γ-1) + ρ|u|²/2.
-template <int ND>
-__device__ __forceinline__ void rec_state(const float* pw, float ig1, float* Q) {
-    const float rho = 2.0f * pw[0];
-    float uu = 0.0f;
-#pragma unroll
-    for (int k = 0; k < ND; ++k) {
-        const float u = 2.0f * pw[1 + k];
-        Q[1 + k] = rho * u;
-        uu = fmaf(u, u, uu);
-    }
-    Q[0] = rho;
-    Q[ND + 1] = fmaf(0.5f * rho, uu, pw[0] * rcp(pw[ND + 1]) * ig1);
 }
 
 // All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
@@ -348,26 +342,29 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
         if (ND > 2) fn[sd][ND - 1] = t.z;
         fj[sd] = t.w;
     }
-    // the neighbours' traces, Y and J at both line ends (faces are interior)
+    // both traces at both line ends, exactly as the forward reads them (the own end node
+    // and the neighbour's, also inside the block: the LxF wave-speed maximum takes its
+    // branch from these values), and the neighbour's Y and J (faces are interior)
+    const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
+    const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
     uint32_t xo[2];
-    bool inblk[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
         const int kp = finfo_nbpos(xinf[sd]);
-        inblk[sd] = xnb[sd] >= e0 && xnb[sd] < e0 + nel;
-        xo[sd] = xnb[sd] >= 0 && !inblk[sd]
-                     ? ((uint32_t)xnb[sd] * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
-                     : 0xffffffffu;
+        xo[sd] = xnb[sd] >= 0 ? ((uint32_t)xnb[sd] * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0)) * 4u
+                              : 0xffffffffu;
     }
-    float xq[2][NV], xw[2][NV], xj[2];
+    float oq[2][NV], xq[2][NV], xw[2][NV], xj[2];
 #pragma unroll
     for (int v = 0; v < NV; ++v) {
         const rsrc_t rv = make_rsrc(a.Q + v * Nd, (uint32_t)Nd * 4u);
         const rsrc_t ryv = make_rsrc(A.Y + v * Nd, (uint32_t)Nd * 4u);
+        const rsrc_t rq = make_rsrc(a.Q + v * Nd + n0, blk_bytes);
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) {
             xq[sd][v] = bload(rv, xo[sd]);
             xw[sd][v] = bload(ryv, xo[sd]);
+            oq[sd][v] = bload(rq, (uint32_t)(base + (sd ? (N - 1) * stride : 0)) * 4u);
         }
     }
     {
@@ -417,24 +414,12 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
     __syncthreads();
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
-        if (inblk[sd]) {
-            const int kp = finfo_nbpos(xinf[sd]);
-            const int node = (xnb[sd] - e0) * NPE + line_node<ND, N>(kp >> 1, lf, (kp & 1) ? N - 1 : 0);
-            float pw[NP];
-            load_rec<ND, NN>(sR, node, pw);
-            rec_state<ND>(pw, ig1, xq[sd]);
+        const float cj = A.c * rcp(xj[sd]);
 #pragma unroll
-            for (int v = 0; v < NV; ++v) xw[sd][v] = pw[NV + v];
-        } else {
-            const float cj = A.c * rcp(xj[sd]);
-#pragma unroll
-            for (int v = 0; v < NV; ++v) xw[sd][v] *= cj;
-        }
+        for (int v = 0; v < NV; ++v) xw[sd][v] *= cj;
     }
 
     // ---- line tasks --------------------------------------------------------------------
-    const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
-    const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
     float acc[N][NV];
     {
         // face terms of both line ends: S̄ = -co·w_own + cn·w_nb with co, cn = ½·jac·∂g of
@@ -443,9 +428,8 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) {
             const int nd = base + (sd ? (N - 1) * stride : 0);
-            float o[NV], G[NV], ob[NV], pw[NP];
+            float G[NV], ob[NV], pw[NP];
             load_rec<ND, NN>(sR, nd, pw);
-            rec_state<ND>(pw, ig1, o);
             const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
             float nown[ND];
 #pragma unroll
@@ -458,7 +442,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
                 G[v] = fmaf(cn, xw[sd][v], -co * pw[NV + v]);
                 ob[v] = 0.0f;
             }
-            bad |= !face_vjp<ND>(o, xq[sd], nown, G, gamma, gm1, a.diss, ob);
+            bad |= !face_vjp<ND>(oq[sd], xq[sd], nown, G, gamma, gm1, a.diss, ob);
             if (line_ok) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) sF[((lle * NFC + 2 * ldir + sd) * NV + v) * NPF + lf] = ob[v];

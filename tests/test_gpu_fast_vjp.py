@@ -8,7 +8,8 @@ Interfaces.jl:111-136, Equations/Euler.jl:474-536):
   the log-mean;
 * the dot-product identity <W, J V> = <J^T W, V> against the oracle's FP64 dual-number JVP
   (the oracle's rhs is pinned by tests/test_oracle.py and tests/test_curved.py);
-* the reverse RK steps (SSPRK33 and CK2N54) against the plain FP32 path.
+* the reverse RK steps (SSPRK33 and CK2N54) against the FP64 adjoint at the same stage
+  states, next to the plain FP32 path.
 The hand-derived reverse formulas were also checked against central differences of the
 forward fluxes in float64 (series and log branches, the LxF face flux) before the kernel
 ran.
@@ -53,6 +54,14 @@ def _rel(a, b):
     return np.max(np.abs(a - b) / np.max(np.abs(b), axis=0))
 
 
+def _rel2(a, b):
+    """Relative L2 error: the measure for FP32 adjoints, where the LxF wave-speed maximum
+    has a kink (λ = max of the two sides' |u·n| + c): at a face node where the two nearly
+    tie, two FP32 evaluations may take different branches, and one node's cotangent then
+    differs by the jump of the derivative."""
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
 @pytest.mark.parametrize("amp", [0.1, 0.6], ids=["series", "log"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: "{}d-n{}-{}".format(*c))
 def test_fast_vjp_matches_plain_fp32_vjp(case, amp):
@@ -61,9 +70,9 @@ def test_fast_vjp_matches_plain_fp32_vjp(case, amp):
     Q = periodic_euler_state(X, nd, 1.0, seed=5, amp=amp)
     W = np.random.default_rng(6).standard_normal(Q.shape)
     got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
-    err = _rel(got, ref)
-    print(f"{case} amp {amp}: fast vs plain FP32 VJP, max rel err {err:.2e}")
-    assert err <= 3e-5
+    err, emax = _rel2(got, ref), _rel(got, ref)
+    print(f"{case} amp {amp}: fast vs plain FP32 VJP, relative L2 {err:.2e}, max {emax:.2e}")
+    assert err <= 2e-6
 
 
 @pytest.mark.parametrize("case", [(3, 7, 2), (3, 5, 2), (2, 6, 4)], ids=["3d-n7", "3d-n5", "2d-n6"])
@@ -82,23 +91,33 @@ def test_fast_vjp_dot_product_with_oracle_jvp(case):
 
 
 @pytest.mark.parametrize("solver", [F.SSPRK33, F.CarpenterKennedy2N54], ids=["ssprk33", "ck2n54"])
-def test_fast_vjp_reverse_rk_step_matches_plain(solver):
+def test_fast_vjp_reverse_rk_step_against_fp64(solver):
+    """One reverse RK step at stages recorded in FP32: the fast and the plain FP32 adjoint
+    against the FP64 adjoint at the same (FP32-rounded) stage states, relative L2 (see
+    _rel2).  FP32 round-off alone gives about 1e-7 (the plain kernel's figure); the bound
+    leaves room for a face node whose LxF wave-speed maximum takes the other branch (on the
+    CK2N54 case one node of stage 2 has wave speeds 1.18949268 and 1.18949280, within an
+    FP32 ulp: about 1e-6 relative L2)."""
     disc, orc, X = _fp32_disc(3, 6, 3)
+    d64 = build_curved(3, 6, 3, "split", "ch", precision="fp64")[0]
     Q = periodic_euler_state(X, 3, 1.0, seed=12, amp=0.15)
     dt = 0.2 * orc.max_dt(Q, 1.0)
     S = 3 if solver is F.SSPRK33 else 5
     stages = [disc.to_device(Q)] + [disc.new_state() for _ in range(S - 1)]
-    u_next = disc.new_state()
-    A.rk_step_record(disc, solver, dt, stages, u_next)
+    A.rk_step_record(disc, solver, dt, stages, disc.new_state())
+    st64 = [d64.to_device(disc.to_host(x).astype(np.float64)) for x in stages]
     ybar = np.random.default_rng(2).standard_normal(Q.shape)
-    out = {}
+    ref = d64.to_host(A.rk_step_vjp(d64, solver, dt, st64, d64.to_device(ybar)))
+    d64.check()
+    err = {}
     for plain in (0, 1):
         disc.set_option(1, plain)
         ub = disc.to_device(ybar)
         A.rk_step_vjp(disc, solver, dt, stages, ub)
         disc.check()
-        out[plain] = disc.to_host(ub).astype(np.float64)
+        got = disc.to_host(ub).astype(np.float64)
+        err[plain] = (_rel2(got, ref), _rel(got, ref))
     disc.set_option(1, 0)
-    err = _rel(out[0], out[1])
-    print(f"{solver.__name__ if hasattr(solver, '__name__') else solver}: fast vs plain reverse step {err:.2e}")
-    assert err <= 3e-5
+    print(f"{'SSPRK33' if solver is F.SSPRK33 else 'CK2N54'} reverse step vs FP64, relative L2 (max): "
+          f"fast {err[0][0]:.2e} ({err[0][1]:.2e}), plain FP32 {err[1][0]:.2e} ({err[1][1]:.2e})")
+    assert err[0][0] <= 2e-6 and err[1][0] <= 2e-6
