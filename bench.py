@@ -344,17 +344,13 @@ def secondary_workload(name, steps=None, warmup=2):
     return out
 
 
-def c5_adjoint(nsteps=100, every=10):
-    """C5 adjoint: the curved 32^3, N=6 mesh and state of the C5 leg, loss Σ Jω (ρ(T) - 1)²
-    after 100 SSPRK33 steps (CFL 0.1 of the FP64 state), gradient w.r.t. the initial state
-    with FP64 checkpoints every 10 steps (the last segment's stage inputs kept; FP32 also
-    with every stage input kept).  FP32 (fast forward and adjoint kernels) and FP64; per
-    precision the forward and gradient times (HIP events, after one untimed gradient) and
-    their ratio, and the FP32 gradient's relative L2 error against the FP64 one."""
+def _c5_case():
+    """The C5 leg's discretisation parts and a function building (disc, state) per precision:
+    the curved periodic 32^3 mesh x + 0.05 sin 2πx sin 2πy sin 2πz, N=6, split Chandrasekhar
+    + LxF(StdAverage), and its seeded smooth FP64 state (built once)."""
     import numpy as np
     import torch
     import flouhip as F
-    from flouhip import adjoint as A
     op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
     b = F.LagrangeBasis("GLL", 7)
     std = F.StdHex(b, F.DGSEMrec(b), 5)
@@ -365,15 +361,14 @@ def c5_adjoint(nsteps=100, every=10):
         for k in range(3):
             sn = sn * np.sin(2 * np.pi * X[:, [k]])
         return X + 0.05 * sn
-    out, grads, dt, Q64 = {}, {}, None, None
-    stream = torch.cuda.current_stream()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    for form in ("fp64", "fp32"):
+    state = {}
+
+    def build(form):
         mesh = F.CartesianMesh(3, (0.0,) * 3, (1.0,) * 3, (32,) * 3)
         mesh.apply_periodicBCs(("1", "2"), ("3", "4"), ("5", "6"))
         mesh.apply_mapping(mapping)
         disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
-        if Q64 is None:
+        if "Q64" not in state:
             X = torch.tensor(mesh.node_coords(std), device="cuda").T
             g = torch.Generator(device="cpu")
             g.manual_seed(7)
@@ -386,7 +381,72 @@ def c5_adjoint(nsteps=100, every=10):
             Q64[0] = prim[0]
             Q64[1:4] = prim[0] * prim[1:4]
             Q64[4] = prim[4] / (GAMMA - 1) + prim[0] * (prim[1:4] ** 2).sum(0) / 2
-            del X, prim, modes
+            state["Q64"] = Q64
+        return disc, state["Q64"]
+    return std, eq, build
+
+
+def c5_vjp_kernel(reps=20):
+    """The C5 FP32 rhs VJP alone (profiling run of the fast adjoint kernel): reps calls of
+    flou_hip_rhs_vjp at the C5 state with a seeded cotangent, and reps fast rhs evaluations
+    at the same state, timed with HIP events on the handle's stream.  Algorithmic bytes of
+    one VJP call per node: Q, Y and the accumulated Qbar read (3 x 20 B), the metric and J
+    (40 B), Qbar written (20 B)."""
+    import torch
+    import flouhip as F
+    from flouhip import adjoint as A
+    std, eq, build = _c5_case()
+    disc, Q64 = build("fp32")
+    assert disc.uses_fast_vjp(), "the C5 FP32 adjoint should run the fast kernel"
+    Q = Q64.to(torch.float32).contiguous()
+    g = torch.Generator(device="cpu")
+    g.manual_seed(11)
+    Y = torch.randn(Q.shape, generator=g).to(device="cuda", dtype=torch.float32)
+    Qbar = torch.zeros_like(Q)
+    dQ = torch.empty_like(Q)
+    cfg = F.EquationConfig(disc, eq)
+    for _ in range(3):
+        A.rhs_vjp(Qbar, Y, Q, disc)
+        F.rhs_(dQ, Q, cfg)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        A.rhs_vjp(Qbar, Y, Q, disc)
+    ev[1].record(stream)
+    for _ in range(reps):
+        F.rhs_(dQ, Q, cfg)
+    ev[2].record(stream)
+    torch.cuda.synchronize()
+    disc.check()
+    vms, fms = ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
+    nn = Q.shape[1]
+    vbytes = nn * (3 * 20 + 40 + 20)
+    return {"vjp_ms": vms, "rhs_ms": fms, "vjp_over_rhs": vms / fms, "nodes": nn,
+            "vjp_algorithmic_bytes": vbytes, "vjp_hbm_fraction": vbytes / (vms * 1e-3) / 8e12,
+            "workload": "C5 curved 32^3 N=6 FP32 (the c5 leg's mesh and state), seeded cotangent; "
+                        "fast adjoint kernel vs fast rhs"}
+
+
+def c5_adjoint(nsteps=100, every=10):
+    """C5 adjoint: the curved 32^3, N=6 mesh and state of the C5 leg, loss Σ Jω (ρ(T) - 1)²
+    after 100 SSPRK33 steps (CFL 0.1 of the FP64 state), gradient w.r.t. the initial state
+    with FP64 checkpoints every 10 steps (the last segment's stage inputs kept; FP32 also
+    with every stage input kept).  FP32 (fast forward and adjoint kernels) and FP64; per
+    precision the forward and gradient times (HIP events, after one untimed gradient) and
+    their ratio, and the FP32 gradient's relative L2 error against the FP64 one."""
+    import numpy as np
+    import torch
+    import flouhip as F
+    from flouhip import adjoint as A
+    std, eq, build = _c5_case()
+    out, grads, dt = {}, {}, None
+    stream = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for form in ("fp64", "fp32"):
+        disc, Q64 = build(form)
+        if dt is None:
             dt = F.get_max_dt(Q64, disc, eq, 0.1)
         Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()
         wq = (torch.tensor(disc.elem_jac, device="cuda")
@@ -489,7 +549,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp"], default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
     ap.add_argument("--no-secondary", action="store_true", help="skip the C2 and C5 workloads")
@@ -511,6 +571,9 @@ def main():
         return
     if args.only == "c5adj":  # the C5 adjoint alone
         print(json.dumps({"c5_adjoint": c5_adjoint()}), flush=True)
+        return
+    if args.only == "c5vjp":  # profiling run of the C5 FP32 adjoint kernel alone
+        print(json.dumps({"c5_vjp": c5_vjp_kernel()}), flush=True)
         return
     if args.only in ("c2", "c5"):  # profiling run of one secondary workload alone
         print(json.dumps({args.only: secondary_workload(args.only, args.steps, args.warmup)}), flush=True)

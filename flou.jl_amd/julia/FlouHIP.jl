@@ -28,7 +28,7 @@ using Flou
 using Flou: FlouCommon, FlouSpatial, FlouTime, FlouBiz
 
 export HIPDisc, HIPState, KennedyGruber
-export device, host, monitor, rhs, integrate_steps, gpu_limiter_callback
+export device, host, monitor, rhs, integrate_steps, gpu_limiter_callback, uses_fast_kernel, uses_fast_vjp
 
 const libflouhip = get(ENV, "FLOUHIP_LIB", normpath(joinpath(@__DIR__, "..", "libflouhip.so")))
 const libhip = "libamdhip64"
@@ -348,11 +348,19 @@ end
     set_option!(hd, which, value)
 
 Options of `flou_hip_set_option`: 0 = reproduce the reference's two 3-D surface-flux
-slips (default 1), 1 = force the FP32 plain kernel.
+slips (default 1), 1 = force the FP32 plain kernels (rhs and adjoint).
 """
 set_option!(hd::HIPDisc, which::Integer, value::Integer) =
     check(hd.ptr, ccall((:flou_hip_set_option, libflouhip), Cint, (Ptr{Cvoid}, Int32, Int32),
                         hd.ptr, which, value), "flou_hip_set_option")
+
+"Does this FP32 handle run the fast rhs kernel?  (query 2 of `flou_hip_set_option`)"
+uses_fast_kernel(hd::HIPDisc) =
+    ccall((:flou_hip_set_option, libflouhip), Cint, (Ptr{Cvoid}, Int32, Int32), hd.ptr, 2, 0) == 1
+
+"Does this FP32 handle's adjoint run the fast kernel?  (query 3 of `flou_hip_set_option`)"
+uses_fast_vjp(hd::HIPDisc) =
+    ccall((:flou_hip_set_option, libflouhip), Cint, (Ptr{Cvoid}, Int32, Int32), hd.ptr, 3, 0) == 1
 
 # ----------------------------------------------------------------------------------------
 # Device state: Flou's (ndofs, nvars) column-major state matrix in HIP device memory
