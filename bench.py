@@ -386,18 +386,23 @@ def _c5_case():
     return std, eq, build
 
 
-def c5_vjp_kernel(reps=20):
-    """The C5 FP32 rhs VJP alone (profiling run of the fast adjoint kernel): reps calls of
-    flou_hip_rhs_vjp at the C5 state with a seeded cotangent, and reps fast rhs evaluations
-    at the same state, timed with HIP events on the handle's stream.  Algorithmic bytes of
-    one VJP call per node: Q, Y and the accumulated Qbar read (3 x 20 B), the metric and J
-    (40 B), Qbar written (20 B)."""
+def vjp_kernel(name, reps=20):
+    """An FP32 rhs VJP alone (profiling run of the fast adjoint kernel): reps calls of
+    flou_hip_rhs_vjp at the workload's state with a seeded cotangent, and reps fast rhs
+    evaluations at the same state, timed with HIP events on the handle's stream.
+    c5: the C5 mesh and state; c3: the C3 (64^3, N=5, Taylor-Green) mesh and state.
+    Algorithmic bytes of one VJP call per node: Q, Y and the accumulated Qbar read
+    (3 x 20 B), Qbar written (20 B), and on the curved mesh the metric and J (40 B)."""
     import torch
     import flouhip as F
     from flouhip import adjoint as A
-    std, eq, build = _c5_case()
-    disc, Q64 = build("fp32")
-    assert disc.uses_fast_vjp(), "the C5 FP32 adjoint should run the fast kernel"
+    if name == "c5":
+        std, eq, build = _c5_case()
+        disc, Q64 = build("fp32")
+    else:
+        disc, eq, std = build_disc("fp32")
+        Q64 = taylor_green_state(disc, std)
+    assert disc.uses_fast_vjp(), "this FP32 adjoint should run the fast kernel"
     Q = Q64.to(torch.float32).contiguous()
     g = torch.Generator(device="cpu")
     g.manual_seed(11)
@@ -422,11 +427,11 @@ def c5_vjp_kernel(reps=20):
     disc.check()
     vms, fms = ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
     nn = Q.shape[1]
-    vbytes = nn * (3 * 20 + 40 + 20)
+    vbytes = nn * (3 * 20 + 20 + (40 if name == "c5" else 0))
     return {"vjp_ms": vms, "rhs_ms": fms, "vjp_over_rhs": vms / fms, "nodes": nn,
             "vjp_algorithmic_bytes": vbytes, "vjp_hbm_fraction": vbytes / (vms * 1e-3) / 8e12,
-            "workload": "C5 curved 32^3 N=6 FP32 (the c5 leg's mesh and state), seeded cotangent; "
-                        "fast adjoint kernel vs fast rhs"}
+            "workload": ("C5 curved 32^3 N=6" if name == "c5" else "C3 64^3 N=5 Taylor-Green")
+                        + " FP32 (the bench's mesh and state), seeded cotangent; fast adjoint kernel vs fast rhs"}
 
 
 def c5_adjoint(nsteps=100, every=10):
@@ -549,7 +554,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp"], default=None,
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp", "c3vjp"],
+                    default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
     ap.add_argument("--no-secondary", action="store_true", help="skip the C2 and C5 workloads")
@@ -572,8 +578,8 @@ def main():
     if args.only == "c5adj":  # the C5 adjoint alone
         print(json.dumps({"c5_adjoint": c5_adjoint()}), flush=True)
         return
-    if args.only == "c5vjp":  # profiling run of the C5 FP32 adjoint kernel alone
-        print(json.dumps({"c5_vjp": c5_vjp_kernel()}), flush=True)
+    if args.only in ("c5vjp", "c3vjp"):  # profiling run of an FP32 adjoint kernel alone
+        print(json.dumps({args.only[:2] + "_vjp": vjp_kernel(args.only[:2])}), flush=True)
         return
     if args.only in ("c2", "c5"):  # profiling run of one secondary workload alone
         print(json.dumps({args.only: secondary_workload(args.only, args.steps, args.warmup)}), flush=True)

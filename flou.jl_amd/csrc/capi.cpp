@@ -76,7 +76,7 @@ struct flou_hip_handle {
     int fegeom_stride = 0;
     void* d_fdsharp = nullptr;
     void* d_fnj = nullptr;      // curved fast kernel: per face node (normal, surface J), float4
-    // FP32 adjoint on the curved fast path: its kernel, S_ij = ω_i D♯_ij and 1/ω
+    // FP32 adjoint on the fast path: its kernel, S_ij = ω_i D♯_ij and 1/ω
     FastVjpLauncher vfast = nullptr;
     void* d_fwsh = nullptr;
     void* d_fiw = nullptr;
@@ -679,7 +679,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                 fnj[q * 4 + 3] = (float)fg[q * FG + 3 * nd];
             }
         }
-        if (h->lfast && h->curved && h->v32 && std::find(nb.begin(), nb.end(), -1) == nb.end()) {
+        if (h->lfast && h->v32 && std::find(nb.begin(), nb.end(), -1) == nb.end()) {
             // the FP32 adjoint on this path (Chandrasekhar volume flux, every face interior):
             // S_ij = ω_i D♯_ij, antisymmetric off the zero diagonal (SBP on GLL nodes)
             double smax = 0.0, asym = 0.0;
@@ -694,7 +694,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                     fws[i * n + j] = (float)(0.5 * (sij - sji));
                 }
             }
-            if (asym <= 1e-10 * smax) h->vfast = fast_vjp_launcher(nd, n, h->tp);
+            if (asym <= 1e-10 * smax) h->vfast = fast_vjp_launcher(nd, n, h->tp, h->curved != 0);
         }
         if (h->lfast) {
             fds.resize(n * n);

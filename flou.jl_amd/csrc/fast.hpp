@@ -40,7 +40,7 @@ typedef hipError_t (*FastLauncher)(const FastArgs&, hipStream_t);
 // axis-aligned affine elements or (curved) with per-node metric and J.
 FastLauncher fast_launcher(int nd, int n, int tp, bool curved);
 
-// FP32 adjoint pass on the curved fast path, the same contract as VjpArgs (common.hpp):
+// FP32 adjoint pass on the fast path, the same contract as VjpArgs (common.hpp):
 //   o0 = a0·P0 + a1·P1 + (∂out/∂Q)^T(c·Y) at f.Q,  o1 = b0·o0 + b1·P1 + b2·P2 (o1 optional),
 // with out the rhs (f.Q, geometry, face tables, lift, γ and the LxF intensity as in FastArgs;
 // f.out, f.ubase, f.reg, f.dt, f.A, f.B, f.mode and f.first unused).
@@ -60,8 +60,9 @@ struct FastVjpArgs {
 
 typedef hipError_t (*FastVjpLauncher)(const FastVjpArgs&, hipStream_t);
 
-// Curved 2-D and 3-D Euler split form with the Chandrasekhar two-point flux, n = 4..8,
-// every face interior (periodic or between elements); nullptr otherwise.
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp);
+// 2-D and 3-D Euler split form with the Chandrasekhar two-point flux, n = 4..8, on the
+// fast path's elements (curved, or axis-aligned affine), every face interior (periodic or
+// between elements); nullptr otherwise.
+FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved);
 
 }  // namespace flouhip

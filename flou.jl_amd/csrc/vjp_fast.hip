@@ -1,4 +1,4 @@
-// FP32 adjoint of the curved fast path: the vector-Jacobian product of the rhs of the
+// FP32 adjoint of the fast path: the vector-Jacobian product of the rhs of the
 // split-form Euler operator with the Chandrasekhar two-point flux and the LxF(StdAverage)
 // or StdAverage surface flux on GLL nodes (SplitDivOperator, interface_fluxes!, the FR
 // lift and 1/J: OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:
@@ -21,7 +21,8 @@
 //   * per node: Σ_d of the line rows, one pullback (a, v, β) -> (ρ, m, E), the face terms
 //     of the node's faces, then the epilogue
 //       o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2  (o1 optional).
-// Faces must all be interior (periodic or between elements) and of orientation 0.
+// Curved elements (per-node metric and J) and axis-aligned affine ones (per-element
+// metric); faces must all be interior (periodic or between elements) and of orientation 0.
 #include "common.hpp"
 #include "fast.hpp"
 #include "fast_util.hpp"
@@ -278,7 +279,10 @@ __device__ __forceinline__ bool face_vjp(const float* o, const float* x, const f
     return c2 >= 0.0f && c2x >= 0.0f;
 }
 
-template <int ND, int N>
+// CURV: per-node metric and J (ngeom) and per-face-node normal and surface J (fnj); else
+// axis-aligned affine elements: per-element Ja_dd/J and 1/J (egeom, stride 0 when every
+// element is alike) and one frame and surface J per face (fgeom).
+template <int ND, int N, bool CURV>
 __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A) {
     using C = VCfg<ND, N>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
@@ -326,21 +330,43 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
             y[r][v] = bload(ry, (uint32_t)(tid + r * TPB) * 4u);
         }
     }
+    if constexpr (CURV) {
 #pragma unroll
-    for (int c = 0; c < ND * ND + 1; ++c) {
-        const rsrc_t rg = make_rsrc(a.ngeom + c * Nd + n0, blk_bytes);
+        for (int c = 0; c < ND * ND + 1; ++c) {
+            const rsrc_t rg = make_rsrc(a.ngeom + c * Nd + n0, blk_bytes);
 #pragma unroll
-        for (int r = 0; r < NPT; ++r) g[r][c] = bload(rg, (uint32_t)(tid + r * TPB) * 4u);
+            for (int r = 0; r < NPT; ++r) g[r][c] = bload(rg, (uint32_t)(tid + r * TPB) * 4u);
+        }
+    } else {
+        // column d of the metric is Ja_dd e_d; Ja_dd = (Ja_dd/J)·J
+#pragma unroll
+        for (int r = 0; r < NPT; ++r) {
+            const int j = tid + r * TPB;
+            const int le = (j < NN ? j : 0) / NPE;
+            const float* eg = a.egeom + (int64_t)(e0 + (le < nel ? le : 0)) * a.egeom_stride;
+            const float J = rcp(eg[3]);
+#pragma unroll
+            for (int c = 0; c < ND * ND; ++c) g[r][c] = c % (ND + 1) == 0 ? eg[c / (ND + 1)] * J : 0.0f;
+            g[r][ND * ND] = J;
+        }
     }
-    // face normal and surface Jacobian of both line ends (per face node, master side)
+    // face normal and surface Jacobian of both line ends (master side; curved: per face
+    // node, affine: per face)
     float fn[2][ND], fj[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
-        const float4 t = line_ok ? a.fnj[(int64_t)xfg[sd] * NPF + lf] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
-        fn[sd][0] = t.x;
-        if (ND > 1) fn[sd][1] = t.y;
-        if (ND > 2) fn[sd][ND - 1] = t.z;
-        fj[sd] = t.w;
+        if constexpr (CURV) {
+            const float4 t = line_ok ? a.fnj[(int64_t)xfg[sd] * NPF + lf] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
+            fn[sd][0] = t.x;
+            if (ND > 1) fn[sd][1] = t.y;
+            if (ND > 2) fn[sd][ND - 1] = t.z;
+            fj[sd] = t.w;
+        } else {
+            const float* fg = a.fgeom + (int64_t)(line_ok ? xfg[sd] : 0) * (3 * ND + 1);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) fn[sd][k] = fg[k];
+            fj[sd] = fg[3 * ND];
+        }
     }
     // both traces at both line ends, exactly as the forward reads them (the own end node
     // and the neighbour's, also inside the block: the LxF wave-speed maximum takes its
@@ -367,10 +393,14 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
             oq[sd][v] = bload(rq, (uint32_t)(base + (sd ? (N - 1) * stride : 0)) * 4u);
         }
     }
-    {
+    if constexpr (CURV) {
         const rsrc_t rj = make_rsrc(a.ngeom + ND * ND * Nd, (uint32_t)Nd * 4u);
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) xj[sd] = bload(rj, xo[sd]);
+    } else {
+#pragma unroll
+        for (int sd = 0; sd < 2; ++sd)
+            xj[sd] = rcp(a.egeom[(int64_t)(xnb[sd] >= 0 ? xnb[sd] : 0) * a.egeom_stride + 3]);
     }
     // state, primitives, metric and w to LDS
 #pragma unroll
@@ -559,32 +589,32 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
     if (err) atomicOr(a.errflag, err);
 }
 
-template <int ND, int N>
+template <int ND, int N, bool CURV>
 hipError_t launch_fvjp(const FastVjpArgs& a, hipStream_t s) {
     using C = VCfg<ND, N>;
     const int nb = (a.f.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_fvjp<ND, N>), dim3(nb), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_fvjp<ND, N, CURV>), dim3(nb), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
-template <int ND>
+template <int ND, bool CURV>
 FastVjpLauncher fvjp_n(int n) {
     switch (n) {
-        case 4: return &launch_fvjp<ND, 4>;
-        case 5: return &launch_fvjp<ND, 5>;
-        case 6: return &launch_fvjp<ND, 6>;
-        case 7: return &launch_fvjp<ND, 7>;
-        case 8: return &launch_fvjp<ND, 8>;
+        case 4: return &launch_fvjp<ND, 4, CURV>;
+        case 5: return &launch_fvjp<ND, 5, CURV>;
+        case 6: return &launch_fvjp<ND, 6, CURV>;
+        case 7: return &launch_fvjp<ND, 7, CURV>;
+        case 8: return &launch_fvjp<ND, 8, CURV>;
         default: return nullptr;
     }
 }
 
 }  // namespace fast
 
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp) {
+FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved) {
     if (tp != TP_CHANDRASEKHAR) return nullptr;
-    if (nd == 2) return fast::fvjp_n<2>(n);
-    if (nd == 3) return fast::fvjp_n<3>(n);
+    if (nd == 2) return curved ? fast::fvjp_n<2, true>(n) : fast::fvjp_n<2, false>(n);
+    if (nd == 3) return curved ? fast::fvjp_n<3, true>(n) : fast::fvjp_n<3, false>(n);
     return nullptr;
 }
 

@@ -1,6 +1,6 @@
-"""The FP32 fast adjoint kernel (flou.jl_amd/csrc/vjp_fast.hip) on curved periodic meshes
-(the split form with the Chandrasekhar flux and LxF; OpDivergence.jl:200-299,
-Interfaces.jl:111-136, Equations/Euler.jl:474-536):
+"""The FP32 fast adjoint kernel (flou.jl_amd/csrc/vjp_fast.hip) on curved and on
+axis-aligned affine periodic meshes (the split form with the Chandrasekhar flux and LxF;
+OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:474-536):
 
 * the rhs VJP against the plain FP32 VJP (k_vjp, switched on with option 1) on the same
   inputs, over 2-D and 3-D, N = 4..8, element counts that leave the last block part-filled,
@@ -19,7 +19,7 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import periodic_euler_state
+from cases import GAMMA, oracle_for, periodic_euler_state
 from flouhip import adjoint as A
 from test_gpu_curved import build_curved
 
@@ -37,6 +37,21 @@ def _fp32_disc(nd, n, nel):
     disc, orc, eq, X, J, std = build_curved(nd, n, nel, "split", "ch", precision="fp32")
     assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
     return disc, orc, X
+
+
+def _affine_fp32_disc(nd, n, nel, lengths):
+    """Periodic Cartesian mesh on a box with unequal sides (Ja_dd differs per direction)."""
+    b = F.LagrangeBasis("GLL", n)
+    std = [None, None, F.StdQuad, F.StdHex][nd](b, F.DGSEMrec(b), nd + 2)
+    mesh = F.CartesianMesh(nd, (0.0,) * nd, tuple(lengths), (nel,) * nd)
+    mesh.apply_periodicBCs(*[(str(2 * k + 1), str(2 * k + 2)) for k in range(nd)])
+    eq = F.EulerEquation(nd, GAMMA)
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    disc = F.MultielementDisc(mesh, std, eq, op, precision="fp32")
+    assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
+    orc = oracle_for(nd, n, [nel] * nd, [0.0] * nd, list(lengths), [1] * nd, O.EQ_EULER, [GAMMA], O.VOL_SPLIT,
+                     O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+    return disc, orc, mesh.node_coords(std)
 
 
 def _vjp(disc, Q, W, plain):
@@ -73,6 +88,37 @@ def test_fast_vjp_matches_plain_fp32_vjp(case, amp):
     err, emax = _rel2(got, ref), _rel(got, ref)
     print(f"{case} amp {amp}: fast vs plain FP32 VJP, relative L2 {err:.2e}, max {emax:.2e}")
     assert err <= 2e-6
+
+
+AFFINE = [(3, 6, 3, (1.0, 1.5, 0.75)), (3, 5, 3, (2.0, 1.0, 1.0)), (3, 7, 2, (1.0, 1.0, 1.0)),
+          (2, 8, 4, (1.0, 2.0)), (2, 5, 6, (1.5, 1.0))]
+
+
+@pytest.mark.parametrize("case", AFFINE, ids=lambda c: "{}d-n{}-{}".format(*c[:3]))
+def test_fast_vjp_affine_matches_plain_fp32_vjp(case):
+    nd, n, nel, lengths = case
+    disc, orc, X = _affine_fp32_disc(nd, n, nel, lengths)
+    Q = periodic_euler_state(X, nd, 1.0, seed=15, amp=0.2)
+    W = np.random.default_rng(16).standard_normal(Q.shape)
+    got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
+    err, emax = _rel2(got, ref), _rel(got, ref)
+    print(f"affine {case}: fast vs plain FP32 VJP, relative L2 {err:.2e}, max {emax:.2e}")
+    assert err <= 2e-6
+
+
+@pytest.mark.parametrize("case", [AFFINE[0], AFFINE[3]], ids=["3d-n6", "2d-n8"])
+def test_fast_vjp_affine_dot_product_with_oracle_jvp(case):
+    nd, n, nel, lengths = case
+    disc, orc, X = _affine_fp32_disc(nd, n, nel, lengths)
+    Q = periodic_euler_state(X, nd, 1.0, seed=17, amp=0.2).astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(18)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
+    JtW = _vjp(disc, Q, W, plain=False)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(JtW * V))
+    err = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"affine {case}: <W,JV> = {lhs:.8e}  <J^T W,V> (fast FP32) = {rhs:.8e}  rel {err:.2e}")
+    assert err <= 2e-5
 
 
 @pytest.mark.parametrize("case", [(3, 7, 2), (3, 5, 2), (2, 6, 4)], ids=["3d-n7", "3d-n5", "2d-n6"])
