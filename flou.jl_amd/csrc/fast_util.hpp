@@ -29,5 +29,13 @@ __device__ __forceinline__ void bstore(rsrc_t r, uint32_t off, float x) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, (int)off, 0, 0);
 }
 
+// The log-mean path of a wave's pair blocks: 2 when every line of the wave allows the
+// first-order form, else 1 when every line allows the series, else 0 (each path is valid
+// for the lines of the cheaper ones).  One path per wave: lines of one wave on different
+// paths would run both pair blocks one after the other.
+__device__ __forceinline__ int wave_path(bool first_order, bool series) {
+    return __all(first_order) ? 2 : (__all(series) ? 1 : 0);
+}
+
 }  // namespace fast
 }  // namespace flouhip

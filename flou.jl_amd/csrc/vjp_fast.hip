@@ -63,13 +63,19 @@ struct VCfg {
 // cancels for small u (at u = 0.01 the bracket is 0.0067, so F's error, about 1e-6 from
 // the difference of two FP32 logarithms, grows 150-fold), so for 0.01 <= u < 0.25 both
 // come from the series Σ u^k/(2k+1) to k = 11 and its derivative (FP32 errors 6e-8 and
-// 1e-7), above that from the log form.  SER: the caller knows that u < 0.01.  lxi, lxj:
-// ln x_i, ln x_j (unused on the series branch).
-template <bool SER>
+// 1e-7), above that from the log form.  PATH 1: the caller knows that u < 0.01 (the
+// series); PATH 2: u < 1e-4, F = 1 + u/3 and dF/du = 1/3 + 2u/5 (truncation below 3e-9
+// relative); PATH 0: any u.  lxi, lxj: ln x_i, ln x_j (used by PATH 0 only).
+template <int PATH>
 __device__ __forceinline__ void lm_factor(float u, float f, float lxi, float lxj, float& F, float& dF) {
+    if (PATH == 2) {
+        F = fmaf(u, 1.0f / 3.0f, 1.0f);
+        dF = fmaf(u, 2.0f / 5.0f, 1.0f / 3.0f);
+        return;
+    }
     const float Fs = fmaf(u, fmaf(u, fmaf(u, 1.0f / 7.0f, 1.0f / 5.0f), 1.0f / 3.0f), 1.0f);
     const float dFs = fmaf(u, fmaf(u, 3.0f / 7.0f, 2.0f / 5.0f), 1.0f / 3.0f);
-    if (SER) {
+    if (PATH == 1) {
         F = Fs;
         dF = dFs;
     } else {
@@ -95,8 +101,9 @@ __device__ __forceinline__ void lm_factor(float u, float f, float lxi, float lxj
 //   h = (F_β/(γ-1) + F_ρ)/(β_i + β_j) + 2 v_i·v_j
 //     (= 1/(2(γ-1)β̂) - (|u_i|² + |u_j|²)/4 + p̄/ρ̂ + |ū|²),
 //   F♯ = (m, m ū + p̄ n̄, m h),
-// F_ρ, F_β the log-mean factors of ρ and β.
-template <int ND, bool SER>
+// F_ρ, F_β the log-mean factors of ρ and β (PATH as in lm_factor; on PATH 2 also
+// 1/F_ρ = 1 - u/3 to first order, truncation below 1e-9).
+template <int ND, int PATH>
 __device__ __forceinline__ void pair_vjp(float ai, float aj, const float* vi, const float* vj, float bi, float bj,
                                          float lai, float laj, float lbi, float lbj, const float* Mi,
                                          const float* Mj, const float* G, float s, float ig1, float* gi, float* gj) {
@@ -117,9 +124,9 @@ __device__ __forceinline__ void pair_vjp(float ai, float aj, const float* vi, co
     const float rb = rcp(sb), ihs = rcp(hs);
     const float fr = dA * ihs, ur = fr * fr, fb = db * rb, u2 = fb * fb;
     float Fr, dFr, Fb, dFb;
-    lm_factor<SER>(ur, fr, lai, laj, Fr, dFr);
-    lm_factor<SER>(u2, fb, lbi, lbj, Fb, dFb);
-    const float iFr = rcp(Fr);
+    lm_factor<PATH>(ur, fr, lai, laj, Fr, dFr);
+    lm_factor<PATH>(u2, fb, lbi, lbj, Fb, dFb);
+    const float iFr = PATH == 2 ? fmaf(-ur, 1.0f / 3.0f, 1.0f) : rcp(Fr);
     const float rho = hs * iFr;
     const float m = rho * un;
     const float Xf = fmaf(Fb, ig1, Fr);
@@ -191,7 +198,7 @@ __device__ __forceinline__ void load_metric(const float* __restrict__ sMd, int j
 // All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
 // j's read from LDS (records at sR, the metric vector of the line's direction at sMd), w
 // scaled by 1/ω of the node's position on the line.
-template <int ND, int N, int NN, bool SER>
+template <int ND, int N, int NN, int PATH>
 __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const float* __restrict__ sMd, int base,
                                          int stride, const float* __restrict__ iw, const float* __restrict__ wsh,
                                          float ig1, float (&acc)[N][ND + 2]) {
@@ -205,7 +212,7 @@ __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const fl
         const float iwi = iw[i];
 #pragma unroll
         for (int v = 0; v < NV; ++v) wi[v] = pi[NV + v] * iwi;
-        const float lai = SER ? 0.0f : lg(pi[0]), lbi = SER ? 0.0f : lg(pi[ND + 1]);
+        const float lai = PATH ? 0.0f : lg(pi[0]), lbi = PATH ? 0.0f : lg(pi[ND + 1]);
 #pragma unroll
         for (int j = i + 1; j < N; ++j) {
             const int nj = base + j * stride;
@@ -215,8 +222,8 @@ __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const fl
             const float iwj = iw[j];
 #pragma unroll
             for (int v = 0; v < NV; ++v) G[v] = fmaf(pj[NV + v], iwj, -wi[v]);
-            const float laj = SER ? 0.0f : lg(pj[0]), lbj = SER ? 0.0f : lg(pj[ND + 1]);
-            pair_vjp<ND, SER>(pi[0], pj[0], pi + 1, pj + 1, pi[ND + 1], pj[ND + 1], lai, laj, lbi, lbj, Mi, Mj, G,
+            const float laj = PATH ? 0.0f : lg(pj[0]), lbj = PATH ? 0.0f : lg(pj[ND + 1]);
+            pair_vjp<ND, PATH>(pi[0], pj[0], pi + 1, pj + 1, pi[ND + 1], pj[ND + 1], lai, laj, lbi, lbj, Mi, Mj, G,
                               wsh[i * N + j], ig1, acc[i], acc[j]);
             // keep each pair's loads and arithmetic together: the registers then hold the
             // line's rows, node i and one pair (the scheduler would otherwise hoist the
@@ -283,7 +290,7 @@ __device__ __forceinline__ bool face_vjp(const float* o, const float* x, const f
 // axis-aligned affine elements: per-element Ja_dd/J and 1/J (egeom, stride 0 when every
 // element is alike) and one frame and surface J per face (fgeom).
 template <int ND, int N, bool CURV>
-__global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A) {
+__global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
     using C = VCfg<ND, N>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
     constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
@@ -490,14 +497,20 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB)) k_fvjp(const FastVjpArgs A
 #pragma unroll
             for (int v = 0; v < NV; ++v) acc[p][v] = 0.0f;
         }
-        // the series branch for every pair of the line when the spreads of ρ and β allow
-        // it (u < 0.01 on the line's extreme pair, with a margin), as in the forward
+        // one log-mean path for every pair of the wave's lines when the spreads of ρ and β
+        // allow it (u < 1e-4: first order, u < 0.01: the series, on each line's extreme
+        // pair with a margin), as in the forward
         const float da = amax - amin, sa = amax + amin, dbm = bmax - bmin, sbm = bmax + bmin;
         const float* const sMd = sM + ldir * MW * NN;
-        if (da * da * 1.0001f < 1e-2f * sa * sa && dbm * dbm * 1.0001f < 1e-2f * sbm * sbm)
-            line_vjp<ND, N, NN, true>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
+        const float qa = da * da * 1.0001f, pa = sa * sa, qb = dbm * dbm * 1.0001f, pb = sbm * sbm;
+        const int path = wave_path(!line_ok || (qa < 1e-4f * pa && qb < 1e-4f * pb),
+                                   !line_ok || (qa < 1e-2f * pa && qb < 1e-2f * pb));
+        if (path == 2)
+            line_vjp<ND, N, NN, 2>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
+        else if (path == 1)
+            line_vjp<ND, N, NN, 1>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
         else
-            line_vjp<ND, N, NN, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
+            line_vjp<ND, N, NN, 0>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
     }
 
     // epilogue operands (null pointers read as zero)
