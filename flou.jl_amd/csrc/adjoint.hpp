@@ -290,10 +290,13 @@ FLOU_HD inline void node_prims(const T* Q, T gamma, T* R) {
 // twopoint_vjp2 with the per-node primitives and reciprocals given (node_prims) and the
 // pair's reciprocals shared: 1/(β1+β2) from the β log-mean, 1/ρ̂ = 2F_ρ/(ρ1+ρ2) and
 // 1/β̂ = 2F_β/(β1+β2) from the log-mean factors (four divisions per pair on the series
-// branch).  Equal to twopoint_vjp2 up to rounding.
-template <typename T, int ND, int TP>
+// branch).  Equal to twopoint_vjp2 up to rounding.  PRIM (Chandrasekhar only): Qb1 and Qb2
+// receive the cotangents of the primitives (ρ, u_1..u_d, p) instead, for a caller that
+// sums them over all pairs of a node and applies prim_pullback_u once.
+template <typename T, int ND, int TP, bool PRIM = false>
 FLOU_HD inline void twopoint_vjp2p(const T* Q1, const T* R1, const T* Q2, const T* R2, const T* n, T gamma,
                                    T ig1, const T* Gb, T* Qb1, T* Qb2) {
+    static_assert(!PRIM || TP == TP_CHANDRASEKHAR, "primitive cotangents: Chandrasekhar flux only");
     if constexpr (TP == TP_STDAVG) {
         T h[ND + 2];
         for (int v = 0; v < ND + 2; ++v) h[v] = T(0.5) * Gb[v];
@@ -367,8 +370,19 @@ FLOU_HD inline void twopoint_vjp2p(const T* Q1, const T* R1, const T* Q2, const 
             pb1 = -bb1 * b1 * ip1;
             pb2 = -bb2 * b2 * ip2;
         }
-        prim_pullback_u<T, ND>(ir1, u1, gamma, rb1, ub1, pb1, Qb1);
-        prim_pullback_u<T, ND>(ir2, u2, gamma, rb2, ub2, pb2, Qb2);
+        if constexpr (PRIM) {
+            Qb1[0] += rb1;
+            Qb2[0] += rb2;
+            for (int k = 0; k < ND; ++k) {
+                Qb1[1 + k] += ub1[k];
+                Qb2[1 + k] += ub2[k];
+            }
+            Qb1[ND + 1] += pb1;
+            Qb2[ND + 1] += pb2;
+        } else {
+            prim_pullback_u<T, ND>(ir1, u1, gamma, rb1, ub1, pb1, Qb1);
+            prim_pullback_u<T, ND>(ir2, u2, gamma, rb2, ub2, pb2, Qb2);
+        }
     }
 }
 

@@ -83,6 +83,10 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     // barrier every node adds the sides it received.
     constexpr int NPAIR = N * (N - 1) / 2;
     constexpr bool PS = VOL == VOL_SPLIT && EQ == EQ_EULER && EPB * NV * NPF * NPAIR * (int)sizeof(T) <= 65536;
+    // Chandrasekhar: the pair sides are cotangents of the primitives (ρ, u, p), own and
+    // received summed over every pair and direction and pulled back to (ρ, m, E) once per
+    // node (the pullback is linear), instead of twice per pair
+    constexpr bool PP = PS && TP == TP_CHANDRASEKHAR;
     __shared__ T sPo[PS ? EPB : 1][PS ? NV : 1][PS ? NPF * NPAIR : 1];
     // Per-node primitives for the pair phase (adj::node_prims without β, which is rebuilt
     // from ρ and 1/p), computed once per node instead of once per pair and direction, when
@@ -378,10 +382,12 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
         }
         M = Mnode;
     }
-    T ri[PS ? NR : 1];
+    T ri[PS ? NR : 1], Vp[PP ? NV : 1];
     if constexpr (PS) {
         if (valid) adj::node_prims<T, ND>(qi, a.gamma, ri);
     }
+#pragma unroll
+    for (int v = 0; v < (PP ? NV : 1); ++v) Vp[v] = T(0);
 #pragma unroll
     for (int dir = 0; dir < ND; ++dir) {
         int f, ii;
@@ -418,7 +424,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
 #pragma unroll
                     for (int c = 0; c < ND; ++c)
                         nbar[c] = CURV ? (n[c] + a.ngeom[(c + ND * dir) * Nd + (int64_t)e * NPE + nj]) / T(2) : n[c];
-                    adj::twopoint_vjp2p<T, ND, TP>(qi, ri, qj, rj, nbar, a.gamma, ig1, Gb, V, Qbb);
+                    adj::twopoint_vjp2p<T, ND, TP, PP>(qi, ri, qj, rj, nbar, a.gamma, ig1, Gb, PP ? Vp : V, Qbb);
                     const int slot = pair_slot<N, NPE>(k, nj, f, jj);
 #pragma unroll
                     for (int v = 0; v < NV; ++v) sPo[le][v][slot] = Qbb[v];
@@ -429,8 +435,9 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
             for (int k = 1; k <= N / 2; ++k) {  // the sides received from the owners ii - k
                 if (N % 2 == 0 && k == N / 2 && ii < N / 2) break;
                 const int slot = pair_slot<N, NPE>(k, node, f, ii);
+                T* acc = PP ? Vp : V;
 #pragma unroll
-                for (int v = 0; v < NV; ++v) V[v] = V[v] + sPo[le][v][slot];
+                for (int v = 0; v < NV; ++v) acc[v] = acc[v] + sPo[le][v][slot];
             }
             const T d = Dsh[ii * N + ii];
             if (d != T(0)) {  // zero on GLL nodes
@@ -506,6 +513,7 @@ __global__ void __launch_bounds__((Cfg<ND, N>::TPB)) k_vjp(const VjpArgs<T> A) {
     }
 
     if (!valid) return;
+    if constexpr (PP) adj::prim_pullback_u<T, ND>(ri[ND + 1], ri, a.gamma, Vp[0], Vp + 1, Vp[ND + 1], V);
     // epilogue
     const int64_t gi = (int64_t)e * NPE + node;
 #pragma unroll
