@@ -679,8 +679,8 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                 fnj[q * 4 + 3] = (float)fg[q * FG + 3 * nd];
             }
         }
-        if (h->lfast && h->v32 && std::find(nb.begin(), nb.end(), -1) == nb.end()) {
-            // the FP32 adjoint on this path (Chandrasekhar volume flux, every face interior):
+        if (h->lfast && h->v32) {
+            // the FP32 adjoint on this path (Chandrasekhar volume flux):
             // S_ij = ω_i D♯_ij, antisymmetric off the zero diagonal (SBP on GLL nodes)
             double smax = 0.0, asym = 0.0;
             fws.resize(n * n);

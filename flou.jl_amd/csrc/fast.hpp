@@ -61,8 +61,7 @@ struct FastVjpArgs {
 typedef hipError_t (*FastVjpLauncher)(const FastVjpArgs&, hipStream_t);
 
 // 2-D and 3-D Euler split form with the Chandrasekhar two-point flux, n = 4..8, on the
-// fast path's elements (curved, or axis-aligned affine), every face interior (periodic or
-// between elements); nullptr otherwise.
+// fast path's elements (curved, or axis-aligned affine) and boundaries; nullptr otherwise.
 FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved);
 
 }  // namespace flouhip

@@ -22,7 +22,8 @@
 //     of the node's faces, then the epilogue
 //       o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2  (o1 optional).
 // Curved elements (per-node metric and J) and axis-aligned affine ones (per-element
-// metric); faces must all be interior (periodic or between elements) and of orientation 0.
+// metric); faces of orientation 0; interior faces and Dirichlet, inflow, outflow and slip
+// walls.
 #include "common.hpp"
 #include "fast.hpp"
 #include "fast_util.hpp"
@@ -233,57 +234,84 @@ __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const fl
     }
 }
 
-// Reverse product, with respect to the own trace o, of the face flux as the own element
-// sees it at a line end (Interfaces.jl:18-23, Equations/Euler.jl:99-165 in the own frame):
-//   S = (f(o) + f(x))·n + λ·diss·(o - x),  λ = max(|u_o·n| + c_o, |u_x·n| + c_x),
-// with n the own outward normal and x the neighbour's trace; adds (∂S/∂o)^T G to ob.
-// Returns false if a sound speed is not real.
+// One side Q of the face flux as the own element sees it at a line end (Interfaces.jl:
+// 18-23, Equations/Euler.jl:99-165 in the own frame):
+//   S = (f(o) + f(x))·n + λ·diss·(o - x),  λ = max(a_o, a_x),  a = |u·n| + c,
+// with n the own outward normal, o the own and x the exterior trace.
 template <int ND>
-__device__ __forceinline__ bool face_vjp(const float* o, const float* x, const float* n, const float* G,
-                                         float gamma, float gm1, float diss, float* ob) {
+struct FaceSide {
+    float ir, u[ND], uu, un, p, c2, c, a;
+};
+template <int ND>
+__device__ __forceinline__ FaceSide<ND> face_side(const float* Q, const float* n, float gamma, float gm1) {
     constexpr int NV = ND + 2;
-    const float ir = rcp(o[0]);
-    float u[ND];
+    FaceSide<ND> s;
+    s.ir = rcp(Q[0]);
 #pragma unroll
-    for (int k = 0; k < ND; ++k) u[k] = o[1 + k] * ir;
-    float uu = u[0] * u[0], un = u[0] * n[0], gm = G[1] * o[1], gn = G[1] * n[0];
-    float mx2 = x[1] * x[1], mxn = x[1] * n[0];
+    for (int k = 0; k < ND; ++k) s.u[k] = Q[1 + k] * s.ir;
+    s.uu = s.u[0] * s.u[0];
+    s.un = s.u[0] * n[0];
 #pragma unroll
     for (int k = 1; k < ND; ++k) {
-        uu = fmaf(u[k], u[k], uu);
-        un = fmaf(u[k], n[k], un);
-        gm = fmaf(G[1 + k], o[1 + k], gm);
-        gn = fmaf(G[1 + k], n[k], gn);
-        mx2 = fmaf(x[1 + k], x[1 + k], mx2);
-        mxn = fmaf(x[1 + k], n[k], mxn);
+        s.uu = fmaf(s.u[k], s.u[k], s.uu);
+        s.un = fmaf(s.u[k], n[k], s.un);
     }
-    const float p = gm1 * fmaf(-0.5f * o[0], uu, o[NV - 1]);
-    const float c2 = gamma * p * ir;
-    const float c = sq(c2);
-    const float ao = fabsf(un) + c;
-    const float irx = rcp(x[0]);
-    const float px = gm1 * fmaf(-0.5f * mx2, irx, x[NV - 1]);
-    const float c2x = gamma * px * irx;
-    const float ax = fabsf(mxn * irx) + sq(c2x);
-    const float lam = fmaxf(ao, ax) * diss;
-    float unb = fmaf(G[NV - 1], o[NV - 1] + p, gm);  // of u_o·n
-    float pb = fmaf(G[NV - 1], un, gn);               // of p_o
-    // λ's dependence on o when the own wave speed is the larger one
+    s.p = gm1 * fmaf(-0.5f * Q[0], s.uu, Q[NV - 1]);
+    s.c2 = gamma * s.p * s.ir;
+    s.c = sq(s.c2);
+    s.a = fabsf(s.un) + s.c;
+    return s;
+}
+
+// Adds to qb the reverse product, with respect to one side Q, of f(Q)·n for the cotangent
+// G, plus cb times the gradient of that side's wave speed a = |u·n| + c (cb: the cotangent
+// of λ when this side's a is the maximum, else 0).
+template <int ND>
+__device__ __forceinline__ void face_side_vjp(const float* Q, const FaceSide<ND>& s, const float* n, const float* G,
+                                              float gamma, float gm1, float cb, float* qb) {
+    constexpr int NV = ND + 2;
+    float gm = G[1] * Q[1], gn = G[1] * n[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        gm = fmaf(G[1 + k], Q[1 + k], gm);
+        gn = fmaf(G[1 + k], n[k], gn);
+    }
+    float unb = fmaf(G[NV - 1], Q[NV - 1] + s.p, gm);  // of u·n
+    float pb = fmaf(G[NV - 1], s.un, gn);               // of p
+    unb = fmaf(cb, s.un >= 0.0f ? 1.0f : -1.0f, unb);
+    const float c2b = 0.5f * cb * rcp(s.c);
+    pb = fmaf(c2b, gamma * s.ir, pb);
+    const float pg = gm1 * pb;
+    const float uir = unb * s.ir;
+    qb[0] += fmaf(-c2b * s.c2, s.ir, fmaf(-uir, s.un, 0.5f * pg * s.uu));
+#pragma unroll
+    for (int k = 0; k < ND; ++k) qb[1 + k] += fmaf(G[1 + k], s.un, fmaf(G[0] + uir, n[k], -pg * s.u[k]));
+    qb[NV - 1] += fmaf(G[NV - 1], s.un, pg);
+}
+
+// Reverse product of S for the cotangent G: (∂S/∂o)^T G added to ob and, when xb is given,
+// (∂S/∂x)^T G added to xb (an exterior state that depends on the own trace: outflow and
+// slip walls).  The own side takes λ's derivative when a_o >= a_x.  Returns false if a
+// sound speed is not real.
+template <int ND>
+__device__ __forceinline__ bool face_vjp(const float* o, const float* x, const float* n, const float* G,
+                                         float gamma, float gm1, float diss, float* ob, float* xb) {
+    constexpr int NV = ND + 2;
+    const FaceSide<ND> so = face_side<ND>(o, n, gamma, gm1), sx = face_side<ND>(x, n, gamma, gm1);
+    const float lam = fmaxf(so.a, sx.a) * diss;
     float dl = (o[0] - x[0]) * G[0];
 #pragma unroll
     for (int v = 1; v < NV; ++v) dl = fmaf(o[v] - x[v], G[v], dl);
-    const float cb = ao >= ax ? dl * diss : 0.0f;  // of c_o (and of |u_o·n|)
-    unb = fmaf(cb, un >= 0.0f ? 1.0f : -1.0f, unb);
-    const float c2b = 0.5f * cb * rcp(c);
-    pb = fmaf(c2b, gamma * ir, pb);
-    const float pg = gm1 * pb;
-    const float uir = unb * ir;
-    ob[0] += fmaf(-c2b * c2, ir, fmaf(-uir, un, fmaf(0.5f * pg, uu, lam * G[0])));
+    const bool own = so.a >= sx.a;
 #pragma unroll
-    for (int k = 0; k < ND; ++k)
-        ob[1 + k] += fmaf(G[1 + k], un, fmaf(G[0] + uir, n[k], fmaf(-pg, u[k], lam * G[1 + k])));
-    ob[NV - 1] += fmaf(G[NV - 1], un, pg + lam * G[NV - 1]);
-    return c2 >= 0.0f && c2x >= 0.0f;
+    for (int v = 0; v < NV; ++v) ob[v] = fmaf(lam, G[v], ob[v]);
+    face_side_vjp<ND>(o, so, n, G, gamma, gm1, own ? dl * diss : 0.0f, ob);
+    if (xb) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v) xb[v] = fmaf(-lam, G[v], xb[v]);
+        face_side_vjp<ND>(x, sx, n, G, gamma, gm1, own ? 0.0f : dl * diss, xb);
+    }
+    return so.c2 >= 0.0f && sx.c2 >= 0.0f;
 }
 
 // CURV: per-node metric and J (ngeom) and per-face-node normal and surface J (fnj); else
@@ -377,7 +405,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     }
     // both traces at both line ends, exactly as the forward reads them (the own end node
     // and the neighbour's, also inside the block: the LxF wave-speed maximum takes its
-    // branch from these values), and the neighbour's Y and J (faces are interior)
+    // branch from these values), and the neighbour's Y and J (interior faces)
     const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
     const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
     uint32_t xo[2];
@@ -460,26 +488,68 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     float acc[N][NV];
     {
         // face terms of both line ends: S̄ = -co·w_own + cn·w_nb with co, cn = ½·jac·∂g of
-        // the own and the neighbour's end
+        // the own and the neighbour's end (boundary faces: -co·w_own); on a boundary face
+        // the exterior state is the Dirichlet or inflow state, the own trace (outflow) or
+        // the own trace with the normal momentum reflected (slip), m - 2(m·n)n
+        // (Interfaces.jl:25-49, Equations/Euler.jl:69-94), and for the last two its
+        // cotangent is carried back to the own trace through that map
         bool bad = false;
 #pragma unroll
         for (int sd = 0; sd < 2; ++sd) {
             const int nd = base + (sd ? (N - 1) * stride : 0);
-            float G[NV], ob[NV], pw[NP];
+            float G[NV], ob[NV], xb[NV], xs[NV], pw[NP];
             load_rec<ND, NN>(sR, nd, pw);
             const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
             float nown[ND];
 #pragma unroll
             for (int k = 0; k < ND; ++k) nown[k] = s * fn[sd][k];
+            const bool bnd = xnb[sd] == -1;
             const int kp = finfo_nbpos(xinf[sd]);
             const float co = 0.5f * fj[sd] * (sd ? a.lift_p : a.lift_m);
             const float cn = 0.5f * fj[sd] * ((kp & 1) ? a.lift_p : a.lift_m);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                G[v] = fmaf(cn, xw[sd][v], -co * pw[NV + v]);
+                G[v] = bnd ? -co * pw[NV + v] : fmaf(cn, xw[sd][v], -co * pw[NV + v]);
                 ob[v] = 0.0f;
+                xb[v] = 0.0f;
+                xs[v] = xq[sd][v];
             }
-            bad |= !face_vjp<ND>(oq[sd], xq[sd], nown, G, gamma, gm1, a.diss, ob);
+            int type = -1;
+            if (bnd) {
+                const int bd = finfo_bd(xinf[sd]);
+                type = a.bctype[bd];
+                if (type == BC_DIRICHLET) {
+                    const float* src = a.bcstate + ((int64_t)a.fid[(e0 + lle) * NFC + 2 * ldir + sd] * NPF + lf) * NV;
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xs[v] = src[v];
+                } else if (type == BC_INFLOW) {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xs[v] = a.bcconst[bd * NV + v];
+                } else {
+#pragma unroll
+                    for (int v = 0; v < NV; ++v) xs[v] = oq[sd][v];
+                    if (type == BC_SLIP) {
+                        float mn = xs[1] * fn[sd][0];
+#pragma unroll
+                        for (int k = 1; k < ND; ++k) mn = fmaf(xs[1 + k], fn[sd][k], mn);
+#pragma unroll
+                        for (int k = 0; k < ND; ++k) xs[1 + k] = fmaf(-2.0f * mn, fn[sd][k], xs[1 + k]);
+                    }
+                }
+            }
+            const bool dep = type == BC_OUTFLOW || type == BC_SLIP;
+            bad |= !face_vjp<ND>(oq[sd], xs, nown, G, gamma, gm1, a.diss, ob, dep ? xb : nullptr);
+            if (dep) {
+                if (type == BC_SLIP) {
+                    float mb = xb[1] * fn[sd][0];
+#pragma unroll
+                    for (int k = 1; k < ND; ++k) mb = fmaf(xb[1 + k], fn[sd][k], mb);
+#pragma unroll
+                    for (int k = 0; k < ND; ++k) xb[1 + k] = fmaf(-2.0f * mb, fn[sd][k], xb[1 + k]);
+                }
+#pragma unroll
+                for (int v = 0; v < NV; ++v) ob[v] += xb[v];
+            }
             if (line_ok) {
 #pragma unroll
                 for (int v = 0; v < NV; ++v) sF[((lle * NFC + 2 * ldir + sd) * NV + v) * NPF + lf] = ob[v];

@@ -9,7 +9,10 @@ OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:474-536):
 * the dot-product identity <W, J V> = <J^T W, V> against the oracle's FP64 dual-number JVP
   (the oracle's rhs is pinned by tests/test_oracle.py and tests/test_curved.py);
 * the reverse RK steps (SSPRK33 and CK2N54) against the FP64 adjoint at the same stage
-  states, next to the plain FP32 path.
+  states, next to the plain FP32 path;
+* boundaries (Interfaces.jl:25-49, Equations/Euler.jl:69-94): inflow, outflow, Dirichlet
+  and slip walls on affine meshes, slip, inflow and outflow on a curved mesh with wavy
+  walls -- against the plain FP32 VJP and the oracle's JVP.
 The hand-derived reverse formulas were also checked against central differences of the
 forward fluxes in float64 (series and log branches, the LxF face flux) before the kernel
 ran.
@@ -19,7 +22,8 @@ import pytest
 
 import flouhip as F
 import oracle_lib as O
-from cases import GAMMA, oracle_for, periodic_euler_state
+from cases import GAMMA, oracle_for, periodic_euler_state, random_euler_state
+from flouhip import mesh as M
 from flouhip import adjoint as A
 from test_gpu_curved import build_curved
 
@@ -167,3 +171,105 @@ def test_fast_vjp_reverse_rk_step_against_fp64(solver):
     print(f"{'SSPRK33' if solver is F.SSPRK33 else 'CK2N54'} reverse step vs FP64, relative L2 (max): "
           f"fast {err[0][0]:.2e} ({err[0][1]:.2e}), plain FP32 {err[1][0]:.2e} ({err[1][1]:.2e})")
     assert err[0][0] <= 2e-6 and err[1][0] <= 2e-6
+
+
+def _wall_case(kind, nd, axis):
+    """One axis non-periodic with the given boundary kind, as in the fast-rhs boundary test
+    (tests/test_gpu_parity.py), FP32, with the FP64 oracle on the same boundaries."""
+    from test_gpu_parity import build
+    periodic = [1] * nd
+    periodic[axis] = 0
+    nel = [2] * nd if nd == 3 else [4, 5]
+    nel[axis] = 3
+    n = 6 if nd == 3 else 8
+    mesh, std, equation, op, orc = build(nd, n, nel, "split", "ch", periodic=periodic, finish=[2 * np.pi] * nd)
+    lo, hi = str(2 * axis + 1), str(2 * axis + 2)
+    qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7]) if nd == 3 else np.array([1.1, 0.3, -0.05, 2.7])
+    X, FX = orc.coords()
+    if kind == "inflow-outflow":
+        bcs = {lo: F.EulerInflowBC(qin), hi: F.EulerOutflowBC()}
+        orc.set_bc(0, O.BC_INFLOW, qin)
+        orc.set_bc(1, O.BC_OUTFLOW)
+    elif kind == "slip-outflow":
+        bcs = {lo: F.EulerSlipBC(), hi: F.EulerOutflowBC()}
+        orc.set_bc(0, O.BC_SLIP)
+        orc.set_bc(1, O.BC_OUTFLOW)
+    else:
+        def qext(x):
+            r = 1.0 + 0.1 * np.sin(x[1]) * np.cos(x[-1])
+            mom = [0.2 * r, 0.05 * r, -0.03 * r][:nd]
+            return np.array([r] + mom + [2.5 + 0.1 * np.cos(x[0] + x[-1])])
+
+        bcs = {lo: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x)),
+               hi: F.GenericBC(lambda _q, x, _f, _t, _e: qext(x))}
+        bc = np.zeros((orc.nf, orc.npf, nd + 2))
+        for f in range(orc.nf):
+            for k in range(orc.npf):
+                bc[f, k] = qext(FX[f, k])
+        orc.set_bc(0, O.BC_DIRICHLET, bc)
+        orc.set_bc(1, O.BC_DIRICHLET, bc)
+    disc = F.MultielementDisc(mesh, std, equation, op, bcs, precision="fp32")
+    assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
+    return disc, orc, random_euler_state(X, nd, seed=21, amp=0.1)
+
+
+@pytest.mark.parametrize("nd,axis", [(3, 0), (3, 2), (2, 1)], ids=["3d-x", "3d-z", "2d-y"])
+@pytest.mark.parametrize("kind", ["inflow-outflow", "dirichlet", "slip-outflow"])
+def test_fast_vjp_boundaries(kind, nd, axis):
+    disc, orc, Q = _wall_case(kind, nd, axis)
+    Q = Q.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(22)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
+    err = _rel2(got, ref)
+    _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(got * V))
+    dot = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"{kind} {nd}-D axis {axis}: fast vs plain FP32 VJP relative L2 {err:.2e} (max {_rel(got, ref):.2e}); "
+          f"dot product vs the oracle's JVP {dot:.2e}")
+    assert err <= 2e-6 and dot <= 2e-5
+
+
+@pytest.mark.parametrize("kind", ["slip", "inflow-outflow"])
+def test_fast_vjp_curved_wavy_walls(kind):
+    """Curved 2-D mesh with wavy walls (rotated face frames at the slip reflection)."""
+    n, ne = 6, 4
+    b = F.LagrangeBasis("GLL", n)
+    std = F.StdQuad(b, F.DGSEMrec(b), 4)
+    mesh = F.CartesianMesh(2, (0.0, 0.0), (1.0, 1.0), (ne, ne))
+    mesh.apply_periodicBCs(("1", "2"))
+
+    def wavy(X):
+        Y = X.copy()
+        Y[:, 1] = X[:, 1] + 0.08 * np.sin(2 * np.pi * X[:, 0])
+        return Y
+
+    mesh.apply_mapping(wavy)
+    eq = F.EulerEquation(2, GAMMA)
+    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    qin = np.array([1.1, 0.2, 0.1, 2.6])
+    bcs = ({"3": F.EulerSlipBC(), "4": F.EulerSlipBC()} if kind == "slip"
+           else {"3": F.EulerInflowBC(qin), "4": F.EulerOutflowBC()})
+    disc = F.MultielementDisc(mesh, std, eq, op, bcs, precision="fp32")
+    assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
+    J, met, ev, fr, fj = M.curved_geometry(mesh, std)
+    orc = O.OracleDisc(2, n, [ne, ne], [0.0, 0.0], [1.0, 1.0], [1, 0], O.build_ops(n, "GLL"), O.EQ_EULER,
+                       [GAMMA], O.VOL_SPLIT, O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+    orc.set_curved(met, J, fr.reshape(-1), fj, ev)
+    if kind == "slip":
+        orc.set_bc(0, O.BC_SLIP)
+        orc.set_bc(1, O.BC_SLIP)
+    else:
+        orc.set_bc(0, O.BC_INFLOW, qin)
+        orc.set_bc(1, O.BC_OUTFLOW)
+    Q = random_euler_state(mesh.node_coords(std), 2, seed=8, amp=0.2).astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(23)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
+    err = _rel2(got, ref)
+    _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(got * V))
+    dot = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"curved wavy {kind} walls: fast vs plain FP32 VJP relative L2 {err:.2e}; dot product vs the "
+          f"oracle's JVP {dot:.2e}")
+    assert err <= 2e-6 and dot <= 2e-5
