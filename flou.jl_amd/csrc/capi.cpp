@@ -694,7 +694,8 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
                     fws[i * n + j] = (float)(0.5 * (sij - sji));
                 }
             }
-            if (asym <= 1e-10 * smax) h->vfast = fast_vjp_launcher(nd, n, h->tp, h->curved != 0);
+            const bool walls = std::find(nb.begin(), nb.end(), -1) != nb.end();
+            if (asym <= 1e-10 * smax) h->vfast = fast_vjp_launcher(nd, n, h->tp, h->curved != 0, walls);
         }
         if (h->lfast) {
             fds.resize(n * n);

@@ -61,7 +61,8 @@ struct FastVjpArgs {
 typedef hipError_t (*FastVjpLauncher)(const FastVjpArgs&, hipStream_t);
 
 // 2-D and 3-D Euler split form with the Chandrasekhar two-point flux, n = 4..8, on the
-// fast path's elements (curved, or axis-aligned affine) and boundaries; nullptr otherwise.
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved);
+// fast path's elements (curved, or axis-aligned affine); walls: the mesh has boundary
+// faces; nullptr otherwise.
+FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved, bool walls);
 
 }  // namespace flouhip

@@ -316,8 +316,9 @@ __device__ __forceinline__ bool face_vjp(const float* o, const float* x, const f
 
 // CURV: per-node metric and J (ngeom) and per-face-node normal and surface J (fnj); else
 // axis-aligned affine elements: per-element Ja_dd/J and 1/J (egeom, stride 0 when every
-// element is alike) and one frame and surface J per face (fgeom).
-template <int ND, int N, bool CURV>
+// element is alike) and one frame and surface J per face (fgeom).  WALLS: the mesh has
+// boundary faces (without, the boundary code is compiled out).
+template <int ND, int N, bool CURV, bool WALLS>
 __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
     using C = VCfg<ND, N>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
             float nown[ND];
 #pragma unroll
             for (int k = 0; k < ND; ++k) nown[k] = s * fn[sd][k];
-            const bool bnd = xnb[sd] == -1;
+            const bool bnd = WALLS && xnb[sd] == -1;
             const int kp = finfo_nbpos(xinf[sd]);
             const float co = 0.5f * fj[sd] * (sd ? a.lift_p : a.lift_m);
             const float cn = 0.5f * fj[sd] * ((kp & 1) ? a.lift_p : a.lift_m);
@@ -537,7 +538,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
                     }
                 }
             }
-            const bool dep = type == BC_OUTFLOW || type == BC_SLIP;
+            const bool dep = WALLS && (type == BC_OUTFLOW || type == BC_SLIP);
             bad |= !face_vjp<ND>(oq[sd], xs, nown, G, gamma, gm1, a.diss, ob, dep ? xb : nullptr);
             if (dep) {
                 if (type == BC_SLIP) {
@@ -672,32 +673,38 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     if (err) atomicOr(a.errflag, err);
 }
 
-template <int ND, int N, bool CURV>
+template <int ND, int N, bool CURV, bool WALLS>
 hipError_t launch_fvjp(const FastVjpArgs& a, hipStream_t s) {
     using C = VCfg<ND, N>;
     const int nb = (a.f.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_fvjp<ND, N, CURV>), dim3(nb), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_fvjp<ND, N, CURV, WALLS>), dim3(nb), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
-template <int ND, bool CURV>
+template <int ND, bool CURV, bool WALLS>
 FastVjpLauncher fvjp_n(int n) {
     switch (n) {
-        case 4: return &launch_fvjp<ND, 4, CURV>;
-        case 5: return &launch_fvjp<ND, 5, CURV>;
-        case 6: return &launch_fvjp<ND, 6, CURV>;
-        case 7: return &launch_fvjp<ND, 7, CURV>;
-        case 8: return &launch_fvjp<ND, 8, CURV>;
+        case 4: return &launch_fvjp<ND, 4, CURV, WALLS>;
+        case 5: return &launch_fvjp<ND, 5, CURV, WALLS>;
+        case 6: return &launch_fvjp<ND, 6, CURV, WALLS>;
+        case 7: return &launch_fvjp<ND, 7, CURV, WALLS>;
+        case 8: return &launch_fvjp<ND, 8, CURV, WALLS>;
         default: return nullptr;
     }
 }
 
+template <int ND>
+FastVjpLauncher fvjp_nd(int n, bool curved, bool walls) {
+    if (curved) return walls ? fvjp_n<ND, true, true>(n) : fvjp_n<ND, true, false>(n);
+    return walls ? fvjp_n<ND, false, true>(n) : fvjp_n<ND, false, false>(n);
+}
+
 }  // namespace fast
 
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved) {
+FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved, bool walls) {
     if (tp != TP_CHANDRASEKHAR) return nullptr;
-    if (nd == 2) return curved ? fast::fvjp_n<2, true>(n) : fast::fvjp_n<2, false>(n);
-    if (nd == 3) return curved ? fast::fvjp_n<3, true>(n) : fast::fvjp_n<3, false>(n);
+    if (nd == 2) return fast::fvjp_nd<2>(n, curved, walls);
+    if (nd == 3) return fast::fvjp_nd<3>(n, curved, walls);
     return nullptr;
 }
 
