@@ -245,6 +245,44 @@ SECONDARY = {
 }
 
 
+def _c2_prim(X, g):
+    """C2's seeded primitive state (ρ, u, v, p) at the node coordinates X (2, nodes):
+    rho = 1 + 0.2 sin(π(x+y)) + 8 Fourier modes of 0.01, u, v = 0.5 + 0.05 smooth, p = 1."""
+    import torch
+    x, y = X[0], X[1]
+    rho = 1 + 0.2 * torch.sin(math.pi * (x + y))
+    for _ in range(8):
+        kx, ky = torch.randint(1, 4, (2,), generator=g).tolist()
+        rho = rho + 0.01 / 8 * torch.sin(math.pi * (kx * x + ky * y) + float(torch.rand(1, generator=g)) * 6.283)
+    ph = (torch.rand(2, generator=g) * 6.283).tolist()
+    u = 0.5 + 0.05 * torch.sin(math.pi * x + ph[0]) * torch.cos(math.pi * y)
+    v = 0.5 + 0.05 * torch.cos(math.pi * x) * torch.sin(math.pi * y + ph[1])
+    return torch.stack([rho, u, v, torch.ones_like(x)])
+
+
+def _c2_disc(form):
+    """The C2 discretisation (periodic 256^2 quads on [0,2]^2, N=7, split Kennedy-Gruber +
+    LxF(StdAverage)) in the given precision and its seeded FP64 state."""
+    import torch
+    import flouhip as F
+    b = F.LagrangeBasis("GLL", 8)
+    std = F.StdQuad(b, F.DGSEMrec(b), 4)
+    eq = F.EulerEquation(2, GAMMA)
+    mesh = F.CartesianMesh(2, (0.0, 0.0), (2.0, 2.0), (256, 256))
+    mesh.apply_periodicBCs(("1", "2"), ("3", "4"))
+    op = F.SplitDivOperator(F.KennedyGruber(), F.LxF(F.StdAverage(), 1.0))
+    disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
+    X = torch.tensor(mesh.node_coords(std), device="cuda").T
+    g = torch.Generator(device="cpu")
+    g.manual_seed(7)
+    prim = _c2_prim(X, g)
+    Q64 = torch.empty_like(prim)
+    Q64[0] = prim[0]
+    Q64[1:3] = prim[0] * prim[1:3]
+    Q64[3] = prim[3] / (GAMMA - 1) + prim[0] * (prim[1:3] ** 2).sum(0) / 2
+    return disc, eq, Q64
+
+
 def secondary_workload(name, steps=None, warmup=2):
     """C2: 2-D Euler on a periodic 256^2 quad mesh over [0,2]^2, N=7, split Kennedy-Gruber +
     LxF(StdAverage), SSPRK33 (CFL 0.1 of the initial state), seeded state rho = 1 +
@@ -289,15 +327,7 @@ def secondary_workload(name, steps=None, warmup=2):
             g = torch.Generator(device="cpu")
             g.manual_seed(7)
             if name == "c2":
-                x, y = X[0], X[1]
-                rho = 1 + 0.2 * torch.sin(math.pi * (x + y))
-                for _ in range(8):
-                    kx, ky = torch.randint(1, 4, (2,), generator=g).tolist()
-                    rho = rho + 0.01 / 8 * torch.sin(math.pi * (kx * x + ky * y) + float(torch.rand(1, generator=g)) * 6.283)
-                ph = (torch.rand(2, generator=g) * 6.283).tolist()
-                u = 0.5 + 0.05 * torch.sin(math.pi * x + ph[0]) * torch.cos(math.pi * y)
-                v = 0.5 + 0.05 * torch.cos(math.pi * x) * torch.sin(math.pi * y + ph[1])
-                prim = torch.stack([rho, u, v, torch.ones_like(x)])
+                prim = _c2_prim(X, g)
             else:
                 ph = (torch.rand(5, 3, generator=g) * 6.283).tolist()
                 modes = [torch.sin(2 * math.pi * X[0] + ph[v][0]) * torch.sin(2 * math.pi * X[1] + ph[v][1])
@@ -390,15 +420,18 @@ def vjp_kernel(name, reps=20):
     """An FP32 rhs VJP alone (profiling run of the fast adjoint kernel): reps calls of
     flou_hip_rhs_vjp at the workload's state with a seeded cotangent, and reps fast rhs
     evaluations at the same state, timed with HIP events on the handle's stream.
-    c5: the C5 mesh and state; c3: the C3 (64^3, N=5, Taylor-Green) mesh and state.
+    c5: the C5 mesh and state; c3: the C3 (64^3, N=5, Taylor-Green) mesh and state; c2: the
+    C2 (2-D 256^2, N=7, Kennedy-Gruber) mesh and state.
     Algorithmic bytes of one VJP call per node: Q, Y and the accumulated Qbar read
-    (3 x 20 B), Qbar written (20 B), and on the curved mesh the metric and J (40 B)."""
+    (3 x NV x 4 B), Qbar written (NV x 4 B), and on the curved mesh the metric and J (40 B)."""
     import torch
     import flouhip as F
     from flouhip import adjoint as A
     if name == "c5":
         std, eq, build = _c5_case()
         disc, Q64 = build("fp32")
+    elif name == "c2":
+        disc, eq, Q64 = _c2_disc("fp32")
     else:
         disc, eq, std = build_disc("fp32")
         Q64 = taylor_green_state(disc, std)
@@ -426,12 +459,12 @@ def vjp_kernel(name, reps=20):
     torch.cuda.synchronize()
     disc.check()
     vms, fms = ev[0].elapsed_time(ev[1]) / reps, ev[1].elapsed_time(ev[2]) / reps
-    nn = Q.shape[1]
-    vbytes = nn * (3 * 20 + 20 + (40 if name == "c5" else 0))
+    nv, nn = Q.shape
+    vbytes = nn * (4 * nv * 4 + (40 if name == "c5" else 0))
+    label = {"c5": "C5 curved 32^3 N=6", "c3": "C3 64^3 N=5 Taylor-Green", "c2": "C2 2-D 256^2 N=7 Kennedy-Gruber"}
     return {"vjp_ms": vms, "rhs_ms": fms, "vjp_over_rhs": vms / fms, "nodes": nn,
             "vjp_algorithmic_bytes": vbytes, "vjp_hbm_fraction": vbytes / (vms * 1e-3) / 8e12,
-            "workload": ("C5 curved 32^3 N=6" if name == "c5" else "C3 64^3 N=5 Taylor-Green")
-                        + " FP32 (the bench's mesh and state), seeded cotangent; fast adjoint kernel vs fast rhs"}
+            "workload": label[name] + " FP32 (the bench's mesh and state), seeded cotangent; fast adjoint kernel vs fast rhs"}
 
 
 def c5_adjoint(nsteps=100, every=10):
@@ -554,7 +587,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp", "c3vjp"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp", "c3vjp", "c2vjp"],
                     default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
@@ -578,7 +611,7 @@ def main():
     if args.only == "c5adj":  # the C5 adjoint alone
         print(json.dumps({"c5_adjoint": c5_adjoint()}), flush=True)
         return
-    if args.only in ("c5vjp", "c3vjp"):  # profiling run of an FP32 adjoint kernel alone
+    if args.only in ("c5vjp", "c3vjp", "c2vjp"):  # profiling run of an FP32 adjoint kernel alone
         print(json.dumps({args.only[:2] + "_vjp": vjp_kernel(args.only[:2])}), flush=True)
         return
     if args.only in ("c2", "c5"):  # profiling run of one secondary workload alone

@@ -34,8 +34,10 @@ EXACT = ["-ffp-contract=off"]
 def units():
     u = [("capi.o", "capi.cpp", EXACT), ("reduce.o", "reduce.hip", EXACT), ("limiter.o", "limiter.hip", EXACT),
          ("output.o", "output.hip", []),
-         ("rhs_fast.o", "rhs_fast.hip", ["-ffp-contract=fast"]),
-         ("vjp_fast.o", "vjp_fast.hip", ["-ffp-contract=fast"])]
+         ("rhs_fast.o", "rhs_fast.hip", ["-ffp-contract=fast"])]
+    # the fast adjoint, one unit per two-point flux (StdAverage, Chandrasekhar, Kennedy-Gruber)
+    for tp, name in ((0, "std"), (1, "ch"), (2, "kg")):
+        u.append((f"vjp_fast_{name}.o", "vjp_fast.hip", ["-ffp-contract=fast", f"-DFLOUHIP_FVJP_TP={tp}"]))
     for prec in ("F64", "F32"):
         for nd in (1, 2, 3):
             u.append((f"rhs_plain_{prec.lower()}_{nd}d.o", "rhs_plain.hip",

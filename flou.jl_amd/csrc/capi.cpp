@@ -680,7 +680,7 @@ int flou_hip_create(const flou_hip_desc* d, flou_hip_handle** out) {
             }
         }
         if (h->lfast && h->v32) {
-            // the FP32 adjoint on this path (Chandrasekhar volume flux):
+            // the FP32 adjoint on this path (any of the three two-point fluxes):
             // S_ij = ω_i D♯_ij, antisymmetric off the zero diagonal (SBP on GLL nodes)
             double smax = 0.0, asym = 0.0;
             fws.resize(n * n);

@@ -60,9 +60,25 @@ struct FastVjpArgs {
 
 typedef hipError_t (*FastVjpLauncher)(const FastVjpArgs&, hipStream_t);
 
-// 2-D and 3-D Euler split form with the Chandrasekhar two-point flux, n = 4..8, on the
-// fast path's elements (curved, or axis-aligned affine); walls: the mesh has boundary
-// faces; nullptr otherwise.
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved, bool walls);
+namespace fast {
+// the kernels of one two-point flux (one translation unit each, which instantiates its own)
+template <int TP>
+FastVjpLauncher fvjp_tp(int nd, int n, bool curved, bool walls);
+extern template FastVjpLauncher fvjp_tp<0>(int, int, bool, bool);
+extern template FastVjpLauncher fvjp_tp<1>(int, int, bool, bool);
+extern template FastVjpLauncher fvjp_tp<2>(int, int, bool, bool);
+}  // namespace fast
+
+// 2-D and 3-D Euler split form with the StdAverage (tp 0), Chandrasekhar (1) or
+// Kennedy-Gruber (2) two-point flux, n = 4..8, on the fast path's elements (curved, or
+// axis-aligned affine); walls: the mesh has boundary faces; nullptr otherwise.
+inline FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved, bool walls) {
+    switch (tp) {
+        case 0: return fast::fvjp_tp<0>(nd, n, curved, walls);
+        case 1: return fast::fvjp_tp<1>(nd, n, curved, walls);
+        case 2: return fast::fvjp_tp<2>(nd, n, curved, walls);
+        default: return nullptr;
+    }
+}
 
 }  // namespace flouhip

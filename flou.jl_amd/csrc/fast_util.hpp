@@ -37,5 +37,11 @@ __device__ __forceinline__ int wave_path(bool first_order, bool series) {
     return __all(first_order) ? 2 : (__all(series) ? 1 : 0);
 }
 
+// slot-array position of block node j (PAD: rows of N nodes, one slot between rows)
+template <int N, bool PAD>
+__device__ __forceinline__ int pnode(int j) {
+    return PAD ? j + j / N : j;
+}
+
 }  // namespace fast
 }  // namespace flouhip

@@ -271,11 +271,6 @@ __device__ __forceinline__ void store_line(float* __restrict__ R, const LineAddr
     }
 }
 
-// slot-array position of block node j (PAD: rows of N nodes, one float between rows)
-template <int N, bool PAD>
-__device__ __forceinline__ int pnode(int j) {
-    return PAD ? j + j / N : j;
-}
 
 // line-frame slot of physical variable v in a line of direction d (momenta cycle)
 template <int ND>

@@ -1,25 +1,31 @@
 // FP32 adjoint of the fast path: the vector-Jacobian product of the rhs of the
-// split-form Euler operator with the Chandrasekhar two-point flux and the LxF(StdAverage)
-// or StdAverage surface flux on GLL nodes (SplitDivOperator, interface_fluxes!, the FR
-// lift and 1/J: OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:
-// 99-165, 474-536, MultielementDiscontinuous.jl:132-137), fused with the reverse RK stage
-// epilogue (FastVjpArgs).  The same mathematics as k_vjp (vjp_plain.hip), organised like
+// split-form Euler operator with the Chandrasekhar, Kennedy-Gruber or StdAverage
+// two-point flux and the LxF(StdAverage) or StdAverage surface flux on GLL nodes
+// (SplitDivOperator, interface_fluxes!, the FR lift and 1/J: OpDivergence.jl:200-299,
+// Interfaces.jl:111-136, Equations/Euler.jl:99-165, 385-536,
+// MultielementDiscontinuous.jl:132-137), fused with the reverse RK stage epilogue
+// (FastVjpArgs).  The same mathematics as k_vjp (vjp_plain.hip), organised like
 // the forward fast kernel k_fastc (rhs_fast.hip):
 //   * EPB elements per block, one thread per (element, direction, line);
 //   * staging: per node Q, Y, the metric and J from HBM; to LDS one record of 16-byte
-//     vectors per node with the halved primitives a = ρ/2, v = u/2, β = ρ/(2p) and
-//     w = c·Y/J, and the metric vector Ja^d/2 of each direction (a partner node costs
-//     four 128-bit reads, and eight consecutive lanes fall on distinct banks for lines of
-//     every direction); each line thread also loads the own and the neighbour's trace
-//     and the neighbour's Y/J at its two line ends;
+//     vectors per node with the flux's node inputs and w = c·Y/J, and the metric vector
+//     Ja^d/2 of each direction (a partner node costs four 128-bit reads, and eight
+//     consecutive lanes fall on distinct banks for lines of every direction); node
+//     inputs: Chandrasekhar (a, v.., β) = (ρ/2, u/2, ρ/(2p)), Kennedy-Gruber
+//     (a, v.., s, e) = (ρ/2, u/2, p/2, E/(2ρ)), StdAverage (u.., H, |u|²) with
+//     H = (E + p)/ρ; each line thread also loads the own and the neighbour's trace and
+//     the neighbour's Y/J at its two line ends;
 //   * line phase: the reverse product of the face flux at both line ends with respect to
 //     the own trace (conservative variables, into a face array in LDS), then every pair
 //     of the line once.  Off the (zero) diagonal of D♯ on GLL nodes S_ij = ω_i D♯_ij is
 //     antisymmetric, so the cotangent of the pair flux F♯(i, j), -D♯_ij w_i - D♯_ji w_j,
 //     is S_ij (w_j/ω_j - w_i/ω_i); its reverse product is accumulated for both nodes in
-//     registers, as cotangents of (a, v, β);
-//   * per node: Σ_d of the line rows, one pullback (a, v, β) -> (ρ, m, E), the face terms
-//     of the node's faces, then the epilogue
+//     registers, as cotangents of the node inputs (StdAverage: of (ρ, m, E) directly; on
+//     affine elements, where n̄ is constant along a line, the reverse product is linear
+//     in the pair cotangent, so the line accumulates Σ_j S_ij (w_j/ω_j - w_i/ω_i) per
+//     node and applies one reverse product per node);
+//   * per node: Σ_d of the line rows, one pullback of the node inputs to (ρ, m, E), the
+//     face terms of the node's faces, then the epilogue
 //       o0 = a0·P0 + a1·P1 + V,  o1 = b0·o0 + b1·P1 + b2·P2  (o1 optional).
 // Curved elements (per-node metric and J) and axis-aligned affine ones (per-element
 // metric); faces of orientation 0; interior faces and Dirichlet, inflow, outflow and slip
@@ -37,7 +43,7 @@ constexpr int fvjp_epb() {
                    : (N == 4 ? 16 : (N == 5 ? 32 : (N == 6 ? 16 : (N == 7 ? 9 : 8))));
 }
 
-template <int ND, int N>
+template <int ND, int N, int TP>
 struct VCfg {
     static constexpr int NV = ND + 2, NFC = 2 * ND;
     static constexpr int NPE = ipow(N, ND), NPF = ipow(N, ND - 1), NL = ND * NPF;
@@ -45,16 +51,24 @@ struct VCfg {
     static constexpr int TPB = ((EPB * NL + 63) / 64) * 64;
     static constexpr int NN = EPB * NPE;
     static constexpr int NPT = (NN + TPB - 1) / TPB;
-    // LDS: per node NR 16-byte records of (a, v.., β, w..) ([NR][NN] float4), then per
+    // node arrays in LDS hold block node j at pnode(j) = j + j/N (one slot after every row
+    // of N nodes) for even N: the lanes of x-lines, N nodes apart, then read 16-byte records
+    // an odd number of records apart (distinct banks for eight consecutive lanes)
+    // (not at 3-D N=6, where the larger arrays would hold a CU to two blocks instead of three)
+    static constexpr bool PAD = N % 2 == 0 && !(ND == 3 && N == 6);
+    static constexpr int NNP = PAD ? NN + NN / N : NN;
+    // NA node inputs of the flux (their cotangents are the line rows), then w (NV)
+    static constexpr int NA = TP == TP_KENNEDY_GRUBER ? ND + 3 : ND + 2;
+    // LDS: per node NR 16-byte records of (inputs.., w..) ([NR][NN] float4), then per
     // direction the metric vector Ja^d/2 ([ND][NN] of MW floats).  The row sums of the ND
-    // directions ([ND*NV][NN] floats) go over them once the lines are done.  Then the face
+    // directions ([ND*NA][NN] floats) go over them once the lines are done.  Then the face
     // cotangents [EPB][2ND][NV][NPF].
-    static constexpr int PW = 2 * NV, NR = (PW + 3) / 4, MW = ND == 3 ? 4 : 2;
+    static constexpr int PW = NA + NV, NR = (PW + 3) / 4, NP = 4 * NR, MW = ND == 3 ? 4 : 2;
     static constexpr int OFF_R = 0;
-    static constexpr int OFF_M = 4 * NR * NN;
-    static constexpr int OFF_F = OFF_M + ND * MW * NN;
+    static constexpr int OFF_M = 4 * NR * NNP;
+    static constexpr int OFF_F = OFF_M + ND * MW * NNP;
     static constexpr int LDS = r4(OFF_F + EPB * NFC * NV * NPF);
-    static_assert(ND * NV * NN <= OFF_F, "row sums must fit in the staging area");
+    static_assert((ND + 1) * NA * NNP <= OFF_F, "the rows (and an unread slot) must fit in the staging area");
     static_assert(EPB * NL >= 64, "lanes past the line tasks repeat a line of the block");
 };
 
@@ -167,10 +181,87 @@ __device__ __forceinline__ void pair_vjp(float ai, float aj, const float* vi, co
     }
 }
 
-// Node j's record (a, v.., β, w..) from the NR float4 arrays at sR.
-template <int ND, int NN>
-__device__ __forceinline__ void load_rec(const float4* __restrict__ sR, int j, float (&pw)[VCfg<ND, 4>::NR * 4]) {
-    constexpr int NR = VCfg<ND, 4>::NR;
+// Reverse product of the Kennedy-Gruber two-point flux along n̄ = M_i + M_j (Kennedy &
+// Gruber 2008, the form of the forward kernels) for the flux cotangent G, as cotangents of
+// the halved node inputs (a, v.., s, e) = (ρ/2, u/2, p/2, E/(2ρ)) at p_i, p_j.  Every
+// input enters through the sum of the two nodes' values, so both nodes take the same
+// cotangent, scaled by s and added to gi and gj.  Forward:
+//   ρ̄ = a_i + a_j, ū = v_i + v_j, p̄ = s_i + s_j, ē = e_i + e_j, m = ρ̄ ū·n̄,
+//   F♯ = (m, m ū + p̄ n̄, m ē + p̄ ū·n̄).
+template <int ND>
+__device__ __forceinline__ void pair_kg(const float* pi, const float* pj, const float* Mi, const float* Mj,
+                                        const float* G, float s, float* gi, float* gj) {
+    constexpr int NV = ND + 2;
+    float n[ND], ub[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) {
+        n[k] = Mi[k] + Mj[k];
+        ub[k] = pi[1 + k] + pj[1 + k];
+    }
+    float un = ub[0] * n[0], gu = G[1] * ub[0], gn = G[1] * n[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        un = fmaf(ub[k], n[k], un);
+        gu = fmaf(G[1 + k], ub[k], gu);
+        gn = fmaf(G[1 + k], n[k], gn);
+    }
+    const float rho = pi[0] + pj[0], p = pi[ND + 1] + pj[ND + 1], e = pi[ND + 2] + pj[ND + 2];
+    const float m = rho * un;
+    const float mb = fmaf(G[NV - 1], e, G[0] + gu);    // m̄
+    const float unb = fmaf(mb, rho, G[NV - 1] * p);    // of ū·n̄
+    float t[NV + 1];
+    t[0] = mb * un;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) t[1 + k] = fmaf(m, G[1 + k], unb * n[k]);
+    t[ND + 1] = fmaf(G[NV - 1], un, gn);
+    t[ND + 2] = G[NV - 1] * m;
+#pragma unroll
+    for (int c = 0; c < NV + 1; ++c) {
+        gi[c] = fmaf(s, t[c], gi[c]);
+        gj[c] = fmaf(s, t[c], gj[c]);
+    }
+}
+
+// Adds s·(∂(f(Q)·n)/∂Q)^T G to g for one node Q given by u, H = (E + p)/ρ and |u|²
+// (Equations/Euler.jl:385-472, one side of the StdAverage flux); gn = Σ_k G_{1+k} n_k.
+//   f·n = (m·n, m (u·n) + p n, (E + p) u·n)
+template <int ND>
+__device__ __forceinline__ void std_side(const float* u, float H, float uu, const float* n, const float* G, float gn,
+                                         float gm1, float s, float* g) {
+    constexpr int NV = ND + 2;
+    float un = u[0] * n[0], U = fmaf(G[1], u[0], G[NV - 1] * H);
+#pragma unroll
+    for (int k = 1; k < ND; ++k) {
+        un = fmaf(u[k], n[k], un);
+        U = fmaf(G[1 + k], u[k], U);
+    }
+    const float pg = gm1 * fmaf(G[NV - 1], un, gn);  // (γ-1) × cotangent of p
+    g[0] = fmaf(s, fmaf(-U, un, 0.5f * pg * uu), g[0]);
+    const float t = G[0] + U;
+#pragma unroll
+    for (int k = 0; k < ND; ++k) g[1 + k] = fmaf(s, fmaf(G[1 + k], un, fmaf(t, n[k], -pg * u[k])), g[1 + k]);
+    g[NV - 1] = fmaf(s, fmaf(G[NV - 1], un, pg), g[NV - 1]);
+}
+
+// Reverse product of the StdAverage two-point flux ½(f(Q_i) + f(Q_j))·n̄, n̄ = M_i + M_j
+// (Equations/Euler.jl:385-472) for the cotangent G: each node's own side, scaled by s, in
+// conservative variables, from the node inputs (u.., H, |u|²) at p_i, p_j.
+template <int ND>
+__device__ __forceinline__ void pair_std(const float* pi, const float* pj, const float* Mi, const float* Mj,
+                                         const float* G, float s, float gm1, float* gi, float* gj) {
+    float n[ND];
+#pragma unroll
+    for (int k = 0; k < ND; ++k) n[k] = Mi[k] + Mj[k];
+    float gn = G[1] * n[0];
+#pragma unroll
+    for (int k = 1; k < ND; ++k) gn = fmaf(G[1 + k], n[k], gn);
+    std_side<ND>(pi, pi[ND], pi[ND + 1], n, G, gn, gm1, 0.5f * s, gi);
+    std_side<ND>(pj, pj[ND], pj[ND + 1], n, G, gn, gm1, 0.5f * s, gj);
+}
+
+// Node j's record (inputs.., w..) from the NR float4 arrays at sR.
+template <int NR, int NN>
+__device__ __forceinline__ void load_rec(const float4* __restrict__ sR, int j, float (&pw)[NR * 4]) {
 #pragma unroll
     for (int r = 0; r < NR; ++r) {
         const float4 t = sR[r * NN + j];
@@ -198,38 +289,74 @@ __device__ __forceinline__ void load_metric(const float* __restrict__ sMd, int j
 
 // All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
 // j's read from LDS (records at sR, the metric vector of the line's direction at sMd), w
-// scaled by 1/ω of the node's position on the line.
-template <int ND, int N, int NN, int PATH>
+// scaled by 1/ω of the node's position on the line.  LIN (StdAverage on affine elements,
+// n̄ = 2 Ja/2 the same for every pair of the line): the pairs only accumulate
+// Σ_j S_ij (w_j/ω_j - w_i/ω_i), and each node then takes the reverse product of
+// ½ f(Q)·n̄ = f(Q)·(Ja/2) for that sum.
+template <int ND, int N, int NN, int TP, int PATH, bool LIN>
 __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const float* __restrict__ sMd, int base,
                                          int stride, const float* __restrict__ iw, const float* __restrict__ wsh,
-                                         float ig1, float (&acc)[N][ND + 2]) {
-    constexpr int NV = ND + 2, NP = VCfg<ND, 4>::NR * 4;
+                                         float ig1, float gm1, float (&acc)[N][VCfg<ND, N, TP>::NA]) {
+    using C = VCfg<ND, N, TP>;
+    constexpr int NV = ND + 2, NP = C::NP, NR = C::NR, W0 = C::NA;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const int ni = base + i * stride;
         float pi[NP], Mi[ND], wi[NV];
-        load_rec<ND, NN>(sR, ni, pi);
-        load_metric<ND>(sMd, ni, Mi);
+        load_rec<NR, NN>(sR, ni, pi);
+        if (!LIN) load_metric<ND>(sMd, ni, Mi);
         const float iwi = iw[i];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) wi[v] = pi[NV + v] * iwi;
-        const float lai = PATH ? 0.0f : lg(pi[0]), lbi = PATH ? 0.0f : lg(pi[ND + 1]);
+        for (int v = 0; v < NV; ++v) wi[v] = pi[W0 + v] * iwi;
+        const bool lgs = TP == TP_CHANDRASEKHAR && PATH == 0;
+        const float lai = lgs ? lg(pi[0]) : 0.0f, lbi = lgs ? lg(pi[ND + 1]) : 0.0f;
 #pragma unroll
         for (int j = i + 1; j < N; ++j) {
             const int nj = base + j * stride;
             float pj[NP], Mj[ND], G[NV];
-            load_rec<ND, NN>(sR, nj, pj);
-            load_metric<ND>(sMd, nj, Mj);
+            load_rec<NR, NN>(sR, nj, pj);
+            if (!LIN) load_metric<ND>(sMd, nj, Mj);
             const float iwj = iw[j];
 #pragma unroll
-            for (int v = 0; v < NV; ++v) G[v] = fmaf(pj[NV + v], iwj, -wi[v]);
-            const float laj = PATH ? 0.0f : lg(pj[0]), lbj = PATH ? 0.0f : lg(pj[ND + 1]);
-            pair_vjp<ND, PATH>(pi[0], pj[0], pi + 1, pj + 1, pi[ND + 1], pj[ND + 1], lai, laj, lbi, lbj, Mi, Mj, G,
-                              wsh[i * N + j], ig1, acc[i], acc[j]);
+            for (int v = 0; v < NV; ++v) G[v] = fmaf(pj[W0 + v], iwj, -wi[v]);
+            const float s = wsh[i * N + j];
+            if constexpr (TP == TP_CHANDRASEKHAR) {
+                const float laj = lgs ? lg(pj[0]) : 0.0f, lbj = lgs ? lg(pj[ND + 1]) : 0.0f;
+                pair_vjp<ND, PATH>(pi[0], pj[0], pi + 1, pj + 1, pi[ND + 1], pj[ND + 1], lai, laj, lbi, lbj, Mi, Mj,
+                                   G, s, ig1, acc[i], acc[j]);
+            } else if constexpr (TP == TP_KENNEDY_GRUBER) {
+                pair_kg<ND>(pi, pj, Mi, Mj, G, s, acc[i], acc[j]);
+            } else if constexpr (LIN) {
+#pragma unroll
+                for (int v = 0; v < NV; ++v) {
+                    acc[i][v] = fmaf(s, G[v], acc[i][v]);
+                    acc[j][v] = fmaf(s, G[v], acc[j][v]);
+                }
+            } else {
+                pair_std<ND>(pi, pj, Mi, Mj, G, s, gm1, acc[i], acc[j]);
+            }
             // keep each pair's loads and arithmetic together: the registers then hold the
             // line's rows, node i and one pair (the scheduler would otherwise hoist the
             // loads of every pair and spill)
             __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    if constexpr (LIN) {
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+            const int np = base + p * stride;
+            float pp[NP], Mp[ND], G[NV];
+            load_rec<NR, NN>(sR, np, pp);
+            load_metric<ND>(sMd, np, Mp);
+            float gn = acc[p][1] * Mp[0];
+#pragma unroll
+            for (int k = 1; k < ND; ++k) gn = fmaf(acc[p][1 + k], Mp[k], gn);
+#pragma unroll
+            for (int v = 0; v < NV; ++v) {
+                G[v] = acc[p][v];
+                acc[p][v] = 0.0f;
+            }
+            std_side<ND>(pp, pp[ND], pp[ND + 1], Mp, G, gn, gm1, 1.0f, acc[p]);
         }
     }
 }
@@ -317,16 +444,16 @@ __device__ __forceinline__ bool face_vjp(const float* o, const float* x, const f
 // CURV: per-node metric and J (ngeom) and per-face-node normal and surface J (fnj); else
 // axis-aligned affine elements: per-element Ja_dd/J and 1/J (egeom, stride 0 when every
 // element is alike) and one frame and surface J per face (fgeom).  WALLS: the mesh has
-// boundary faces (without, the boundary code is compiled out).
-template <int ND, int N, bool CURV, bool WALLS>
-__global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
-    using C = VCfg<ND, N>;
+// boundary faces (without, the boundary code is compiled out).  TP: the two-point flux.
+template <int ND, int N, int TP, bool CURV, bool WALLS>
+__global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
+    using C = VCfg<ND, N, TP>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
-    constexpr int TPB = C::TPB, NN = C::NN, NPT = C::NPT;
+    constexpr int TPB = C::TPB, NN = C::NN, NNP = C::NNP, NPT = C::NPT;
     const FastArgs& a = A.f;
     __shared__ float4 smem[(C::LDS + 3) / 4];
     float* const sm = reinterpret_cast<float*>(smem);
-    constexpr int NR = C::NR, NP = 4 * NR, MW = C::MW;
+    constexpr int NR = C::NR, NP = C::NP, MW = C::MW, NA = C::NA;
     float4* const sR = reinterpret_cast<float4*>(sm + C::OFF_R);
     float* const sM = sm + C::OFF_M;
     float* const sF = sm + C::OFF_F;
@@ -340,9 +467,11 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     int err = 0;
 
     // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
+    // direction-major: consecutive lanes run the lines of one direction over the block's
+    // elements, so a wave reads records with one stride where EPB·NPF allows it
     const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
-    const int lle = tl / NL, lt = tl - (tl / NL) * NL;
-    const int ldir = lt / NPF, lf = lt - (lt / NPF) * NPF;
+    const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
+    const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_ok = tid < EPB * NL && lle < nel;
     int xnb[2], xinf[2], xfg[2];
 #pragma unroll
@@ -407,8 +536,11 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     // both traces at both line ends, exactly as the forward reads them (the own end node
     // and the neighbour's, also inside the block: the LxF wave-speed maximum takes its
     // branch from these values), and the neighbour's Y and J (interior faces)
-    const int base = lle * NPE + line_node<ND, N>(ldir, lf, 0);
-    const int stride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    // the line's first node and node step in the block (gbase, gstride) and in LDS
+    const int gbase = lle * NPE + line_node<ND, N>(ldir, lf, 0);
+    const int gstride = ldir == 0 ? 1 : (ldir == 1 ? N : NPF);
+    const int base = pnode<N, C::PAD>(gbase);
+    const int stride = C::PAD ? (ldir == 0 ? 1 : (ldir == 1 ? N + 1 : N * (N + 1))) : gstride;
     uint32_t xo[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
@@ -426,7 +558,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
         for (int sd = 0; sd < 2; ++sd) {
             xq[sd][v] = bload(rv, xo[sd]);
             xw[sd][v] = bload(ryv, xo[sd]);
-            oq[sd][v] = bload(rq, (uint32_t)(base + (sd ? (N - 1) * stride : 0)) * 4u);
+            oq[sd][v] = bload(rq, (uint32_t)(gbase + (sd ? (N - 1) * gstride : 0)) * 4u);
         }
     }
     if constexpr (CURV) {
@@ -443,6 +575,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
         if (j >= NN) continue;
+        const int pj = pnode<N, C::PAD>(j);
         const float rho = q[r][0];
         const float ir = rcp(rho);
         float u[ND];
@@ -456,25 +589,37 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
         if (j < nel * NPE && (!(rho > 0.0f) || !(p > 0.0f) || !(J > 0.0f))) err |= ERR_LOG_NEG;
         const float cj = A.c * rcp(J);
         float pw[NP];
-        pw[0] = 0.5f * rho;
+        if constexpr (TP == TP_STDAVG) {
 #pragma unroll
-        for (int k = 0; k < ND; ++k) pw[1 + k] = 0.5f * u[k];
-        pw[ND + 1] = 0.5f * rho * rcp(p);
+            for (int k = 0; k < ND; ++k) pw[k] = u[k];
+            pw[ND] = (q[r][NV - 1] + p) * ir;
+            pw[ND + 1] = uu;
+        } else {
+            pw[0] = 0.5f * rho;
 #pragma unroll
-        for (int c = 0; c < NV; ++c) pw[NV + c] = cj * y[r][c];
+            for (int k = 0; k < ND; ++k) pw[1 + k] = 0.5f * u[k];
+            if constexpr (TP == TP_CHANDRASEKHAR) {
+                pw[ND + 1] = 0.5f * rho * rcp(p);
+            } else {
+                pw[ND + 1] = 0.5f * p;
+                pw[ND + 2] = 0.5f * q[r][NV - 1] * ir;
+            }
+        }
 #pragma unroll
-        for (int c = 2 * NV; c < NP; ++c) pw[c] = 0.0f;
+        for (int c = 0; c < NV; ++c) pw[NA + c] = cj * y[r][c];
+#pragma unroll
+        for (int c = NA + NV; c < NP; ++c) pw[c] = 0.0f;
 #pragma unroll
         for (int rr = 0; rr < NR; ++rr)
-            sR[rr * NN + j] = make_float4(pw[4 * rr], pw[4 * rr + 1], pw[4 * rr + 2], pw[4 * rr + 3]);
+            sR[rr * NNP + pj] = make_float4(pw[4 * rr], pw[4 * rr + 1], pw[4 * rr + 2], pw[4 * rr + 3]);
         // metric: column d of the node's matrix is Ja^d; stored halved per direction
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             if constexpr (ND == 3)
-                reinterpret_cast<float4*>(sM + d * MW * NN)[j] =
+                reinterpret_cast<float4*>(sM + d * MW * NNP)[pj] =
                     make_float4(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1], 0.5f * g[r][ND * d + 2], 0.0f);
             else
-                reinterpret_cast<float2*>(sM + d * MW * NN)[j] = make_float2(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1]);
+                reinterpret_cast<float2*>(sM + d * MW * NNP)[pj] = make_float2(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1]);
         }
     }
     __syncthreads();
@@ -486,7 +631,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     }
 
     // ---- line tasks --------------------------------------------------------------------
-    float acc[N][NV];
+    float acc[N][NA];
     {
         // face terms of both line ends: S̄ = -co·w_own + cn·w_nb with co, cn = ½·jac·∂g of
         // the own and the neighbour's end (boundary faces: -co·w_own); on a boundary face
@@ -499,7 +644,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
         for (int sd = 0; sd < 2; ++sd) {
             const int nd = base + (sd ? (N - 1) * stride : 0);
             float G[NV], ob[NV], xb[NV], xs[NV], pw[NP];
-            load_rec<ND, NN>(sR, nd, pw);
+            load_rec<NR, NNP>(sR, nd, pw);
             const float s = finfo_side(xinf[sd]) ? -1.0f : 1.0f;
             float nown[ND];
 #pragma unroll
@@ -510,7 +655,7 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
             const float cn = 0.5f * fj[sd] * ((kp & 1) ? a.lift_p : a.lift_m);
 #pragma unroll
             for (int v = 0; v < NV; ++v) {
-                G[v] = bnd ? -co * pw[NV + v] : fmaf(cn, xw[sd][v], -co * pw[NV + v]);
+                G[v] = bnd ? -co * pw[NA + v] : fmaf(cn, xw[sd][v], -co * pw[NA + v]);
                 ob[v] = 0.0f;
                 xb[v] = 0.0f;
                 xs[v] = xq[sd][v];
@@ -558,30 +703,36 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
         }
         if (bad && line_ok) err |= ERR_SQRT_NEG;
 
-        float amax = 0.0f, amin = 0.0f, bmax = 0.0f, bmin = 0.0f;
 #pragma unroll
-        for (int p = 0; p < N; ++p) {
-            const int nd = base + p * stride;
-            const float ap = sR[nd].x, bp = sm[C::OFF_R + 4 * (((ND + 1) / 4) * NN + nd) + (ND + 1) % 4];
-            amax = p ? fmaxf(amax, ap) : ap; amin = p ? fminf(amin, ap) : ap;
-            bmax = p ? fmaxf(bmax, bp) : bp; bmin = p ? fminf(bmin, bp) : bp;
+        for (int p = 0; p < N; ++p)
 #pragma unroll
-            for (int v = 0; v < NV; ++v) acc[p][v] = 0.0f;
+            for (int v = 0; v < NA; ++v) acc[p][v] = 0.0f;
+        const float* const sMd = sM + ldir * MW * NNP;
+        if constexpr (TP == TP_CHANDRASEKHAR) {
+            float amax = 0.0f, amin = 0.0f, bmax = 0.0f, bmin = 0.0f;
+#pragma unroll
+            for (int p = 0; p < N; ++p) {
+                const int nd = base + p * stride;
+                const float ap = sR[nd].x, bp = sm[C::OFF_R + 4 * (((ND + 1) / 4) * NNP + nd) + (ND + 1) % 4];
+                amax = p ? fmaxf(amax, ap) : ap; amin = p ? fminf(amin, ap) : ap;
+                bmax = p ? fmaxf(bmax, bp) : bp; bmin = p ? fminf(bmin, bp) : bp;
+            }
+            // one log-mean path for every pair of the wave's lines when the spreads of ρ
+            // and β allow it (u < 1e-4: first order, u < 0.01: the series, on each line's
+            // extreme pair with a margin), as in the forward
+            const float da = amax - amin, sa = amax + amin, dbm = bmax - bmin, sbm = bmax + bmin;
+            const float qa = da * da * 1.0001f, pa = sa * sa, qb = dbm * dbm * 1.0001f, pb = sbm * sbm;
+            const int path = wave_path(!line_ok || (qa < 1e-4f * pa && qb < 1e-4f * pb),
+                                       !line_ok || (qa < 1e-2f * pa && qb < 1e-2f * pb));
+            if (path == 2)
+                line_vjp<ND, N, NNP, TP, 2, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+            else if (path == 1)
+                line_vjp<ND, N, NNP, TP, 1, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+            else
+                line_vjp<ND, N, NNP, TP, 0, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+        } else {
+            line_vjp<ND, N, NNP, TP, 0, TP == TP_STDAVG && !CURV>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
         }
-        // one log-mean path for every pair of the wave's lines when the spreads of ρ and β
-        // allow it (u < 1e-4: first order, u < 0.01: the series, on each line's extreme
-        // pair with a margin), as in the forward
-        const float da = amax - amin, sa = amax + amin, dbm = bmax - bmin, sbm = bmax + bmin;
-        const float* const sMd = sM + ldir * MW * NN;
-        const float qa = da * da * 1.0001f, pa = sa * sa, qb = dbm * dbm * 1.0001f, pb = sbm * sbm;
-        const int path = wave_path(!line_ok || (qa < 1e-4f * pa && qb < 1e-4f * pb),
-                                   !line_ok || (qa < 1e-2f * pa && qb < 1e-2f * pb));
-        if (path == 2)
-            line_vjp<ND, N, NN, 2>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
-        else if (path == 1)
-            line_vjp<ND, N, NN, 1>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
-        else
-            line_vjp<ND, N, NN, 0>(sR, sMd, base, stride, A.iw, A.wsh, ig1, acc);
     }
 
     // epilogue operands (null pointers read as zero)
@@ -600,13 +751,17 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
         }
     }
     __syncthreads();  // the lines are done: the staging slots are dead
-    if (line_ok) {
-        // rows of direction d in slots [d*NV, d*NV + NV)
-        float* R = sm + ldir * NV * NN;
+    {
+        // rows of direction d in slots [d*NA, d*NA + NA); a lane without a line of its own
+        // (past the tasks, or on a missing element of the last block) stores into the unread
+        // slot ND instead.  (A conditional store would let the compiler sink the pair
+        // arithmetic past the barrier into that branch and keep every partner record loaded
+        // before it live across the line phase.)
+        float* R = sm + (line_ok ? ldir : ND) * NA * NNP;
 #pragma unroll
         for (int p = 0; p < N; ++p)
 #pragma unroll
-            for (int c = 0; c < NV; ++c) R[c * NN + base + p * stride] = acc[p][c];
+            for (int c = 0; c < NA; ++c) R[c * NNP + base + p * stride] = acc[p][c];
     }
     __syncthreads();
 
@@ -614,35 +769,50 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
         const int j = tid + r * TPB;
-        const int jj = j < NN ? j : 0;
-        float t[NV];
+        const int jj = j < NN ? j : 0, pjj = pnode<N, C::PAD>(jj);
+        float t[NA];
 #pragma unroll
-        for (int v = 0; v < NV; ++v) {
-            t[v] = sm[v * NN + jj];
+        for (int v = 0; v < NA; ++v) {
+            t[v] = sm[v * NNP + pjj];
 #pragma unroll
-            for (int d = 1; d < ND; ++d) t[v] += sm[(d * NV + v) * NN + jj];
+            for (int d = 1; d < ND; ++d) t[v] += sm[(d * NA + v) * NNP + pjj];
         }
-        // (a, v, β) = (ρ/2, u/2, ρ/(2p)); u = m/ρ, p = (γ-1)(E - ρ|u|²/2)
-        const float rho = q[r][0], ir = rcp(rho);
-        float u[ND];
+        float V[NV];
+        if constexpr (TP == TP_STDAVG) {
 #pragma unroll
-        for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * ir;
-        float uu = u[ND - 1] * u[ND - 1];
+            for (int v = 0; v < NV; ++v) V[v] = t[v];
+        } else {
+            // Chandrasekhar (a, v, β) = (ρ/2, u/2, ρ/(2p)), Kennedy-Gruber
+            // (a, v, s, e) = (ρ/2, u/2, p/2, E/(2ρ)); u = m/ρ, p = (γ-1)(E - ρ|u|²/2)
+            const float rho = q[r][0], ir = rcp(rho);
+            float u[ND];
 #pragma unroll
-        for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
-        const float p = gm1 * fmaf(-0.5f * rho, uu, q[r][NV - 1]);
-        const float ip = rcp(p);
-        const float rb = 0.5f * fmaf(t[NV - 1], ip, t[0]);       // of ρ at fixed (u, p)
-        const float pg = -gm1 * t[NV - 1] * (0.5f * rho * ip) * ip;  // (γ-1) × cotangent of p
-        float V[NV], ubu = 0.0f;
+            for (int k = 0; k < ND; ++k) u[k] = q[r][1 + k] * ir;
+            float uu = u[ND - 1] * u[ND - 1];
 #pragma unroll
-        for (int k = 0; k < ND; ++k) {
-            const float ubk = 0.5f * t[1 + k];
-            ubu = fmaf(ubk, u[k], ubu);
-            V[1 + k] = fmaf(ubk, ir, -pg * u[k]);
+            for (int k = ND - 2; k >= 0; --k) uu = fmaf(u[k], u[k], uu);
+            float rb, pg, eb;  // of ρ at fixed (u, p, E), (γ-1) × of p, of E at fixed ρ
+            if constexpr (TP == TP_CHANDRASEKHAR) {
+                const float p = gm1 * fmaf(-0.5f * rho, uu, q[r][NV - 1]);
+                const float ip = rcp(p);
+                rb = 0.5f * fmaf(t[NV - 1], ip, t[0]);
+                pg = -gm1 * t[NV - 1] * (0.5f * rho * ip) * ip;
+                eb = 0.0f;
+            } else {
+                eb = 0.5f * t[ND + 2] * ir;
+                rb = fmaf(-eb, q[r][NV - 1] * ir, 0.5f * t[0]);
+                pg = 0.5f * gm1 * t[ND + 1];
+            }
+            float ubu = 0.0f;
+#pragma unroll
+            for (int k = 0; k < ND; ++k) {
+                const float ubk = 0.5f * t[1 + k];
+                ubu = fmaf(ubk, u[k], ubu);
+                V[1 + k] = fmaf(ubk, ir, -pg * u[k]);
+            }
+            V[0] = fmaf(-ubu, ir, fmaf(0.5f * pg, uu, rb));
+            V[NV - 1] = pg + eb;
         }
-        V[0] = fmaf(-ubu, ir, fmaf(0.5f * pg, uu, rb));
-        V[NV - 1] = pg;
         // faces of the node's element that it lies on
         const int le = jj / NPE, node = jj - (jj / NPE) * NPE;
         int ix[3];
@@ -673,39 +843,45 @@ __global__ void __launch_bounds__((VCfg<ND, N>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(c
     if (err) atomicOr(a.errflag, err);
 }
 
-template <int ND, int N, bool CURV, bool WALLS>
+template <int ND, int N, int TP, bool CURV, bool WALLS>
 hipError_t launch_fvjp(const FastVjpArgs& a, hipStream_t s) {
-    using C = VCfg<ND, N>;
+    using C = VCfg<ND, N, TP>;
     const int nb = (a.f.ne + C::EPB - 1) / C::EPB;
-    hipLaunchKernelGGL((k_fvjp<ND, N, CURV, WALLS>), dim3(nb), dim3(C::TPB), 0, s, a);
+    hipLaunchKernelGGL((k_fvjp<ND, N, TP, CURV, WALLS>), dim3(nb), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
 }
 
-template <int ND, bool CURV, bool WALLS>
+template <int ND, int TP, bool CURV, bool WALLS>
 FastVjpLauncher fvjp_n(int n) {
     switch (n) {
-        case 4: return &launch_fvjp<ND, 4, CURV, WALLS>;
-        case 5: return &launch_fvjp<ND, 5, CURV, WALLS>;
-        case 6: return &launch_fvjp<ND, 6, CURV, WALLS>;
-        case 7: return &launch_fvjp<ND, 7, CURV, WALLS>;
-        case 8: return &launch_fvjp<ND, 8, CURV, WALLS>;
+        case 4: return &launch_fvjp<ND, 4, TP, CURV, WALLS>;
+        case 5: return &launch_fvjp<ND, 5, TP, CURV, WALLS>;
+        case 6: return &launch_fvjp<ND, 6, TP, CURV, WALLS>;
+        case 7: return &launch_fvjp<ND, 7, TP, CURV, WALLS>;
+        case 8: return &launch_fvjp<ND, 8, TP, CURV, WALLS>;
         default: return nullptr;
     }
 }
 
-template <int ND>
+template <int ND, int TP>
 FastVjpLauncher fvjp_nd(int n, bool curved, bool walls) {
-    if (curved) return walls ? fvjp_n<ND, true, true>(n) : fvjp_n<ND, true, false>(n);
-    return walls ? fvjp_n<ND, false, true>(n) : fvjp_n<ND, false, false>(n);
+    if (curved) return walls ? fvjp_n<ND, TP, true, true>(n) : fvjp_n<ND, TP, true, false>(n);
+    return walls ? fvjp_n<ND, TP, false, true>(n) : fvjp_n<ND, TP, false, false>(n);
 }
 
-}  // namespace fast
-
-FastVjpLauncher fast_vjp_launcher(int nd, int n, int tp, bool curved, bool walls) {
-    if (tp != TP_CHANDRASEKHAR) return nullptr;
-    if (nd == 2) return fast::fvjp_nd<2>(n, curved, walls);
-    if (nd == 3) return fast::fvjp_nd<3>(n, curved, walls);
+template <int TP>
+FastVjpLauncher fvjp_tp(int nd, int n, bool curved, bool walls) {
+    if (nd == 2) return fvjp_nd<2, TP>(n, curved, walls);
+    if (nd == 3) return fvjp_nd<3, TP>(n, curved, walls);
     return nullptr;
 }
 
+// this translation unit holds the kernels of one two-point flux (FLOUHIP_FVJP_TP, the
+// value of TP_STDAVG, TP_CHANDRASEKHAR or TP_KENNEDY_GRUBER), so the three build in
+// parallel
+static_assert(TP_STDAVG == 0 && TP_CHANDRASEKHAR == 1 && TP_KENNEDY_GRUBER == 2,
+              "the flux numbers of fast_vjp_launcher");
+template FastVjpLauncher fvjp_tp<FLOUHIP_FVJP_TP>(int, int, bool, bool);
+
+}  // namespace fast
 }  // namespace flouhip

@@ -165,8 +165,8 @@ const char* flou_hip_create_error(void);
  *     FP32 kernel where it applies (default).
  *  2: query, returns 1 if this FP32 handle has the fast kernel, else 0 (value ignored).
  *  3: query, returns 1 if this FP32 handle's adjoint (flou_hip_rhs_vjp, flou_hip_rk_step_vjp)
- *     has the fast kernel (where the fast rhs runs, with the Chandrasekhar two-point flux;
- *     option 1 switches it off with the fast rhs), else 0. */
+ *     has the fast kernel (wherever the fast rhs runs; option 1 switches it off with the
+ *     fast rhs), else 0. */
 int flou_hip_set_option(flou_hip_handle* h, int32_t which, int32_t value);
 
 /* Stream the handle launches on (hipStream_t; NULL = default stream). */

@@ -1,6 +1,7 @@
 """The FP32 fast adjoint kernel (flou.jl_amd/csrc/vjp_fast.hip) on curved and on
-axis-aligned affine periodic meshes (the split form with the Chandrasekhar flux and LxF;
-OpDivergence.jl:200-299, Interfaces.jl:111-136, Equations/Euler.jl:474-536):
+axis-aligned affine periodic meshes (the split form with the Chandrasekhar, Kennedy-Gruber
+or StdAverage two-point flux and LxF; OpDivergence.jl:200-299, Interfaces.jl:111-136,
+Equations/Euler.jl:385-536):
 
 * the rhs VJP against the plain FP32 VJP (k_vjp, switched on with option 1) on the same
   inputs, over 2-D and 3-D, N = 4..8, element counts that leave the last block part-filled,
@@ -25,7 +26,7 @@ import oracle_lib as O
 from cases import GAMMA, oracle_for, periodic_euler_state, random_euler_state
 from flouhip import mesh as M
 from flouhip import adjoint as A
-from test_gpu_curved import build_curved
+from test_gpu_curved import TP, build_curved
 
 pytestmark = pytest.mark.gpu
 
@@ -37,24 +38,24 @@ CASES = [(3, 4, 3), (3, 5, 2), (3, 6, 3), (3, 7, 2), (3, 8, 2),
          (2, 4, 5), (2, 5, 6), (2, 6, 5), (2, 7, 4), (2, 8, 3)]
 
 
-def _fp32_disc(nd, n, nel):
-    disc, orc, eq, X, J, std = build_curved(nd, n, nel, "split", "ch", precision="fp32")
+def _fp32_disc(nd, n, nel, tp="ch"):
+    disc, orc, eq, X, J, std = build_curved(nd, n, nel, "split", tp, precision="fp32")
     assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
     return disc, orc, X
 
 
-def _affine_fp32_disc(nd, n, nel, lengths):
+def _affine_fp32_disc(nd, n, nel, lengths, tp="ch"):
     """Periodic Cartesian mesh on a box with unequal sides (Ja_dd differs per direction)."""
     b = F.LagrangeBasis("GLL", n)
     std = [None, None, F.StdQuad, F.StdHex][nd](b, F.DGSEMrec(b), nd + 2)
     mesh = F.CartesianMesh(nd, (0.0,) * nd, tuple(lengths), (nel,) * nd)
     mesh.apply_periodicBCs(*[(str(2 * k + 1), str(2 * k + 2)) for k in range(nd)])
     eq = F.EulerEquation(nd, GAMMA)
-    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    op = F.SplitDivOperator(TP[tp][0](), F.LxF(F.StdAverage(), 1.0))
     disc = F.MultielementDisc(mesh, std, eq, op, precision="fp32")
     assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
     orc = oracle_for(nd, n, [nel] * nd, [0.0] * nd, list(lengths), [1] * nd, O.EQ_EULER, [GAMMA], O.VOL_SPLIT,
-                     O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+                     TP[tp][1], O.AVG_STD, O.DISS_LXF, 1.0)
     return disc, orc, mesh.node_coords(std)
 
 
@@ -173,7 +174,7 @@ def test_fast_vjp_reverse_rk_step_against_fp64(solver):
     assert err[0][0] <= 2e-6 and err[1][0] <= 2e-6
 
 
-def _wall_case(kind, nd, axis):
+def _wall_case(kind, nd, axis, tp="ch"):
     """One axis non-periodic with the given boundary kind, as in the fast-rhs boundary test
     (tests/test_gpu_parity.py), FP32, with the FP64 oracle on the same boundaries."""
     from test_gpu_parity import build
@@ -182,7 +183,7 @@ def _wall_case(kind, nd, axis):
     nel = [2] * nd if nd == 3 else [4, 5]
     nel[axis] = 3
     n = 6 if nd == 3 else 8
-    mesh, std, equation, op, orc = build(nd, n, nel, "split", "ch", periodic=periodic, finish=[2 * np.pi] * nd)
+    mesh, std, equation, op, orc = build(nd, n, nel, "split", tp, periodic=periodic, finish=[2 * np.pi] * nd)
     lo, hi = str(2 * axis + 1), str(2 * axis + 2)
     qin = np.array([1.1, 0.3, -0.05, 0.02, 2.7]) if nd == 3 else np.array([1.1, 0.3, -0.05, 2.7])
     X, FX = orc.coords()
@@ -273,3 +274,92 @@ def test_fast_vjp_curved_wavy_walls(kind):
     print(f"curved wavy {kind} walls: fast vs plain FP32 VJP relative L2 {err:.2e}; dot product vs the "
           f"oracle's JVP {dot:.2e}")
     assert err <= 2e-6 and dot <= 2e-5
+
+
+# The Kennedy-Gruber and StdAverage two-point fluxes (their reverse products checked
+# against central differences of the forward fluxes in float64 before the kernel ran):
+# curved and affine (where the StdAverage lines take the linear form), 2-D and 3-D, N = 4..8
+OTHER = [(3, 4, 3, "kg"), (3, 5, 2, "std"), (3, 6, 2, "kg"), (3, 7, 2, "std"), (3, 8, 2, "kg"),
+         (2, 4, 5, "std"), (2, 5, 6, "kg"), (2, 6, 5, "std"), (2, 7, 4, "kg"), (2, 8, 3, "std")]
+
+
+@pytest.mark.parametrize("curved", [True, False], ids=["curved", "affine"])
+@pytest.mark.parametrize("case", OTHER, ids=lambda c: "{}d-n{}-{}-{}".format(*c))
+def test_fast_vjp_kg_std_matches_plain_fp32_vjp(case, curved):
+    nd, n, nel, tp = case
+    if curved:
+        disc, orc, X = _fp32_disc(nd, n, nel, tp)
+    else:
+        disc, orc, X = _affine_fp32_disc(nd, n, nel, (1.0, 1.5, 0.75)[:nd], tp)
+    Q = periodic_euler_state(X, nd, 1.0, seed=25, amp=0.4)
+    W = np.random.default_rng(26).standard_normal(Q.shape)
+    got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
+    err, emax = _rel2(got, ref), _rel(got, ref)
+    print(f"{case} {'curved' if curved else 'affine'}: fast vs plain FP32 VJP, relative L2 {err:.2e}, "
+          f"max {emax:.2e}")
+    assert err <= 2e-6
+
+
+@pytest.mark.parametrize("curved", [True, False], ids=["curved", "affine"])
+@pytest.mark.parametrize("case", [(3, 6, 2, "kg"), (3, 5, 2, "std"), (2, 7, 4, "kg"), (2, 8, 3, "std")],
+                         ids=lambda c: "{}d-n{}-{}-{}".format(*c))
+def test_fast_vjp_kg_std_dot_product_with_oracle_jvp(case, curved):
+    nd, n, nel, tp = case
+    if curved:
+        disc, orc, X = _fp32_disc(nd, n, nel, tp)
+    else:
+        disc, orc, X = _affine_fp32_disc(nd, n, nel, (1.0, 1.5, 0.75)[:nd], tp)
+    Q = periodic_euler_state(X, nd, 1.0, seed=27, amp=0.2).astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(28)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
+    JtW = _vjp(disc, Q, W, plain=False)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(JtW * V))
+    err = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"{case} {'curved' if curved else 'affine'}: <W,JV> = {lhs:.8e}  <J^T W,V> (fast FP32) = {rhs:.8e}  "
+          f"rel {err:.2e}")
+    assert err <= 2e-5
+
+
+@pytest.mark.parametrize("tp", ["kg", "std"])
+@pytest.mark.parametrize("kind,nd,axis", [("inflow-outflow", 3, 0), ("dirichlet", 2, 1), ("slip-outflow", 3, 2)],
+                         ids=["inflow-outflow-3d-x", "dirichlet-2d-y", "slip-outflow-3d-z"])
+def test_fast_vjp_kg_std_boundaries(kind, nd, axis, tp):
+    disc, orc, Q = _wall_case(kind, nd, axis, tp)
+    Q = Q.astype(np.float32).astype(np.float64)
+    rng = np.random.default_rng(29)
+    V, W = rng.standard_normal(Q.shape), rng.standard_normal(Q.shape)
+    got, ref = _vjp(disc, Q, W, plain=False), _vjp(disc, Q, W, plain=True)
+    err = _rel2(got, ref)
+    _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
+    lhs, rhs = float(np.sum(W * JV)), float(np.sum(got * V))
+    dot = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
+    print(f"{tp} {kind} {nd}-D axis {axis}: fast vs plain FP32 VJP relative L2 {err:.2e} "
+          f"(max {_rel(got, ref):.2e}); dot product vs the oracle's JVP {dot:.2e}")
+    assert err <= 2e-6 and dot <= 2e-5
+
+
+@pytest.mark.parametrize("tp", ["kg", "std"])
+def test_fast_vjp_kg_std_reverse_rk_step_against_fp64(tp):
+    """One reverse SSPRK33 step at stages recorded in FP32, fast and plain FP32 adjoints
+    against the FP64 adjoint at the same stage states (relative L2, see _rel2)."""
+    disc, orc, X = _fp32_disc(2, 7, 4, tp)
+    d64 = build_curved(2, 7, 4, "split", tp, precision="fp64")[0]
+    Q = periodic_euler_state(X, 2, 1.0, seed=30, amp=0.15)
+    dt = 0.2 * orc.max_dt(Q, 1.0)
+    stages = [disc.to_device(Q)] + [disc.new_state() for _ in range(2)]
+    A.rk_step_record(disc, F.SSPRK33, dt, stages, disc.new_state())
+    st64 = [d64.to_device(disc.to_host(x).astype(np.float64)) for x in stages]
+    ybar = np.random.default_rng(31).standard_normal(Q.shape)
+    ref = d64.to_host(A.rk_step_vjp(d64, F.SSPRK33, dt, st64, d64.to_device(ybar)))
+    d64.check()
+    err = {}
+    for plain in (0, 1):
+        disc.set_option(1, plain)
+        ub = disc.to_device(ybar)
+        A.rk_step_vjp(disc, F.SSPRK33, dt, stages, ub)
+        disc.check()
+        err[plain] = _rel2(disc.to_host(ub).astype(np.float64), ref)
+    disc.set_option(1, 0)
+    print(f"{tp} SSPRK33 reverse step vs FP64, relative L2: fast {err[0]:.2e}, plain FP32 {err[1]:.2e}")
+    assert err[0] <= 2e-6 and err[1] <= 2e-6
