@@ -260,27 +260,34 @@ def _c2_prim(X, g):
     return torch.stack([rho, u, v, torch.ones_like(x)])
 
 
-def _c2_disc(form):
-    """The C2 discretisation (periodic 256^2 quads on [0,2]^2, N=7, split Kennedy-Gruber +
-    LxF(StdAverage)) in the given precision and its seeded FP64 state."""
+def _c2_case():
+    """The C2 leg's discretisation parts and a function building (disc, state) per precision:
+    periodic 256^2 quads on [0,2]^2, N=7, split Kennedy-Gruber + LxF(StdAverage), and its
+    seeded FP64 state (built once)."""
     import torch
     import flouhip as F
     b = F.LagrangeBasis("GLL", 8)
     std = F.StdQuad(b, F.DGSEMrec(b), 4)
     eq = F.EulerEquation(2, GAMMA)
-    mesh = F.CartesianMesh(2, (0.0, 0.0), (2.0, 2.0), (256, 256))
-    mesh.apply_periodicBCs(("1", "2"), ("3", "4"))
     op = F.SplitDivOperator(F.KennedyGruber(), F.LxF(F.StdAverage(), 1.0))
-    disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
-    X = torch.tensor(mesh.node_coords(std), device="cuda").T
-    g = torch.Generator(device="cpu")
-    g.manual_seed(7)
-    prim = _c2_prim(X, g)
-    Q64 = torch.empty_like(prim)
-    Q64[0] = prim[0]
-    Q64[1:3] = prim[0] * prim[1:3]
-    Q64[3] = prim[3] / (GAMMA - 1) + prim[0] * (prim[1:3] ** 2).sum(0) / 2
-    return disc, eq, Q64
+    state = {}
+
+    def build(form):
+        mesh = F.CartesianMesh(2, (0.0, 0.0), (2.0, 2.0), (256, 256))
+        mesh.apply_periodicBCs(("1", "2"), ("3", "4"))
+        disc = F.MultielementDisc(mesh, std, eq, op, precision=form)
+        if "Q64" not in state:
+            X = torch.tensor(mesh.node_coords(std), device="cuda").T
+            g = torch.Generator(device="cpu")
+            g.manual_seed(7)
+            prim = _c2_prim(X, g)
+            Q64 = torch.empty_like(prim)
+            Q64[0] = prim[0]
+            Q64[1:3] = prim[0] * prim[1:3]
+            Q64[3] = prim[3] / (GAMMA - 1) + prim[0] * (prim[1:3] ** 2).sum(0) / 2
+            state["Q64"] = Q64
+        return disc, state["Q64"]
+    return std, eq, build
 
 
 def secondary_workload(name, steps=None, warmup=2):
@@ -431,7 +438,8 @@ def vjp_kernel(name, reps=20):
         std, eq, build = _c5_case()
         disc, Q64 = build("fp32")
     elif name == "c2":
-        disc, eq, Q64 = _c2_disc("fp32")
+        std, eq, build = _c2_case()
+        disc, Q64 = build("fp32")
     else:
         disc, eq, std = build_disc("fp32")
         Q64 = taylor_green_state(disc, std)
@@ -467,18 +475,20 @@ def vjp_kernel(name, reps=20):
             "workload": label[name] + " FP32 (the bench's mesh and state), seeded cotangent; fast adjoint kernel vs fast rhs"}
 
 
-def c5_adjoint(nsteps=100, every=10):
-    """C5 adjoint: the curved 32^3, N=6 mesh and state of the C5 leg, loss Σ Jω (ρ(T) - 1)²
-    after 100 SSPRK33 steps (CFL 0.1 of the FP64 state), gradient w.r.t. the initial state
-    with FP64 checkpoints every 10 steps (the last segment's stage inputs kept; FP32 also
-    with every stage input kept).  FP32 (fast forward and adjoint kernels) and FP64; per
-    precision the forward and gradient times (HIP events, after one untimed gradient) and
-    their ratio, and the FP32 gradient's relative L2 error against the FP64 one."""
+def secondary_adjoint(name, nsteps=100, every=10):
+    """Adjoint of a secondary workload (c5: the curved 32^3, N=6 mesh and state of the C5
+    leg; c2: the 2-D 256^2, N=7 Kennedy-Gruber mesh and state of the C2 leg), loss
+    Σ Jω (ρ(T) - 1)² after 100 SSPRK33 steps (CFL 0.1 of the FP64 state), gradient w.r.t.
+    the initial state with FP64 checkpoints every 10 steps (the last segment's stage inputs
+    kept; FP32 also with every stage input kept).  FP32 (fast forward and adjoint kernels)
+    and FP64; per precision the forward and gradient times (HIP events, after one untimed
+    gradient) and their ratio, and the FP32 gradient's relative L2 error against the FP64
+    one."""
     import numpy as np
     import torch
     import flouhip as F
     from flouhip import adjoint as A
-    std, eq, build = _c5_case()
+    std, eq, build = _c5_case() if name == "c5" else _c2_case()
     out, grads, dt = {}, {}, None
     stream = torch.cuda.current_stream()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -487,8 +497,10 @@ def c5_adjoint(nsteps=100, every=10):
         if dt is None:
             dt = F.get_max_dt(Q64, disc, eq, 0.1)
         Q = Q64.to(disc.torch_dtype(), copy=True).contiguous()
-        wq = (torch.tensor(disc.elem_jac, device="cuda")
-              * torch.tensor(np.tile(std.ω, disc.nelements), device="cuda")).to(disc.torch_dtype())
+        jac = torch.tensor(np.asarray(disc.elem_jac, dtype=np.float64).reshape(-1), device="cuda")
+        if jac.numel() == disc.nelements:  # affine elements: one J per element
+            jac = jac.repeat_interleave(len(std.ω))
+        wq = (jac * torch.tensor(np.tile(std.ω, disc.nelements), device="cuda")).to(disc.torch_dtype())
 
         def seed(u):
             gr = torch.zeros_like(u)
@@ -511,7 +523,7 @@ def c5_adjoint(nsteps=100, every=10):
         out[form] = {"forward_ms": fwd, "gradient_ms": gms, "gradient_over_forward": gms / fwd,
                      "fast_adjoint_kernel": disc.uses_fast_vjp()}
         if form == "fp32":
-            # every stage input kept in HBM (3 x 100 states of 225 MB): no recomputation
+            # every stage input kept in HBM (3 x 100 states): no recomputation
             nseg = -(-nsteps // every)
             grad(nseg)
             torch.cuda.synchronize()
@@ -531,7 +543,8 @@ def c5_adjoint(nsteps=100, every=10):
     out["fp32_gradient_rel_l2_err_vs_fp64"] = float(torch.linalg.norm(grads["fp32"] - grads["fp64"])
                                                   / torch.linalg.norm(grads["fp64"]))
     out["fp32_gradient_speedup_vs_fp64"] = out["fp64"]["gradient_ms"] / out["fp32"]["gradient_ms"]
-    out["workload"] = ("C5 curved 32^3 N=6 (the c5 leg's mesh and state), loss sum Jw (rho(T) - 1)^2 after "
+    out["workload"] = (("C5 curved 32^3 N=6" if name == "c5" else "C2 2-D 256^2 N=7 Kennedy-Gruber")
+                       + f" (the {name} leg's mesh and state), loss sum Jw (rho(T) - 1)^2 after "
                        "100 SSPRK33 steps, gradient with FP64 checkpoints every 10 steps (last segment kept; "
                        "FP32 also with every stage input kept); FP32: fast forward, recomputation and adjoint "
                        "kernels, FP64: plain kernels")
@@ -587,7 +600,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c5adj", "c5vjp", "c3vjp", "c2vjp"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c2adj", "c5adj", "c5vjp", "c3vjp",
+                                       "c2vjp"],
                     default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
@@ -608,8 +622,8 @@ def main():
     if args.only == "adjoint":  # profiling run of the adjoint workload alone
         print(json.dumps({"adjoint": adjoint_ratio()}), flush=True)
         return
-    if args.only == "c5adj":  # the C5 adjoint alone
-        print(json.dumps({"c5_adjoint": c5_adjoint()}), flush=True)
+    if args.only in ("c2adj", "c5adj"):  # one secondary adjoint alone
+        print(json.dumps({args.only[:2] + "_adjoint": secondary_adjoint(args.only[:2])}), flush=True)
         return
     if args.only in ("c5vjp", "c3vjp", "c2vjp"):  # profiling run of an FP32 adjoint kernel alone
         print(json.dumps({args.only[:2] + "_vjp": vjp_kernel(args.only[:2])}), flush=True)
@@ -710,12 +724,12 @@ def main():
         adj = adjoint_ratio()
         out["adjoint_forward_ratio"] = adj["value"]
         out["adjoint"] = adj
-        if not args.no_secondary:
-            out["c5_adjoint"] = c5a = c5_adjoint()
-            print(f"# c5 adjoint: fp32 gradient/forward {c5a['fp32']['gradient_over_forward']:.2f} "
-                  f"(all stages kept {c5a['fp32']['all_stages_kept']['gradient_over_forward']:.2f}), "
-                  f"fp64 {c5a['fp64']['gradient_over_forward']:.2f}, fp32 error "
-                  f"{c5a['fp32_gradient_rel_l2_err_vs_fp64']:.2e}", file=sys.stderr, flush=True)
+        for name in ("c2", "c5") if not args.no_secondary else ():
+            out[name + "_adjoint"] = sa = secondary_adjoint(name)
+            print(f"# {name} adjoint: fp32 gradient/forward {sa['fp32']['gradient_over_forward']:.2f} "
+                  f"(all stages kept {sa['fp32']['all_stages_kept']['gradient_over_forward']:.2f}), "
+                  f"fp64 {sa['fp64']['gradient_over_forward']:.2f}, fp32 error "
+                  f"{sa['fp32_gradient_rel_l2_err_vs_fp64']:.2e}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
