@@ -69,7 +69,8 @@ struct VCfg {
     static constexpr int OFF_F = OFF_M + ND * MW * NNP;
     static constexpr int LDS = r4(OFF_F + EPB * NFC * NV * NPF);
     static_assert((ND + 1) * NA * NNP <= OFF_F, "the rows (and an unread slot) must fit in the staging area");
-    static_assert(EPB * NL >= 64, "lanes past the line tasks repeat the line 64 lanes before");
+    static_assert(EPB * NL >= 64 && TPB - EPB * NL <= EPB * NPF,
+                  "lanes past the line tasks repeat lines of the last direction");
 };
 
 // The log-mean factor F(u) = atanh(f)/f at u = f², f = (x_i - x_j)/(x_i + x_j), and
@@ -469,8 +470,8 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
     // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
     // direction-major: consecutive lanes run the lines of one direction over the block's
     // elements, so a wave reads records with one stride where EPB·NPF allows it; a lane
-    // past the tasks repeats the line 64 lanes before it, usually of its wave's direction
-    const int tl = tid < EPB * NL ? tid : tid - 64;
+    // past the tasks repeats a line of the last direction, like the last live lanes
+    const int tl = tid < EPB * NL ? tid : tid - EPB * NPF;
     const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_ok = tid < EPB * NL && lle < nel;

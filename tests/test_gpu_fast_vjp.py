@@ -231,9 +231,11 @@ def test_fast_vjp_boundaries(kind, nd, axis):
     assert err <= 2e-6 and dot <= 2e-5
 
 
+@pytest.mark.parametrize("tp", ["ch", "kg", "std"])
 @pytest.mark.parametrize("kind", ["slip", "inflow-outflow"])
-def test_fast_vjp_curved_wavy_walls(kind):
-    """Curved 2-D mesh with wavy walls (rotated face frames at the slip reflection)."""
+def test_fast_vjp_curved_wavy_walls(kind, tp):
+    """Curved 2-D mesh with wavy walls (rotated face frames at the slip reflection), each
+    two-point flux."""
     n, ne = 6, 4
     b = F.LagrangeBasis("GLL", n)
     std = F.StdQuad(b, F.DGSEMrec(b), 4)
@@ -247,7 +249,7 @@ def test_fast_vjp_curved_wavy_walls(kind):
 
     mesh.apply_mapping(wavy)
     eq = F.EulerEquation(2, GAMMA)
-    op = F.SplitDivOperator(F.ChandrasekharAverage(), F.LxF(F.StdAverage(), 1.0))
+    op = F.SplitDivOperator(TP[tp][0](), F.LxF(F.StdAverage(), 1.0))
     qin = np.array([1.1, 0.2, 0.1, 2.6])
     bcs = ({"3": F.EulerSlipBC(), "4": F.EulerSlipBC()} if kind == "slip"
            else {"3": F.EulerInflowBC(qin), "4": F.EulerOutflowBC()})
@@ -255,7 +257,7 @@ def test_fast_vjp_curved_wavy_walls(kind):
     assert disc.uses_fast_kernel() and disc.uses_fast_vjp()
     J, met, ev, fr, fj = M.curved_geometry(mesh, std)
     orc = O.OracleDisc(2, n, [ne, ne], [0.0, 0.0], [1.0, 1.0], [1, 0], O.build_ops(n, "GLL"), O.EQ_EULER,
-                       [GAMMA], O.VOL_SPLIT, O.TP_CHANDRASEKHAR, O.AVG_STD, O.DISS_LXF, 1.0)
+                       [GAMMA], O.VOL_SPLIT, TP[tp][1], O.AVG_STD, O.DISS_LXF, 1.0)
     orc.set_curved(met, J, fr.reshape(-1), fj, ev)
     if kind == "slip":
         orc.set_bc(0, O.BC_SLIP)
@@ -271,8 +273,8 @@ def test_fast_vjp_curved_wavy_walls(kind):
     _, JV = orc.rhs_jvp(np.asfortranarray(Q), V)
     lhs, rhs = float(np.sum(W * JV)), float(np.sum(got * V))
     dot = abs(lhs - rhs) / float(np.sum(np.abs(W * JV)))
-    print(f"curved wavy {kind} walls: fast vs plain FP32 VJP relative L2 {err:.2e}; dot product vs the "
-          f"oracle's JVP {dot:.2e}")
+    print(f"curved wavy {kind} walls, {tp}: fast vs plain FP32 VJP relative L2 {err:.2e}; dot product vs "
+          f"the oracle's JVP {dot:.2e}")
     assert err <= 2e-6 and dot <= 2e-5
 
 
