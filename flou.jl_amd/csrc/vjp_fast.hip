@@ -43,7 +43,7 @@ constexpr int fvjp_epb() {
                    : (N == 4 ? 16 : (N == 5 ? 32 : (N == 6 ? 16 : (N == 7 ? 9 : 8))));
 }
 
-template <int ND, int N, int TP>
+template <int ND, int N, int TP, bool CURV>
 struct VCfg {
     static constexpr int NV = ND + 2, NFC = 2 * ND;
     static constexpr int NPE = ipow(N, ND), NPF = ipow(N, ND - 1), NL = ND * NPF;
@@ -54,21 +54,23 @@ struct VCfg {
     // node arrays in LDS hold block node j at pnode(j) = j + j/N (one slot after every row
     // of N nodes) for even N: the lanes of x-lines, N nodes apart, then read 16-byte records
     // an odd number of records apart (distinct banks for eight consecutive lanes)
-    // (not at 3-D N=6, where the larger arrays would hold a CU to two blocks instead of three)
-    static constexpr bool PAD = N % 2 == 0 && !(ND == 3 && N == 6);
+    // (not on curved elements at 3-D N=6, where the larger arrays would hold a CU to two
+    // blocks instead of three)
+    static constexpr bool PAD = N % 2 == 0 && !(CURV && ND == 3 && N == 6);
     static constexpr int NNP = PAD ? NN + NN / N : NN;
     // NA node inputs of the flux (their cotangents are the line rows), then w (NV)
     static constexpr int NA = TP == TP_KENNEDY_GRUBER ? ND + 3 : ND + 2;
-    // LDS: per node NR 16-byte records of (inputs.., w..) ([NR][NN] float4), then per
-    // direction the metric vector Ja^d/2 ([ND][NN] of MW floats).  The row sums of the ND
-    // directions ([ND*NA][NN] floats) go over them once the lines are done.  Then the face
-    // cotangents [EPB][2ND][NV][NPF].
+    // LDS: per node NR 16-byte records of (inputs.., w..) ([NR][NNP] float4), then, on
+    // curved elements, per direction the metric vector Ja^d/2 ([ND][NNP] of MW floats;
+    // affine elements take the line's constant metric vector from the element geometry).
+    // The row sums of the ND directions ([ND*NA][NNP] floats) go over them once the lines
+    // are done.  Then the face cotangents [EPB][2ND][NV][NPF].
     static constexpr int PW = NA + NV, NR = (PW + 3) / 4, NP = 4 * NR, MW = ND == 3 ? 4 : 2;
     static constexpr int OFF_R = 0;
     static constexpr int OFF_M = 4 * NR * NNP;
-    static constexpr int OFF_F = OFF_M + ND * MW * NNP;
+    static constexpr int STG = OFF_M + (CURV ? ND * MW * NNP : 0);
+    static constexpr int OFF_F = STG > ND * NA * NNP ? STG : ND * NA * NNP;
     static constexpr int LDS = r4(OFF_F + EPB * NFC * NV * NPF);
-    static_assert((ND + 1) * NA * NNP <= OFF_F, "the rows (and an unread slot) must fit in the staging area");
     static_assert(EPB * NL >= 64 && TPB - EPB * NL <= EPB * NPF,
                   "lanes past the line tasks repeat lines of the last direction");
 };
@@ -289,23 +291,26 @@ __device__ __forceinline__ void load_metric(const float* __restrict__ sMd, int j
 }
 
 // All pairs of a line, each once (F♯ is symmetric): node i's inputs in registers, node
-// j's read from LDS (records at sR, the metric vector of the line's direction at sMd), w
-// scaled by 1/ω of the node's position on the line.  LIN (StdAverage on affine elements,
-// n̄ = 2 Ja/2 the same for every pair of the line): the pairs only accumulate
-// Σ_j S_ij (w_j/ω_j - w_i/ω_i), and each node then takes the reverse product of
-// ½ f(Q)·n̄ = f(Q)·(Ja/2) for that sum.
-template <int ND, int N, int NN, int TP, int PATH, bool LIN>
-__device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const float* __restrict__ sMd, int base,
-                                         int stride, const float* __restrict__ iw, const float* __restrict__ wsh,
-                                         float ig1, float gm1, float (&acc)[N][VCfg<ND, N, TP>::NA]) {
-    using C = VCfg<ND, N, TP>;
+// j's read from LDS (records at sR; curved: the metric vector of the line's direction at
+// sMd, affine: the line's metric vector Ml for every node), w scaled by 1/ω of the node's
+// position on the line.  LIN (StdAverage on affine elements, n̄ = 2 Ja/2 the same for
+// every pair of the line): the pairs only accumulate Σ_j S_ij (w_j/ω_j - w_i/ω_i), and
+// each node then takes the reverse product of ½ f(Q)·n̄ = f(Q)·(Ja/2) for that sum.
+template <int ND, int N, int NN, int TP, int PATH, bool LIN, bool CURV>
+__device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const float* __restrict__ sMd,
+                                         const float* Ml, int base, int stride, const float* __restrict__ iw,
+                                         const float* __restrict__ wsh, float ig1, float gm1,
+                                         float (&acc)[N][VCfg<ND, N, TP, CURV>::NA]) {
+    using C = VCfg<ND, N, TP, CURV>;
     constexpr int NV = ND + 2, NP = C::NP, NR = C::NR, W0 = C::NA;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         const int ni = base + i * stride;
         float pi[NP], Mi[ND], wi[NV];
         load_rec<NR, NN>(sR, ni, pi);
-        if (!LIN) load_metric<ND>(sMd, ni, Mi);
+        if (CURV) load_metric<ND>(sMd, ni, Mi);
+#pragma unroll
+        for (int k = 0; k < ND; ++k) Mi[k] = CURV ? Mi[k] : Ml[k];
         const float iwi = iw[i];
 #pragma unroll
         for (int v = 0; v < NV; ++v) wi[v] = pi[W0 + v] * iwi;
@@ -316,7 +321,9 @@ __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const fl
             const int nj = base + j * stride;
             float pj[NP], Mj[ND], G[NV];
             load_rec<NR, NN>(sR, nj, pj);
-            if (!LIN) load_metric<ND>(sMd, nj, Mj);
+            if (CURV) load_metric<ND>(sMd, nj, Mj);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) Mj[k] = CURV ? Mj[k] : Ml[k];
             const float iwj = iw[j];
 #pragma unroll
             for (int v = 0; v < NV; ++v) G[v] = fmaf(pj[W0 + v], iwj, -wi[v]);
@@ -348,7 +355,8 @@ __device__ __forceinline__ void line_vjp(const float4* __restrict__ sR, const fl
             const int np = base + p * stride;
             float pp[NP], Mp[ND], G[NV];
             load_rec<NR, NN>(sR, np, pp);
-            load_metric<ND>(sMd, np, Mp);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) Mp[k] = Ml[k];  // LIN: affine elements only
             float gn = acc[p][1] * Mp[0];
 #pragma unroll
             for (int k = 1; k < ND; ++k) gn = fmaf(acc[p][1 + k], Mp[k], gn);
@@ -447,8 +455,8 @@ __device__ __forceinline__ bool face_vjp(const float* o, const float* x, const f
 // element is alike) and one frame and surface J per face (fgeom).  WALLS: the mesh has
 // boundary faces (without, the boundary code is compiled out).  TP: the two-point flux.
 template <int ND, int N, int TP, bool CURV, bool WALLS>
-__global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
-    using C = VCfg<ND, N, TP>;
+__global__ void __launch_bounds__((VCfg<ND, N, TP, CURV>::TPB), (N <= 7 ? 3 : 2)) k_fvjp(const FastVjpArgs A) {
+    using C = VCfg<ND, N, TP, CURV>;
     constexpr int NV = C::NV, NFC = C::NFC, NPE = C::NPE, NPF = C::NPF, NL = C::NL, EPB = C::EPB;
     constexpr int TPB = C::TPB, NN = C::NN, NNP = C::NNP, NPT = C::NPT;
     const FastArgs& a = A.f;
@@ -614,9 +622,9 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
 #pragma unroll
         for (int rr = 0; rr < NR; ++rr)
             sR[rr * NNP + pj] = make_float4(pw[4 * rr], pw[4 * rr + 1], pw[4 * rr + 2], pw[4 * rr + 3]);
-        // metric: column d of the node's matrix is Ja^d; stored halved per direction
+        // metric (curved): column d of the node's matrix is Ja^d; stored halved per direction
 #pragma unroll
-        for (int d = 0; d < ND; ++d) {
+        for (int d = 0; d < ND && CURV; ++d) {
             if constexpr (ND == 3)
                 reinterpret_cast<float4*>(sM + d * MW * NNP)[pj] =
                     make_float4(0.5f * g[r][ND * d], 0.5f * g[r][ND * d + 1], 0.5f * g[r][ND * d + 2], 0.0f);
@@ -710,6 +718,15 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
 #pragma unroll
             for (int v = 0; v < NA; ++v) acc[p][v] = 0.0f;
         const float* const sMd = sM + ldir * MW * NNP;
+        // affine elements: the line's metric vector Ja^d/2 = (Ja_dd/2) e_d at every node
+        // (the values the curved form would stage: Ja_dd = (Ja_dd/J)·J)
+        float Ml[ND];
+        if constexpr (!CURV) {
+            const float* eg = a.egeom + (int64_t)(e0 + (lle < nel ? lle : 0)) * a.egeom_stride;
+            const float J = rcp(eg[3]);
+#pragma unroll
+            for (int k = 0; k < ND; ++k) Ml[k] = k == ldir ? 0.5f * (eg[k] * J) : 0.0f;
+        }
         if constexpr (TP == TP_CHANDRASEKHAR) {
             float amax = 0.0f, amin = 0.0f, bmax = 0.0f, bmin = 0.0f;
 #pragma unroll
@@ -727,15 +744,24 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
             const int path = wave_path(!line_ok || (qa < 1e-4f * pa && qb < 1e-4f * pb),
                                        !line_ok || (qa < 1e-2f * pa && qb < 1e-2f * pb));
             if (path == 2)
-                line_vjp<ND, N, NNP, TP, 2, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+                line_vjp<ND, N, NNP, TP, 2, false, CURV>(sR, sMd, Ml, base, stride, A.iw, A.wsh, ig1, gm1, acc);
             else if (path == 1)
-                line_vjp<ND, N, NNP, TP, 1, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+                line_vjp<ND, N, NNP, TP, 1, false, CURV>(sR, sMd, Ml, base, stride, A.iw, A.wsh, ig1, gm1, acc);
             else
-                line_vjp<ND, N, NNP, TP, 0, false>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+                line_vjp<ND, N, NNP, TP, 0, false, CURV>(sR, sMd, Ml, base, stride, A.iw, A.wsh, ig1, gm1, acc);
         } else {
-            line_vjp<ND, N, NNP, TP, 0, TP == TP_STDAVG && !CURV>(sR, sMd, base, stride, A.iw, A.wsh, ig1, gm1, acc);
+            line_vjp<ND, N, NNP, TP, 0, TP == TP_STDAVG && !CURV, CURV>(sR, sMd, Ml, base, stride, A.iw, A.wsh, ig1,
+                                                                     gm1, acc);
         }
     }
+    // Pin the rows here (an empty asm that emits nothing): the store below is conditional,
+    // and the compiler would otherwise sink the pure pair arithmetic past the barrier into
+    // it, keeping every partner record loaded before the barrier live across the line phase
+    // (Kennedy-Gruber at 3-D N=7: 440 VGPRs spilled).
+#pragma unroll
+    for (int p = 0; p < N; ++p)
+#pragma unroll
+        for (int c = 0; c < NA; ++c) asm volatile("" : "+v"(acc[p][c]));
 
     // epilogue operands (null pointers read as zero)
     float p0[NPT][NV], p1[NPT][NV], p2[NPT][NV];
@@ -753,13 +779,9 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
         }
     }
     __syncthreads();  // the lines are done: the staging slots are dead
-    {
-        // rows of direction d in slots [d*NA, d*NA + NA); a lane without a line of its own
-        // (past the tasks, or on a missing element of the last block) stores into the unread
-        // slot ND instead.  (A conditional store would let the compiler sink the pair
-        // arithmetic past the barrier into that branch and keep every partner record loaded
-        // before it live across the line phase.)
-        float* R = sm + (line_ok ? ldir : ND) * NA * NNP;
+    if (line_ok) {
+        // rows of direction d in slots [d*NA, d*NA + NA)
+        float* R = sm + ldir * NA * NNP;
 #pragma unroll
         for (int p = 0; p < N; ++p)
 #pragma unroll
@@ -847,7 +869,7 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP>::TPB), (N <= 7 ? 3 : 2)) k_fv
 
 template <int ND, int N, int TP, bool CURV, bool WALLS>
 hipError_t launch_fvjp(const FastVjpArgs& a, hipStream_t s) {
-    using C = VCfg<ND, N, TP>;
+    using C = VCfg<ND, N, TP, CURV>;
     const int nb = (a.f.ne + C::EPB - 1) / C::EPB;
     hipLaunchKernelGGL((k_fvjp<ND, N, TP, CURV, WALLS>), dim3(nb), dim3(C::TPB), 0, s, a);
     return hipGetLastError();
