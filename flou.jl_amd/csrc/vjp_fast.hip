@@ -8,13 +8,14 @@
 // the forward fast kernel k_fastc (rhs_fast.hip):
 //   * EPB elements per block, one thread per (element, direction, line);
 //   * staging: per node Q, Y, the metric and J from HBM; to LDS one record of 16-byte
-//     vectors per node with the flux's node inputs and w = c·Y/J, and the metric vector
-//     Ja^d/2 of each direction (a partner node costs four 128-bit reads, and eight
-//     consecutive lanes fall on distinct banks for lines of every direction); node
-//     inputs: Chandrasekhar (a, v.., β) = (ρ/2, u/2, ρ/(2p)), Kennedy-Gruber
-//     (a, v.., s, e) = (ρ/2, u/2, p/2, E/(2ρ)), StdAverage (u.., H, |u|²) with
-//     H = (E + p)/ρ; each line thread also loads the own and the neighbour's trace and
-//     the neighbour's Y/J at its two line ends;
+//     vectors per node with the flux's node inputs and w = c·Y/J, and on curved elements
+//     the metric vector Ja^d/2 of each direction (a partner node costs three or four
+//     128-bit reads; node arrays padded for even N and lanes direction-major, so a
+//     wave's lanes read distinct banks); node inputs: Chandrasekhar (a, v.., β) =
+//     (ρ/2, u/2, ρ/(2p)), Kennedy-Gruber (a, v.., s, e) = (ρ/2, u/2, p/2, E/(2ρ)),
+//     StdAverage (u.., H, |u|²) with H = (E + p)/ρ; affine elements take the line's
+//     constant metric vector from the element geometry; each line thread also loads the
+//     own and the neighbour's trace and the neighbour's Y/J at its two line ends;
 //   * line phase: the reverse product of the face flux at both line ends with respect to
 //     the own trace (conservative variables, into a face array in LDS), then every pair
 //     of the line once.  Off the (zero) diagonal of D♯ on GLL nodes S_ij = ω_i D♯_ij is
