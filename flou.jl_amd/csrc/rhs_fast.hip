@@ -352,12 +352,18 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     // a valid line (their results are dropped), so the whole wave runs one stream.
     // Direction-major: consecutive lanes run the lines of one direction over the block's
     // elements, so most waves read LDS with one stride (C3: -2 %, C2: -5 % per stage
-    // against element-major lanes, whose waves mixed two directions' strides).
-    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
+    // against element-major lanes, whose waves mixed two directions' strides).  When one
+    // direction's lines fit in a wave and the block has one wave per direction (3-D N=7,
+    // 2-D N=7), wave d runs direction d and its spare lanes repeat lines of it (C5: -2.5 %).
+    constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
+    static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
+    const int wk = tid & 63, wd = tid >> 6;
+    const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
+    const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NL);
     const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_in = lle < nel;
-    const bool line_ok = tid < EPB * NL && line_in;
+    const bool line_ok = task && line_in;
 
     // face tables of both line ends (faces 2*ldir and 2*ldir+1), first: they feed the
     // exterior addresses
@@ -856,12 +862,18 @@ __global__ void __launch_bounds__((CCfg<ND, N>::TPB)) k_fastc(const FastArgs a) 
     // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
     // Direction-major: consecutive lanes run the lines of one direction over the block's
     // elements, so most waves read LDS with one stride (C3: -2 %, C2: -5 % per stage
-    // against element-major lanes, whose waves mixed two directions' strides).
-    const int tl = tid < EPB * NL ? tid : tid - EPB * NL;
+    // against element-major lanes, whose waves mixed two directions' strides).  When one
+    // direction's lines fit in a wave and the block has one wave per direction (3-D N=7,
+    // 2-D N=7), wave d runs direction d and its spare lanes repeat lines of it (C5: -2.5 %).
+    constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
+    static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
+    const int wk = tid & 63, wd = tid >> 6;
+    const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
+    const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NL);
     const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_in = lle < nel;
-    const bool line_ok = tid < EPB * NL && line_in;
+    const bool line_ok = task && line_in;
     int xnb[2], xinf[2], xfg[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {

@@ -477,13 +477,19 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP, CURV>::TPB), (N <= 7 ? 3 : 2)
     int err = 0;
 
     // ---- this thread's line task (lanes past the tasks repeat a valid line) ----------
-    // direction-major: consecutive lanes run the lines of one direction over the block's
-    // elements, so a wave reads records with one stride where EPB·NPF allows it; a lane
-    // past the tasks repeats a line of the last direction, like the last live lanes
-    const int tl = tid < EPB * NL ? tid : tid - EPB * NPF;
+    // Direction-major: consecutive lanes run the lines of one direction over the block's
+    // elements, so a wave reads records with one stride where EPB·NPF allows it.  When one
+    // direction's lines fit in a wave and the block has one wave per direction (3-D N=7,
+    // 2-D N=7), wave d runs direction d and its spare lanes repeat lines of it; otherwise
+    // the lanes past the tasks repeat lines of the last direction, like the last live lanes.
+    constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
+    static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
+    const int wk = tid & 63, wd = tid >> 6;
+    const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
+    const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NPF);
     const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
-    const bool line_ok = tid < EPB * NL && lle < nel;
+    const bool line_ok = task && lle < nel;
     int xnb[2], xinf[2], xfg[2];
 #pragma unroll
     for (int sd = 0; sd < 2; ++sd) {
