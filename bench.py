@@ -551,6 +551,48 @@ def secondary_adjoint(name, nsteps=100, every=10):
     return out
 
 
+def c1_steps(nsteps=2000):
+    """C1: 2-D linear advection (1, 0.5) on 16x16 periodic quads over [0,1]^2, N=4 GLL,
+    strong form, upwind LxF, FP64, Gaussian pulse + seeded U(-1e-3, 1e-3) noise, SSPRK33 at
+    a fixed dt (CFL 0.3 of the initial state): one flou_hip_timeintegrate call over nsteps
+    steps (every stage launched from the host), HIP-event timing after a warm-up call.
+    6400 nodes: the stage kernels are launch-bound."""
+    import ctypes
+    import numpy as np
+    import torch
+    import flouhip as F
+    from flouhip import _capi as C
+    b = F.LagrangeBasis("GLL", 5)
+    std = F.StdQuad(b, F.DGSEMrec(b), 1)
+    mesh = F.CartesianMesh(2, (0, 0), (1, 1), (16, 16)).apply_periodicBCs(("1", "2"), ("3", "4"))
+    eq = F.LinearAdvection(1.0, 0.5)
+    disc = F.MultielementDisc(mesh, std, eq, F.StrongDivOperator(F.LxF(F.StdAverage(), 1.0)))
+    X = mesh.node_coords(std)
+    rng = np.random.default_rng(1)
+    Q0 = np.asfortranarray((np.exp(-((X[:, 0] - 0.5) ** 2 + (X[:, 1] - 0.5) ** 2) / 0.01)
+                            + rng.uniform(-1e-3, 1e-3, X.shape[0]))[:, None])
+    dt = F.get_max_dt(disc.to_device(Q0), disc, eq, 0.3)
+    code = F.disc._solver_code(F.SSPRK33)
+    stream = torch.cuda.current_stream()
+    Q = disc.to_device(Q0)
+    disc._sync_stream()
+    t = ctypes.c_double(0.0)
+    C.check(C.lib.flou_hip_timeintegrate(disc.h, code, disc._dptr(Q), dt, 50, 0.0, ctypes.byref(t)), disc.h)
+    Q = disc.to_device(Q0)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    C.check(C.lib.flou_hip_timeintegrate(disc.h, code, disc._dptr(Q), dt, nsteps, 0.0, ctypes.byref(t)), disc.h)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    disc.check()
+    ms = e0.elapsed_time(e1)
+    return {"us_per_step": 1e3 * ms / nsteps, "us_per_stage": 1e3 * ms / (3 * nsteps),
+            "DOF_updates_per_s": disc.ndofs * 3 * nsteps / (ms * 1e-3),
+            "workload": (f"C1 2-D advection 16x16 N=4 FP64 strong/upwind, {disc.ndofs} nodes, {nsteps} SSPRK33 "
+                         "steps at a fixed dt through flou_hip_timeintegrate")}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -600,8 +642,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c2", "c5", "c2adj", "c5adj", "c5vjp", "c3vjp",
-                                       "c2vjp"],
+    ap.add_argument("--only", choices=["fp32", "fp64", "adjoint", "c1", "c2", "c5", "c2adj", "c5adj", "c5vjp",
+                                       "c3vjp", "c2vjp"],
                     default=None,
                     help="time one form only, only the adjoint/forward measurement, or only the "
                          "2-D (c2) or curved 3-D (c5) workload (profiling runs)")
@@ -621,6 +663,9 @@ def main():
     torch.cuda.set_device(0)
     if args.only == "adjoint":  # profiling run of the adjoint workload alone
         print(json.dumps({"adjoint": adjoint_ratio()}), flush=True)
+        return
+    if args.only == "c1":  # the C1 time-step loop alone
+        print(json.dumps({"c1": c1_steps()}), flush=True)
         return
     if args.only in ("c2adj", "c5adj"):  # one secondary adjoint alone
         print(json.dumps({args.only[:2] + "_adjoint": secondary_adjoint(args.only[:2])}), flush=True)
@@ -715,6 +760,9 @@ def main():
                              "speedup_fp64_plain_to_fp32_plain": r64["gpu_ms"] / rp["gpu_ms"],
                              "speedup_fp32_plain_to_fp32_fast": rp["gpu_ms"] / r["gpu_ms"]}
     if not args.no_secondary and not args.only:
+        out["c1"] = c1 = c1_steps()
+        print(f"# c1: {c1['us_per_step']:.1f} us/step ({c1['us_per_stage']:.2f} us/stage)", file=sys.stderr,
+              flush=True)
         for name in ("c2", "c5"):
             out[name] = secondary_workload(name)
             print(f"# {name}: fp32 {out[name]['fp32']['avg_stage_ms']:.3f} ms/stage, "
