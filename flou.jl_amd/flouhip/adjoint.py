@@ -106,18 +106,36 @@ def _step_ptrs(buf, k):
     return (ctypes.c_void_p * S)(*[base + s * sb for s in range(S)])
 
 
+# share of the device memory free at the start of a gradient that keep_segments="auto" may
+# fill with kept stage inputs
+AUTO_KEEP_FRACTION = 0.5
+
+
+def auto_keep_segments(disc, solver, nsteps, checkpoint_every, fraction=AUTO_KEEP_FRACTION):
+    """The number of trailing segments whose stage inputs fit in ``fraction`` of the free
+    device memory (each segment: checkpoint_every x stages states), at least 1."""
+    K = max(1, int(checkpoint_every or nsteps or 1))
+    nseg = -(-nsteps // K)
+    seg_bytes = K * _nstages(_solver_code(solver)) * disc.nvars * disc.ndofs * disc.torch_dtype().itemsize
+    free, _ = torch.cuda.mem_get_info()
+    return max(1, min(nseg, int(fraction * free // seg_bytes)))
+
+
 def forward(disc, solver, Q0, dt, nsteps, checkpoint_every=None, checkpoint_dtype=None, keep_segments=0):
     """Fixed-dt forward run from Q0 (not modified).  Returns (u_T, checkpoints) where
     checkpoints[i] is the state at step i*checkpoint_every, stored as checkpoint_dtype
     (default: the working precision).  The stage inputs of the last ``keep_segments``
     segments stay in device memory (checkpoints.stages; one allocation of
     checkpoint_every x stages states per segment), so that the reverse sweep reverses those
-    segments without recomputing them."""
+    segments without recomputing them; "auto" keeps as many as fit in half of the free
+    device memory (auto_keep_segments), which on a 288 GB device is usually all of them."""
     code = _solver_code(solver)
     K = max(1, int(checkpoint_every or nsteps or 1))
     cdt = checkpoint_dtype or disc.torch_dtype()
     S = _nstages(code)
     nseg = -(-nsteps // K)
+    if keep_segments == "auto":
+        keep_segments = auto_keep_segments(disc, solver, nsteps, K)
     first_kept = max(0, nseg - int(keep_segments))
     cur = [Q0.clone()] + [disc.new_state() for _ in range(S - 1)]
     nxt = disc.new_state()
@@ -183,8 +201,9 @@ def gradient(disc, solver, Q0, dt, nsteps, seed, checkpoint_every=20, checkpoint
 
     seed(u_T) returns dL/du_T (a device state tensor).  Memory: nsteps/checkpoint_every
     checkpoints, the stage inputs of the last keep_segments segments (checkpoint_every x
-    stages states each, kept from the forward run, so those segments are not recomputed),
-    and checkpoint_every x stages states for a segment being recomputed."""
+    stages states each, kept from the forward run, so those segments are not recomputed;
+    "auto": as many as fit in half of the free device memory), and checkpoint_every x
+    stages states for a segment being recomputed."""
     K = max(1, int(checkpoint_every))
     uT, cps = forward(disc, solver, Q0, dt, nsteps, K, checkpoint_dtype, keep_segments)
     ubar = seed(uT).to(disc.torch_dtype()).contiguous().clone()

@@ -233,7 +233,8 @@ def test_adjoint_gradient_matches_central_differences():
 def test_kept_segments_give_the_recomputed_gradient():
     """Reversing segments from the stage inputs the forward run kept gives bitwise the
     gradient of the recomputing sweep (17 steps, checkpoints every 5: a short last
-    segment), for none, the last, two and all four segments kept, SSPRK33 and CK2N54."""
+    segment), for none, the last, two and all four segments kept and for keep_segments
+    "auto" (sized from the free device memory), SSPRK33 and CK2N54."""
     nsteps = 17
     disc, orc, QA, rhoB, wnode, dt = _misfit_problem(8, 4, nsteps)
     wdev = torch.tensor(wnode, device="cuda")
@@ -257,6 +258,15 @@ def test_kept_segments_give_the_recomputed_gradient():
             assert np.array_equal(res[keep][1], res[0][1]), (solver, keep)
         g = disc.to_host(A.gradient(disc, solver, disc.to_device(QA), dt, nsteps, seed, checkpoint_every=5))
         assert np.array_equal(g, res[0][1])
+        # "auto": as many segments as fit in half of the free device memory (here all four)
+        assert A.auto_keep_segments(disc, solver, nsteps, 5) == 4
+        uT, cps = A.forward(disc, solver, disc.to_device(QA), dt, nsteps, 5, keep_segments="auto")
+        assert sorted(cps.stages) == [0, 1, 2, 3]
+        g = disc.to_host(A.gradient(disc, solver, disc.to_device(QA), dt, nsteps, seed, checkpoint_every=5,
+                                    keep_segments="auto"))
+        assert np.array_equal(g, res[0][1])
+        # a budget for one segment keeps one
+        assert A.auto_keep_segments(disc, solver, nsteps, 5, fraction=1e-300) == 1
 
 
 def test_adjoint_full_gradient_matches_dual_numbers():
