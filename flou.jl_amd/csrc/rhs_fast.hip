@@ -357,10 +357,11 @@ __global__ void __launch_bounds__((FCfg<ND, N>::TPB)) k_fast(const FastArgs a) {
     // 2-D N=7), wave d runs direction d and its spare lanes repeat lines of it (C5: -2.5 %).
     constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
     static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
-    const int wk = tid & 63, wd = tid >> 6;
+    // (wave-uniform wave index: the line direction of a WDIR wave lives in a scalar register)
+    const int wk = tid & 63, wd = WDIR ? __builtin_amdgcn_readfirstlane(tid >> 6) : 0;
     const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
     const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NL);
-    const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
+    const int ldir = WDIR ? wd : tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_in = lle < nel;
     const bool line_ok = task && line_in;
@@ -867,10 +868,11 @@ __global__ void __launch_bounds__((CCfg<ND, N>::TPB)) k_fastc(const FastArgs a) 
     // 2-D N=7), wave d runs direction d and its spare lanes repeat lines of it (C5: -2.5 %).
     constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
     static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
-    const int wk = tid & 63, wd = tid >> 6;
+    // (wave-uniform wave index: the line direction of a WDIR wave lives in a scalar register)
+    const int wk = tid & 63, wd = WDIR ? __builtin_amdgcn_readfirstlane(tid >> 6) : 0;
     const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
     const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NL);
-    const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
+    const int ldir = WDIR ? wd : tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_in = lle < nel;
     const bool line_ok = task && line_in;

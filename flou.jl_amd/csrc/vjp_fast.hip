@@ -484,10 +484,11 @@ __global__ void __launch_bounds__((VCfg<ND, N, TP, CURV>::TPB), (N <= 7 ? 3 : 2)
     // the lanes past the tasks repeat lines of the last direction, like the last live lanes.
     constexpr bool WDIR = EPB * NPF <= 64 && TPB == ND * 64;
     static_assert(!WDIR || 2 * EPB * NPF >= 64, "spare lanes repeat lines of their wave's direction");
-    const int wk = tid & 63, wd = tid >> 6;
+    // (wave-uniform wave index: the line direction of a WDIR wave lives in a scalar register)
+    const int wk = tid & 63, wd = WDIR ? __builtin_amdgcn_readfirstlane(tid >> 6) : 0;
     const bool task = WDIR ? wk < EPB * NPF : tid < EPB * NL;
     const int tl = WDIR ? wd * (EPB * NPF) + (task ? wk : wk - EPB * NPF) : (task ? tid : tid - EPB * NPF);
-    const int ldir = tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
+    const int ldir = WDIR ? wd : tl / (EPB * NPF), lr = tl - ldir * (EPB * NPF);
     const int lle = lr / NPF, lf = lr - lle * NPF;
     const bool line_ok = task && lle < nel;
     int xnb[2], xinf[2], xfg[2];
